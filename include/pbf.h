@@ -1,0 +1,169 @@
+/*
+ * pbf.h — C ABI of the MI355X (gfx950) Position Based Fluids step.
+ *
+ * Drop-in for the `Simulator` class of naeioi/PBF-CUDA
+ * (fluids/Simulator.h:7-61, fluids/Simulator.cpp:10-136).  The reference
+ * takes OpenGL buffer names and reads its parameters from a global
+ * singleton; this ABI takes device pointers, an explicit HIP stream and an
+ * explicit parameter block, and reports errors as status codes instead of
+ * exiting the process.
+ *
+ * Conventions
+ *  - Every vector array is either PBF_LAYOUT_FLOAT4 (16 B per particle,
+ *    x,y,z,w; w is not read and is written as 0) or PBF_LAYOUT_FLOAT3
+ *    (12 B packed, the layout of the reference's GL vertex buffers,
+ *    fluids/FluidSystem.cpp:64-84).  The layout is fixed at pbf_create.
+ *  - `stream` is a hipStream_t passed as void* (NULL = the null stream).
+ *    Work is stream-ordered; no call below synchronises the device except
+ *    where its comment says so.
+ *  - All functions return PBF_OK (0) or a PBF_ERR_* code; the message of the
+ *    last failure on a solver is available from pbf_last_error(solver), and
+ *    the last failure of a call without a solver (pbf_create) from
+ *    pbf_last_error(NULL).  Nothing calls exit().
+ *  - A solver is used from one host thread at a time; one solver per stream.
+ */
+#ifndef PBF_H_
+#define PBF_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PBF_ABI_VERSION 1
+
+/* status codes */
+#define PBF_OK 0
+#define PBF_ERR_INVALID_ARGUMENT 1 /* null pointer, n out of range, bad parameter */
+#define PBF_ERR_CAPACITY 2         /* n > capacity, or the box needs more cells than allowed */
+#define PBF_ERR_HIP 3              /* a HIP runtime call failed (message holds hipGetErrorString) */
+#define PBF_ERR_NO_DEVICE 4        /* no HIP device visible */
+#define PBF_ERR_STATE 5            /* call not valid in the solver's current state */
+
+/* caller array layouts */
+#define PBF_LAYOUT_FLOAT3 3
+#define PBF_LAYOUT_FLOAT4 4
+
+typedef struct pbf_vec3 {
+  float x, y, z;
+} pbf_vec3;
+
+/*
+ * Fluid parameters: the solver fields of GUIParams (fluids/GUIParams.h:6-17),
+ * same names and meaning.  Reference defaults are set in
+ * fluids/FluidSystem.cpp:15-25.  Gravity acts along -z
+ * (fluids/Simulator.cu:174).
+ */
+typedef struct pbf_params {
+  int niter;         /* solver iterations per step */
+  float pho0;        /* rest density (mass is implicitly 1) */
+  float g;           /* gravity magnitude, acting along -z */
+  float h;           /* kernel radius == grid cell size */
+  float dt;          /* time step */
+  float lambda_eps;  /* relaxation epsilon in the lambda denominator */
+  float delta_q;     /* s_corr reference distance */
+  float k_corr;      /* s_corr strength */
+  float n_corr;      /* s_corr exponent */
+  float k_boundaryDensity; /* wall density weight (0 = off) */
+  float c_XSPH;      /* XSPH viscosity coefficient */
+} pbf_params;
+
+/* Opaque solver handle.  Replaces `Simulator` (fluids/Simulator.h:7). */
+typedef struct pbf_solver pbf_solver;
+
+/*
+ * Create a solver on the current HIP device.
+ * Replaces Simulator::Simulator(params, ulim, llim) (fluids/Simulator.h:10-26),
+ * which sized its scratch by MAX_PARTICLE_NUM (fluids/helper.h:10) and four
+ * times the initial box cells; here the caller states `capacity` (maximum
+ * particles) and `max_cells` (maximum grid cells over the solver's life;
+ * <= 0 means 4x the cells of the initial box, as the reference).
+ * Allocates device scratch (hipMalloc), so it synchronises.
+ */
+int pbf_create(const pbf_params* params, pbf_vec3 ulim, pbf_vec3 llim,
+               int capacity, long long max_cells, int layout,
+               pbf_solver** out);
+
+/* Free the solver and its scratch.  Replaces Simulator::~Simulator
+ * (fluids/Simulator.h:27-34).  NULL is accepted. */
+void pbf_destroy(pbf_solver* s);
+
+/* Replace the parameters.  Replaces Simulator::loadParams
+ * (fluids/Simulator.cpp:101-115), which read the GUIParams singleton. */
+int pbf_set_params(pbf_solver* s, const pbf_params* params);
+
+/* Read the parameters back.  Replaces Simulator::saveParams
+ * (fluids/Simulator.cpp:117-130). */
+int pbf_get_params(const pbf_solver* s, pbf_params* out);
+
+/* Move the box.  Replaces Simulator::setLim (fluids/Simulator.cpp:132-136).
+ * The grid dims are recomputed on the next step, as the reference does every
+ * step (fluids/Simulator.cu:187-188).  Fails with PBF_ERR_CAPACITY if the
+ * new box needs more than max_cells cells. */
+int pbf_set_lim(pbf_solver* s, pbf_vec3 ulim, pbf_vec3 llim);
+
+/* Read the box back. */
+int pbf_get_lim(const pbf_solver* s, pbf_vec3* ulim, pbf_vec3* llim);
+
+/*
+ * One PBF frame.  Replaces Simulator::step(d_pos, d_npos, d_vel, d_nvel,
+ * d_iid, nparticle) (fluids/Simulator.h:37, fluids/Simulator.cpp:10-99),
+ * with device pointers instead of GL buffer names.
+ *
+ * In:  pos, vel (start-of-frame state), iid (particle identities).
+ * Out: npos, nvel (end-of-frame state).  As in the reference, every array is
+ *      left in the new (cell-sorted, stable) order: pos holds the sorted
+ *      start positions, vel the velocities before XSPH, iid the sorted
+ *      identities.  The caller ping-pongs (pos,vel) <-> (npos,nvel) between
+ *      frames (fluids/FluidSystem.cpp:112-117).
+ * Stream-ordered, does not synchronise.  0 <= n <= capacity.
+ */
+int pbf_step(pbf_solver* s, float* pos, float* npos, float* vel, float* nvel,
+             uint32_t* iid, int n, void* stream);
+
+/*
+ * Density error of a particle set: mean and max over particles of
+ * |rho_i/pho0 - 1|, with rho_i recomputed on `pos` by a fresh grid (self
+ * included, same kernel and wall term as the lambda pass,
+ * fluids/Simulator_kernel.cuh:77-120).  No counterpart in the reference; it
+ * is the multi-frame density statistic.  Writes out[0]=mean, out[1]=max into
+ * host memory, so it synchronises `stream`.  Does not modify `pos`.
+ */
+int pbf_density_error(pbf_solver* s, const float* pos, int n, void* stream,
+                      double out[2]);
+
+/* Per-phase GPU timing with HIP events (replaces the Logger phase timers,
+ * fluids/Logger.cpp:17-82, fluids/Simulator.cpp:44-79, without device-wide
+ * syncs).  enable != 0 starts recording; records are resolved by
+ * pbf_get_timing, which synchronises the recorded events. */
+#define PBF_PHASE_PREDICT 0 /* predict + cell key */
+#define PBF_PHASE_SORT 1    /* radix sort of (key, index) */
+#define PBF_PHASE_GRID 2    /* cell clear + permute + cell ranges */
+#define PBF_PHASE_LAMBDA 3  /* one lambda launch */
+#define PBF_PHASE_DELTA 4   /* one delta-p launch */
+#define PBF_PHASE_VEL 5     /* velocity update */
+#define PBF_PHASE_XSPH 6    /* XSPH viscosity */
+#define PBF_PHASE_STEP 7    /* whole step */
+#define PBF_NUM_PHASES 8
+int pbf_set_timing(pbf_solver* s, int enable);
+/* ms_total[p] = summed GPU ms of phase p, count[p] = number of intervals
+ * since the last call; resets the accumulators. */
+int pbf_get_timing(pbf_solver* s, double ms_total[PBF_NUM_PHASES],
+                   long long count[PBF_NUM_PHASES]);
+
+/* Message of the last failure on `s` (or of the last pbf_create failure on
+ * this thread when s is NULL).  Never NULL. */
+const char* pbf_last_error(const pbf_solver* s);
+
+/* Static name of a status code. */
+const char* pbf_status_string(int code);
+
+/* PBF_ABI_VERSION of the built library. */
+int pbf_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PBF_H_ */

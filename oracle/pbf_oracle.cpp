@@ -1,0 +1,592 @@
+// pbf_oracle.cpp — TEST INFRASTRUCTURE, not product code.
+//
+// Host C++ restatement of the reference PBF step (naeioi/PBF-CUDA).  Only
+// tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load this
+// library; the product path (libpbf.so) never links or calls it.
+//
+// One templated solver, two instantiations:
+//  * T = double — the FP64 oracle: the reference algorithm in FP64.
+//  * T = float  — the FP32 mirror: the same algorithm with a fixed FP32
+//    operation sequence (IEEE +,-,*,/,sqrt and explicit fused multiply-adds,
+//    no other contraction).  The GPU kernels use the same sequence, so their
+//    results must equal the mirror's bit for bit; the FP64 instantiation
+//    bounds how far that sequence is from exact arithmetic.
+// Integer-producing parts (predicted positions feeding the keys, grid dims,
+// keys, query cells) are FP32 in both instantiations, exactly as the GPU.
+// Each function cites the reference lines it restates.
+#include "pbf_oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+// fluids/helper.h:4-9 (M_PI is the reference's 12-digit literal).
+constexpr double kPi = 3.14159265359;
+constexpr double kLimEps = 1e-3;
+constexpr double kKernelEps = 1e-4;
+constexpr double kMaxDp = 0.1;
+
+int nthreads(int t) {
+#ifdef _OPENMP
+  return t > 0 ? t : omp_get_max_threads();
+#else
+  (void)t;
+  return 1;
+#endif
+}
+
+// One axis of getGridxyz / getGridId (fluids/Simulator.cu:30-36, :64-71):
+// (int)((p - llim) / h) clamped to [0, dim-1], all in FP32.  Clamping the
+// FP32 quotient before truncation gives the same integer for every finite
+// quotient and maps NaN to 0.
+inline int cell_coord_f32(float p, float llim, float h, int dim) {
+  const float q = (p - llim) / h;
+  const float c = std::fmin(std::fmax(q, 0.0f), (float)(dim - 1));
+  return (int)c;
+}
+
+inline int cell_coord_f64(double p, double llim, double h, int dim) {
+  const double q = (p - llim) / h;
+  const double c = std::fmin(std::fmax(q, 0.0), (double)(dim - 1));
+  return (int)c;
+}
+
+inline float fma_(float a, float b, float c) { return std::fmaf(a, b, c); }
+inline double fma_(double a, double b, double c) { return std::fma(a, b, c); }
+
+struct Grid {
+  int d[3] = {0, 0, 0};
+  long long cells = 0;
+  std::vector<uint32_t> start, end;
+};
+
+// The 27-cell walk of computeLambda / computetpos / computeXSPH
+// (fluids/Simulator_kernel.cuh:78-104, :153-170, :214-233): dx, then dy, then
+// dz from -1 to +1, cells outside [0, D) skipped, j ascending inside a cell.
+template <class F>
+inline void for_each_candidate(const Grid& g, const int q[3], F&& f) {
+  for (int ox = -1; ox <= 1; ++ox) {
+    const int x = q[0] + ox;
+    if (x < 0 || x >= g.d[0]) continue;
+    for (int oy = -1; oy <= 1; ++oy) {
+      const int y = q[1] + oy;
+      if (y < 0 || y >= g.d[1]) continue;
+      for (int oz = -1; oz <= 1; ++oz) {
+        const int z = q[2] + oz;
+        if (z < 0 || z >= g.d[2]) continue;
+        // xyzToId, fluids/Simulator.cu:52: x-major, z fastest.
+        const long long c = ((long long)x * g.d[1] + y) * g.d[2] + z;
+        for (uint32_t j = g.start[c]; j < g.end[c]; ++j) f(j);
+      }
+    }
+  }
+}
+
+// Step constants.  For T = float they are formed exactly as the GPU host
+// code forms them (FP64 formula rounded once to FP32); for T = double they
+// are the FP64 values.
+template <class T>
+struct Consts {
+  T h, h2, h2_outer, keps, poly6, spiky, spiky_rho0, pho0, lambda_eps, coef_corr, n_corr;
+  int n_corr_int;
+  T k_boundary, wall_c, lo[3], hi[3], ul[3], ll[3], max_dp, dt, inv_dt, c_xsph;
+};
+
+template <class T>
+Consts<T> make_consts(const orc_params* p, const orc_box* box) {
+  Consts<T> c;
+  const double h = (double)p->h;
+  // getPoly6 / getSpikyGrad coefficients (fluids/Simulator.cu:77-83, :94-98).
+  const double poly6 = 315.0 / (64.0 * kPi * std::pow(h, 9));
+  const double spiky = -45.0 / (kPi * std::pow(h, 6));
+  const double dq = (double)p->delta_q;
+  const double wd = h * h - dq * dq;
+  // coef_corr = -k_corr / W(delta_q)^n_corr (fluids/Simulator.cu:235).
+  const double coef_corr = -(double)p->k_corr / std::pow(poly6 * wd * wd * wd, (double)p->n_corr);
+  c.h = (T)p->h;
+  c.h2 = c.h * c.h;
+  c.h2_outer = c.h2 * (T)1.0001;
+  c.keps = (T)kKernelEps;
+  c.poly6 = (T)poly6;
+  c.spiky = (T)spiky;
+  c.spiky_rho0 = (T)(spiky / (double)p->pho0);
+  c.pho0 = (T)p->pho0;
+  c.lambda_eps = (T)p->lambda_eps;
+  c.coef_corr = (T)coef_corr;
+  c.n_corr = (T)p->n_corr;
+  c.n_corr_int = (p->n_corr >= 0.0f && p->n_corr <= 16.0f && std::floor(p->n_corr) == p->n_corr)
+                     ? (int)p->n_corr
+                     : -1;
+  c.k_boundary = (T)p->k_boundaryDensity;
+  c.wall_c = (T)(2.0 * kPi / 3.0);
+  for (int a = 0; a < 3; ++a) {
+    // Box clamp, fluids/Simulator_kernel.cuh:190-192 (bound formed in double).
+    c.lo[a] = (T)((double)box->llim[a] + kLimEps);
+    c.hi[a] = (T)((double)box->ulim[a] - kLimEps);
+    c.ul[a] = (T)box->ulim[a];
+    c.ll[a] = (T)box->llim[a];
+  }
+  c.max_dp = (T)kMaxDp;
+  c.dt = (T)p->dt;
+  c.inv_dt = (T)1 / c.dt;  // h_updateVelocity, fluids/Simulator.cu:129
+  c.c_xsph = (T)p->c_XSPH;
+  return c;
+}
+
+// x^n for the s_corr exponent (powf in computetpos,
+// fluids/Simulator_kernel.cuh:165): binary exponentiation for small integer
+// n, pow otherwise.
+template <class T>
+inline T pow_corr(T w, const Consts<T>& c) {
+  if (c.n_corr_int >= 0) {
+    T r = (T)1, b = w;
+    int e = c.n_corr_int;
+    while (e) {
+      if (e & 1) r = r * b;
+      b = b * b;
+      e >>= 1;
+    }
+    return r;
+  }
+  return std::pow(w, c.n_corr);
+}
+
+// DensityBoundary, fluids/Simulator.cu:139-161.
+template <class T>
+inline T wall_density_at(T d, const Consts<T>& c) {
+  if (d > c.h) return (T)0;
+  if (d <= (T)0) return c.wall_c;
+  return ((c.wall_c * (c.h - d)) * (c.h - d)) * (c.h + d);
+}
+
+template <class T>
+inline T boundary_density(const T* x, const Consts<T>& c) {
+  T s = wall_density_at(c.ul[0] - x[0], c);
+  s = s + wall_density_at(x[0] - c.ll[0], c);
+  s = s + wall_density_at(c.ul[1] - x[1], c);
+  s = s + wall_density_at(x[1] - c.ll[1], c);
+  s = s + wall_density_at(c.ul[2] - x[2], c);
+  s = s + wall_density_at(x[2] - c.ll[2], c);
+  return s;
+}
+
+template <class T>
+inline T dist2(const T* xi, const T* xj, T d[3]) {
+  d[0] = xi[0] - xj[0];
+  d[1] = xi[1] - xj[1];
+  d[2] = xi[2] - xj[2];
+  T r2 = d[0] * d[0];
+  r2 = fma_(d[1], d[1], r2);
+  r2 = fma_(d[2], d[2], r2);
+  return r2;
+}
+
+bool finite_params(const orc_params* p) {
+  const float v[] = {p->pho0, p->g, p->h, p->dt, p->lambda_eps, p->delta_q,
+                     p->k_corr, p->n_corr, p->k_boundaryDensity, p->c_XSPH};
+  for (float f : v)
+    if (!std::isfinite(f)) return false;
+  return p->h > 0.0f && p->dt > 0.0f && p->pho0 > 0.0f && p->niter >= 1 &&
+         p->lambda_eps > 0.0f && p->delta_q >= 0.0f && p->delta_q < p->h;
+}
+
+void build_grid(const std::vector<uint32_t>& skeys, int n, const int dims[3], Grid& g) {
+  g.d[0] = dims[0];
+  g.d[1] = dims[1];
+  g.d[2] = dims[2];
+  g.cells = (long long)dims[0] * dims[1] * dims[2];
+  g.start.assign((size_t)g.cells, 0u);
+  g.end.assign((size_t)g.cells, 0u);
+  orc_cell_ranges(skeys.data(), n, g.cells, g.start.data(), g.end.data());
+}
+
+template <class T>
+void query_cell(const T* x, const orc_box* box, float h, const int dims[3], bool f64, int q[3]) {
+  for (int a = 0; a < 3; ++a) {
+    q[a] = f64 ? cell_coord_f64((double)x[a], (double)box->llim[a], (double)h, dims[a])
+               : cell_coord_f32((float)x[a], box->llim[a], h, dims[a]);
+  }
+}
+
+// Simulator::step, fluids/Simulator.cpp:10-99, with the phases of
+// fluids/Simulator.cu:165-274 and the kernels of
+// fluids/Simulator_kernel.cuh:7-239.
+template <class T>
+int step_impl(const orc_params* p, const orc_box* box, int n, float* pos, float* npos, float* vel,
+              float* nvel, uint32_t* iid, const orc_options* opt, orc_diag* diag) {
+  if (!p || !box || n < 0 || !finite_params(p)) return -1;
+  if (n > 0 && (!pos || !npos || !vel || !nvel || !iid)) return -1;
+  const int stride = opt ? opt->stride : 4;
+  if (stride != 3 && stride != 4) return -1;
+  const bool brute = opt && opt->brute_force;
+  const bool q64 = opt && opt->query_cell_f64;
+  const int TH = nthreads(opt ? opt->threads : 0);
+  int dims[3];
+  if (orc_grid_dims(box, p->h, dims) != 0) return -1;
+  if (n == 0) return 0;
+
+  const float dt = p->dt;
+  const float gvec[3] = {0.0f, 0.0f, -p->g};  // Simulator.cu:174, z is up
+
+  // advect_kernel, fluids/Simulator_kernel.cuh:14-15: vel += dt*g,
+  // npos = pos + dt*vel, as FP32 fused multiply-adds (what the GPU does, so
+  // the cell keys agree bit for bit).
+  std::vector<float> xs((size_t)3 * n);
+  for (int i = 0; i < n; ++i) {
+    for (int a = 0; a < 3; ++a) {
+      const float v1 = std::fmaf(dt, gvec[a], vel[(size_t)i * stride + a]);
+      xs[(size_t)3 * i + a] = std::fmaf(dt, v1, pos[(size_t)i * stride + a]);
+    }
+  }
+
+  // buildGridHash, fluids/Simulator.cu:178-211.
+  std::vector<uint32_t> keys(n), perm(n), skeys(n);
+  orc_cell_keys(xs.data(), 3, n, box, p->h, dims, keys.data());
+  orc_stable_order(keys.data(), n, perm.data());
+  for (int s = 0; s < n; ++s) skeys[s] = keys[perm[s]];
+  Grid g;
+  build_grid(skeys, n, dims, g);
+
+  std::vector<T> x0((size_t)3 * n), x((size_t)3 * n), xn((size_t)3 * n);
+  std::vector<uint32_t> id(n);
+  for (int s = 0; s < n; ++s) {
+    const uint32_t src = perm[s];
+    for (int a = 0; a < 3; ++a) {
+      x0[(size_t)3 * s + a] = (T)pos[(size_t)src * stride + a];
+      x[(size_t)3 * s + a] = (T)xs[(size_t)3 * src + a];
+    }
+    id[s] = iid[src];
+  }
+
+  const Consts<T> c = make_consts<T>(p, box);
+  std::vector<T> lam(n, (T)0), rho(n, (T)0);
+
+  auto candidates = [&](const T* xi, auto&& f) {
+    if (brute) {
+      for (int j = 0; j < n; ++j) f((uint32_t)j);
+    } else {
+      int q[3];
+      query_cell(xi, box, p->h, dims, q64, q);
+      for_each_candidate(g, q, f);
+    }
+  };
+
+  for (int it = 0; it < p->niter; ++it) {
+    // computeLambda, fluids/Simulator_kernel.cuh:52-129: rho sums poly6 over
+    // every candidate, self included (:91); the spiky gradient / pho0 is
+    // summed into grad_i (:93) and, for j != i, its squared norm into
+    // gradj_l2 (:98-100); wall term (:119-120); lambda (:122-123).
+#pragma omp parallel for schedule(dynamic, 256) num_threads(TH)
+    for (int i = 0; i < n; ++i) {
+      const T* xi = &x[(size_t)3 * i];
+      T r = 0, gx = 0, gy = 0, gz = 0, gl2 = 0;
+      candidates(xi, [&](uint32_t j) {
+        T d[3];
+        const T r2 = dist2(xi, &x[(size_t)3 * j], d);
+        if (r2 < c.h2_outer) {
+          if (r2 < c.h2) {
+            const T t = c.h2 - r2;
+            r = r + ((c.poly6 * t) * t) * t;
+          }
+          const T rl = std::sqrt(r2);
+          if (rl >= c.keps && rl < c.h) {
+            const T t = c.h - rl;
+            const T s = ((c.spiky_rho0 * t) * t) / rl;
+            const T ax = s * d[0], ay = s * d[1], az = s * d[2];
+            gx = gx + ax;
+            gy = gy + ay;
+            gz = gz + az;
+            if (j != (uint32_t)i) {
+              gl2 = fma_(ax, ax, gl2);
+              gl2 = fma_(ay, ay, gl2);
+              gl2 = fma_(az, az, gl2);
+            }
+          }
+        }
+      });
+      if (c.k_boundary != (T)0) r = r + c.k_boundary * boundary_density(xi, c);
+      T den = fma_(gx, gx, gl2);
+      den = fma_(gy, gy, den);
+      den = fma_(gz, gz, den);
+      den = den + c.lambda_eps;
+      lam[i] = -(r / c.pho0 - (T)1) / den;
+      rho[i] = r;
+    }
+    // computetpos, fluids/Simulator_kernel.cuh:131-194: sum over j != i of
+    // (lambda_i + lambda_j + s_corr) * spikyGrad (:163-166), / pho0, clamp to
+    // +-MAX_DP (:184), add (:188), clamp into the shrunk box (:190-192).
+    // Jacobi: all reads from x, writes to xn, then x <- xn (thrust::copy_n,
+    // fluids/Simulator.cu:247-248).
+#pragma omp parallel for schedule(dynamic, 256) num_threads(TH)
+    for (int i = 0; i < n; ++i) {
+      const T* xi = &x[(size_t)3 * i];
+      T px = 0, py = 0, pz = 0;
+      candidates(xi, [&](uint32_t j) {
+        if (j == (uint32_t)i) return;
+        T d[3];
+        const T r2 = dist2(xi, &x[(size_t)3 * j], d);
+        if (r2 < c.h2_outer) {
+          const T rl = std::sqrt(r2);
+          if (rl >= c.keps && rl < c.h) {
+            T w = 0;
+            if (r2 < c.h2) {
+              const T t2 = c.h2 - r2;
+              w = ((c.poly6 * t2) * t2) * t2;
+            }
+            const T corr = c.coef_corr * pow_corr(w, c);
+            const T t = c.h - rl;
+            const T gs = ((c.spiky * t) * t) / rl;
+            const T s = ((lam[i] + lam[j]) + corr) * gs;
+            px = fma_(s, d[0], px);
+            py = fma_(s, d[1], py);
+            pz = fma_(s, d[2], pz);
+          }
+        }
+      });
+      if (it == 0 && diag && diag->dp_raw_first) {
+        diag->dp_raw_first[(size_t)3 * i + 0] = (double)px;
+        diag->dp_raw_first[(size_t)3 * i + 1] = (double)py;
+        diag->dp_raw_first[(size_t)3 * i + 2] = (double)pz;
+      }
+      const T dp[3] = {px, py, pz};
+      for (int a = 0; a < 3; ++a) {
+        T v = dp[a] / c.pho0;
+        v = std::fmin(std::fmax(v, -c.max_dp), c.max_dp);
+        T y = xi[a] + v;
+        y = std::fmax(std::fmin(y, c.hi[a]), c.lo[a]);
+        xn[(size_t)3 * i + a] = y;
+      }
+    }
+    x.swap(xn);
+  }
+
+  // updateVelocity, fluids/Simulator.cu:267-274 / :127-137:
+  // v = (npos - pos) * (1/dt).
+  std::vector<T> v((size_t)3 * n), vf((size_t)3 * n);
+  for (size_t k = 0; k < (size_t)3 * n; ++k) v[k] = (x[k] - x0[k]) * c.inv_dt;
+
+  // computeXSPH, fluids/Simulator_kernel.cuh:196-239 (j == i included, rho
+  // of the last lambda pass).  A pair with W = 0 adds exactly 0 to the
+  // reference's sum whenever rho_i + rho_j != 0, so it is skipped.
+#pragma omp parallel for schedule(dynamic, 256) num_threads(TH)
+  for (int i = 0; i < n; ++i) {
+    const T* xi = &x[(size_t)3 * i];
+    const T* vi = &v[(size_t)3 * i];
+    T ax = 0, ay = 0, az = 0;
+    candidates(xi, [&](uint32_t j) {
+      T d[3];
+      const T r2 = dist2(xi, &x[(size_t)3 * j], d);
+      if (r2 < c.h2) {
+        const T t = c.h2 - r2;
+        const T w = ((c.poly6 * t) * t) * t;
+        const T* vj = &v[(size_t)3 * j];
+        const T f = ((T)2 * w) / (rho[i] + rho[j]);
+        ax = fma_(vj[0] - vi[0], f, ax);
+        ay = fma_(vj[1] - vi[1], f, ay);
+        az = fma_(vj[2] - vi[2], f, az);
+      }
+    });
+    vf[(size_t)3 * i + 0] = fma_(c.c_xsph, ax, vi[0]);  // :235
+    vf[(size_t)3 * i + 1] = fma_(c.c_xsph, ay, vi[1]);
+    vf[(size_t)3 * i + 2] = fma_(c.c_xsph, az, vi[2]);
+  }
+
+  // Outputs in sorted order, FP32: the contract of Simulator::step.
+  for (int s = 0; s < n; ++s) {
+    for (int a = 0; a < 3; ++a) {
+      pos[(size_t)s * stride + a] = (float)x0[(size_t)3 * s + a];
+      vel[(size_t)s * stride + a] = (float)v[(size_t)3 * s + a];
+      npos[(size_t)s * stride + a] = (float)x[(size_t)3 * s + a];
+      nvel[(size_t)s * stride + a] = (float)vf[(size_t)3 * s + a];
+    }
+    if (stride == 4) {
+      pos[(size_t)s * 4 + 3] = vel[(size_t)s * 4 + 3] = 0.0f;
+      npos[(size_t)s * 4 + 3] = nvel[(size_t)s * 4 + 3] = 0.0f;
+    }
+    iid[s] = id[s];
+  }
+  if (diag) {
+    for (size_t k = 0; k < (size_t)3 * n; ++k) {
+      if (diag->x_final) diag->x_final[k] = (double)x[k];
+      if (diag->v_final) diag->v_final[k] = (double)vf[k];
+      if (diag->v_pre) diag->v_pre[k] = (double)v[k];
+    }
+    for (int s = 0; s < n; ++s) {
+      if (diag->lambda) diag->lambda[s] = (double)lam[s];
+      if (diag->rho) diag->rho[s] = (double)rho[s];
+    }
+    if (diag->perm) std::memcpy(diag->perm, perm.data(), sizeof(uint32_t) * n);
+    if (diag->keys) std::memcpy(diag->keys, skeys.data(), sizeof(uint32_t) * n);
+  }
+  return 0;
+}
+
+// Density error on a fresh grid: rho as in computeLambda
+// (fluids/Simulator_kernel.cuh:77-120), self included, plus the wall term;
+// per-particle rho in T, the error statistic in FP64.
+template <class T>
+int density_impl(const orc_params* p, const orc_box* box, int n, const float* xin, int stride,
+                 int threads, double out[2], double* err_out) {
+  if (!p || !box || !out || n < 0 || !finite_params(p) || (stride != 3 && stride != 4)) return -1;
+  out[0] = out[1] = 0.0;
+  int dims[3];
+  if (orc_grid_dims(box, p->h, dims) != 0) return -1;
+  if (n == 0) return 0;
+  std::vector<uint32_t> keys(n), perm(n), skeys(n);
+  orc_cell_keys(xin, stride, n, box, p->h, dims, keys.data());
+  orc_stable_order(keys.data(), n, perm.data());
+  std::vector<T> x((size_t)3 * n);
+  for (int s = 0; s < n; ++s) {
+    skeys[s] = keys[perm[s]];
+    for (int a = 0; a < 3; ++a) x[(size_t)3 * s + a] = (T)xin[(size_t)perm[s] * stride + a];
+  }
+  Grid g;
+  build_grid(skeys, n, dims, g);
+  const Consts<T> c = make_consts<T>(p, box);
+  std::vector<double> err(n);
+  const int TH = nthreads(threads);
+#pragma omp parallel for schedule(dynamic, 256) num_threads(TH)
+  for (int i = 0; i < n; ++i) {
+    const T* xi = &x[(size_t)3 * i];
+    int q[3];
+    query_cell(xi, box, p->h, dims, false, q);
+    T r = 0;
+    for_each_candidate(g, q, [&](uint32_t j) {
+      T d[3];
+      const T r2 = dist2(xi, &x[(size_t)3 * j], d);
+      if (r2 < c.h2) {
+        const T t = c.h2 - r2;
+        r = r + ((c.poly6 * t) * t) * t;
+      }
+    });
+    if (c.k_boundary != (T)0) r = r + c.k_boundary * boundary_density(xi, c);
+    err[i] = std::fabs((double)r / (double)c.pho0 - 1.0);
+  }
+  double sum = 0.0, mx = 0.0;
+  for (int i = 0; i < n; ++i) {
+    sum += err[i];
+    mx = std::max(mx, err[i]);
+  }
+  out[0] = sum / n;
+  out[1] = mx;
+  if (err_out) {
+    for (int s = 0; s < n; ++s) err_out[perm[s]] = err[s];
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Simulator::buildGridHash grid dims, fluids/Simulator.cu:187-188:
+// (int)ceilf(diff / h) with diff = ulim - llim, all FP32.
+int orc_grid_dims(const orc_box* box, float h, int dims[3]) {
+  if (!box || !dims || !(h > 0.0f)) return -1;
+  for (int a = 0; a < 3; ++a) {
+    const float diff = box->ulim[a] - box->llim[a];
+    const float q = std::ceil(diff / h);
+    if (!(diff > 0.0f) || !(q >= 1.0f) || q > 1073741824.0f) return -1;
+    dims[a] = (int)q;
+  }
+  return 0;
+}
+
+// getGridId, fluids/Simulator.cu:56-73.
+void orc_cell_keys(const float* x, int stride, int n, const orc_box* box, float h,
+                   const int dims[3], uint32_t* keys) {
+  for (int i = 0; i < n; ++i) {
+    const float* p = x + (size_t)i * stride;
+    const int cx = cell_coord_f32(p[0], box->llim[0], h, dims[0]);
+    const int cy = cell_coord_f32(p[1], box->llim[1], h, dims[1]);
+    const int cz = cell_coord_f32(p[2], box->llim[2], h, dims[2]);
+    keys[i] = (uint32_t)(((long long)cx * dims[1] + cy) * dims[2] + cz);
+  }
+}
+
+// thrust::sort_by_key on uint keys (fluids/Simulator.cu:196-198) is a stable
+// radix sort; std::stable_sort defines the same order.
+void orc_stable_order(const uint32_t* keys, int n, uint32_t* perm) {
+  std::iota(perm, perm + n, 0u);
+  std::stable_sort(perm, perm + n, [keys](uint32_t a, uint32_t b) { return keys[a] < keys[b]; });
+}
+
+// cudaMemset + computeGridRange (fluids/Simulator.cu:201-204,
+// fluids/Simulator_kernel.cuh:21-50), as a linear scan.
+void orc_cell_ranges(const uint32_t* sk, int n, long long cells, uint32_t* start, uint32_t* end) {
+  std::fill(start, start + cells, 0u);
+  std::fill(end, end + cells, 0u);
+  for (int i = 0; i < n; ++i) {
+    if (i == 0 || sk[i] != sk[i - 1]) {
+      start[sk[i]] = (uint32_t)i;
+      if (i > 0) end[sk[i - 1]] = (uint32_t)i;
+    }
+    if (i == n - 1) end[sk[i]] = (uint32_t)n;
+  }
+}
+
+// getPoly6, fluids/Simulator.cu:75-90, in FP64: 315/(64 pi h^9)(h^2-r^2)^3.
+double orc_poly6(double h, double r2) {
+  const double h2 = h * h;
+  if (r2 >= h2) return 0.0;
+  const double d = h2 - r2;
+  return 315.0 / (64.0 * kPi * std::pow(h, 9)) * d * d * d;
+}
+
+// getSpikyGrad, fluids/Simulator.cu:92-107, in FP64:
+// -45/(pi h^6) (h-|r|)^2 r/|r| for KERNAL_EPS <= |r| < h, else 0.
+void orc_spiky_grad(double h, const double r[3], double grad[3]) {
+  const double rl = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+  if (rl >= h || rl < kKernelEps) {
+    grad[0] = grad[1] = grad[2] = 0.0;
+    return;
+  }
+  const double d = h - rl;
+  const double s = -45.0 / (kPi * std::pow(h, 6)) * d * d / rl;
+  grad[0] = s * r[0];
+  grad[1] = s * r[1];
+  grad[2] = s * r[2];
+}
+
+// DensityBoundary, fluids/Simulator.cu:139-161, in FP64.
+double orc_boundary_density(const orc_box* box, double h, const double p[3]) {
+  Consts<double> c{};
+  c.h = h;
+  c.wall_c = 2.0 * kPi / 3.0;
+  for (int a = 0; a < 3; ++a) {
+    c.ul[a] = box->ulim[a];
+    c.ll[a] = box->llim[a];
+  }
+  return boundary_density(p, c);
+}
+
+int orc_step(const orc_params* p, const orc_box* box, int n, float* pos, float* npos, float* vel,
+             float* nvel, uint32_t* iid, const orc_options* opt, orc_diag* diag) {
+  return step_impl<double>(p, box, n, pos, npos, vel, nvel, iid, opt, diag);
+}
+
+int orc_step_f32(const orc_params* p, const orc_box* box, int n, float* pos, float* npos,
+                 float* vel, float* nvel, uint32_t* iid, const orc_options* opt, orc_diag* diag) {
+  return step_impl<float>(p, box, n, pos, npos, vel, nvel, iid, opt, diag);
+}
+
+int orc_density_error(const orc_params* p, const orc_box* box, int n, const float* x, int stride,
+                      int threads, double out[2]) {
+  return density_impl<double>(p, box, n, x, stride, threads, out, nullptr);
+}
+
+int orc_density_error_f32(const orc_params* p, const orc_box* box, int n, const float* x,
+                          int stride, int threads, double out[2], double* err_out) {
+  return density_impl<float>(p, box, n, x, stride, threads, out, err_out);
+}
+
+int orc_max_threads(void) { return nthreads(0); }
+
+}  // extern "C"

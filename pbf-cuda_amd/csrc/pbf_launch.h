@@ -1,0 +1,46 @@
+// pbf_launch.h — host-side launchers of the gfx950 kernels (stream-ordered,
+// no synchronisation).  Every launcher assumes its arguments were validated
+// by the caller (api.cpp): n > 0, buffers sized for n and for the cells of c.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pbf_device.h"
+
+namespace pbf {
+
+// sort.hip
+int radix_tiles(int n);
+// Sorts (k0,v0) by the low key_bits bits of k0, stably; (k1,v1) is scratch.
+// Returns 0 if the result is in (k0,v0), 1 if in (k1,v1).  hist holds
+// 256*radix_tiles(n) words, totals 256.
+int radix_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, int n, int key_bits,
+                     uint32_t* hist, uint32_t* totals, hipStream_t st);
+
+// step.hip
+void launch_predict_key(int layout, const float* pos, const float* vel, int n, const StepConsts& c,
+                        float4* xp, uint32_t* key, uint32_t* val, hipStream_t st);
+void launch_key_only(int layout, const float* pos, int n, const StepConsts& c, float4* xp,
+                     uint32_t* key, uint32_t* val, hipStream_t st);
+void launch_clear_ranges(uint2* ranges, long long cells, hipStream_t st);
+void launch_permute(int layout, const uint32_t* skey, const uint32_t* sval, const float* pos,
+                    const float4* xp, const uint32_t* iid, int n, int cells, float4* x0s,
+                    float4* xa, uint32_t* iid_s, uint2* ranges, hipStream_t st);
+void launch_permute_pos(const uint32_t* skey, const uint32_t* sval, const float4* xp, int n,
+                        int cells, float4* xa, uint2* ranges, hipStream_t st);
+void launch_lambda(const float4* xa, const uint2* ranges, int n, const StepConsts& c, float4* xl,
+                   float* rho, hipStream_t st);
+void launch_delta(const float4* xl, const uint2* ranges, int n, const StepConsts& c, float4* xa,
+                  hipStream_t st);
+void launch_velocity(int layout, const float4* xa, const float4* x0s, const uint32_t* iid_s,
+                     const float* rho, int n, const StepConsts& c, float4* vr, float* pos,
+                     float* vel, uint32_t* iid, hipStream_t st);
+void launch_xsph(int layout, const float4* xa, const float4* vr, const uint2* ranges, int n,
+                 const StepConsts& c, float* npos, float* nvel, hipStream_t st);
+// partial holds 2*density_blocks(n) doubles; out (device) 2 doubles.
+int density_blocks(int n);
+void launch_density_error(const float4* xa, const uint2* ranges, int n, const StepConsts& c,
+                          double* partial, double* out, hipStream_t st);
+
+}  // namespace pbf

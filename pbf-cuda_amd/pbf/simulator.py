@@ -1,0 +1,164 @@
+"""Host-side mirror of the reference's Simulator (fluids/Simulator.h:7-61).
+
+Same method names and argument meaning as the reference class, over the C ABI
+of include/pbf.h.  Differences: buffers are device pointers (torch tensors or
+integer addresses) instead of GL buffer names, parameters are passed
+explicitly instead of read from the GUIParams singleton, and failures raise
+PBFError instead of exiting the process.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Optional, Sequence
+
+from . import _lib
+from ._lib import PBFError, PbfParams, PbfVec3
+from .params import FluidParams
+
+
+def _vec3(v: Sequence[float]) -> PbfVec3:
+    return PbfVec3(float(v[0]), float(v[1]), float(v[2]))
+
+
+def _ptr(buf, layout: int, what: str, n: int, is_vec: bool) -> int:
+    """Device address of a torch tensor (checked) or a raw integer address."""
+    if isinstance(buf, int):
+        return buf
+    import torch
+
+    if not isinstance(buf, torch.Tensor):
+        raise TypeError(f"{what}: expected a torch tensor or a device address, got {type(buf)}")
+    if not buf.is_cuda:
+        raise ValueError(f"{what}: tensor must be on a HIP device")
+    if not buf.is_contiguous():
+        raise ValueError(f"{what}: tensor must be contiguous")
+    if is_vec:
+        if buf.dtype != torch.float32:
+            raise ValueError(f"{what}: dtype must be float32")
+        if buf.numel() < n * layout:
+            raise ValueError(f"{what}: holds {buf.numel()} floats, need {n * layout}")
+    else:
+        if buf.dtype not in (torch.int32, torch.uint32):
+            raise ValueError(f"{what}: dtype must be int32/uint32")
+        if buf.numel() < n:
+            raise ValueError(f"{what}: holds {buf.numel()} ids, need {n}")
+    return buf.data_ptr()
+
+
+def _stream_handle(stream) -> Optional[int]:
+    if stream is None:
+        import torch
+
+        return torch.cuda.current_stream().cuda_stream
+    if isinstance(stream, int):
+        return stream
+    return stream.cuda_stream
+
+
+class Simulator:
+    """Simulator(params, ulim, llim) (fluids/Simulator.h:10).
+
+    capacity: maximum particle count (the reference's MAX_PARTICLE_NUM,
+    fluids/helper.h:10, lifted to a per-instance value).  max_cells: maximum
+    grid cells over the solver's life (<= 0: four times the initial box's,
+    as fluids/Simulator.h:13-14).  layout: 4 for float4 vector arrays, 3 for
+    packed float3 (the reference's GL buffers).
+    """
+
+    def __init__(self, params: FluidParams, ulim: Sequence[float], llim: Sequence[float],
+                 capacity: int, max_cells: int = 0, layout: int = _lib.PBF_LAYOUT_FLOAT4):
+        self._lib = _lib.load()
+        self._h = ctypes.c_void_p()
+        self.layout = int(layout)
+        self.capacity = int(capacity)
+        p = params.to_c()
+        rc = self._lib.pbf_create(ctypes.byref(p), _vec3(ulim), _vec3(llim), self.capacity,
+                                  int(max_cells), self.layout, ctypes.byref(self._h))
+        _lib.check(rc, None)
+
+    # -- lifetime ---------------------------------------------------------
+    def close(self) -> None:
+        """~Simulator (fluids/Simulator.h:27-34)."""
+        if self._h:
+            self._lib.pbf_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _handle(self):
+        if not self._h:
+            raise PBFError(_lib.PBF_ERR_STATE, "simulator is closed")
+        return self._h
+
+    # -- reference API ----------------------------------------------------
+    def step(self, pos, npos, vel, nvel, iid, nparticle: int, stream=None) -> None:
+        """Simulator::step (fluids/Simulator.h:37, fluids/Simulator.cpp:10-99).
+
+        Reads pos/vel/iid, writes npos/nvel; leaves every array in the new
+        cell-sorted order (pos = sorted start positions, vel = velocities
+        before XSPH).  Stream-ordered on `stream` (default: torch's current
+        stream); does not synchronise."""
+        n = int(nparticle)
+        L = self.layout
+        args = (_ptr(pos, L, "pos", n, True), _ptr(npos, L, "npos", n, True),
+                _ptr(vel, L, "vel", n, True), _ptr(nvel, L, "nvel", n, True),
+                _ptr(iid, L, "iid", n, False))
+        h = self._handle()
+        _lib.check(self._lib.pbf_step(h, *args, n, _stream_handle(stream)), h)
+
+    def loadParams(self, params: FluidParams) -> None:
+        """Simulator::loadParams (fluids/Simulator.cpp:101-115), with the
+        parameters passed in instead of read from a singleton."""
+        p = params.to_c()
+        h = self._handle()
+        _lib.check(self._lib.pbf_set_params(h, ctypes.byref(p)), h)
+
+    def saveParams(self) -> FluidParams:
+        """Simulator::saveParams (fluids/Simulator.cpp:117-130): returns them."""
+        p = PbfParams()
+        h = self._handle()
+        _lib.check(self._lib.pbf_get_params(h, ctypes.byref(p)), h)
+        return FluidParams.from_c(p)
+
+    def setLim(self, ulim: Sequence[float], llim: Sequence[float]) -> None:
+        """Simulator::setLim (fluids/Simulator.cpp:132-136)."""
+        h = self._handle()
+        _lib.check(self._lib.pbf_set_lim(h, _vec3(ulim), _vec3(llim)), h)
+
+    def getLim(self):
+        u, l = PbfVec3(), PbfVec3()
+        h = self._handle()
+        _lib.check(self._lib.pbf_get_lim(h, ctypes.byref(u), ctypes.byref(l)), h)
+        return (u.x, u.y, u.z), (l.x, l.y, l.z)
+
+    # -- additions --------------------------------------------------------
+    def density_error(self, pos, nparticle: int, stream=None):
+        """(mean, max) of |rho_i/pho0 - 1| on a fresh grid over pos; synchronises."""
+        n = int(nparticle)
+        out = (ctypes.c_double * 2)()
+        h = self._handle()
+        _lib.check(self._lib.pbf_density_error(h, _ptr(pos, self.layout, "pos", n, True), n,
+                                               _stream_handle(stream), out), h)
+        return float(out[0]), float(out[1])
+
+    def set_timing(self, enable: bool) -> None:
+        h = self._handle()
+        _lib.check(self._lib.pbf_set_timing(h, 1 if enable else 0), h)
+
+    def get_timing(self):
+        """{phase: (total_ms, count)} since the last call (synchronises)."""
+        ms = (ctypes.c_double * _lib.PBF_NUM_PHASES)()
+        cnt = (ctypes.c_longlong * _lib.PBF_NUM_PHASES)()
+        h = self._handle()
+        _lib.check(self._lib.pbf_get_timing(h, ms, cnt), h)
+        return {name: (float(ms[k]), int(cnt[k])) for k, name in enumerate(_lib.PHASES)}

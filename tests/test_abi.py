@@ -1,0 +1,113 @@
+"""C ABI of libpbf.so on the CPU: the library loads, exports every symbol
+include/pbf.h declares, and rejects bad arguments with status codes and
+messages before touching a device (no compute calls here)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from conftest import ROOT
+
+
+def _header_symbols():
+    text = open(os.path.join(ROOT, "include", "pbf.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(pbf_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol(pbf_mod):
+    from pbf import _lib
+
+    lib = _lib.load()
+    declared = _header_symbols()
+    assert len(declared) >= 12
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert set(declared) == set(_lib.SYMBOLS), "ctypes prototypes and header disagree"
+    assert lib.pbf_abi_version() == 1
+
+
+def test_status_strings(pbf_mod):
+    from pbf import _lib
+
+    lib = _lib.load()
+    assert lib.pbf_status_string(0) == b"PBF_OK"
+    assert lib.pbf_status_string(2) == b"PBF_ERR_CAPACITY"
+    assert lib.pbf_status_string(99) == b"PBF_ERR_UNKNOWN"
+
+
+def _create(lib, params, ulim=(4, 2, 2), llim=(0, 0, 0), cap=100, max_cells=0, layout=4):
+    from pbf._lib import PbfVec3
+
+    h = ctypes.c_void_p()
+    rc = lib.pbf_create(ctypes.byref(params.to_c()), PbfVec3(*ulim), PbfVec3(*llim), cap,
+                        max_cells, layout, ctypes.byref(h))
+    return rc, h
+
+
+@pytest.mark.parametrize("field,value,msg", [
+    ("h", 0.0, "h must be > 0"),
+    ("dt", -1.0, "dt must be > 0"),
+    ("pho0", float("nan"), "non-finite"),
+    ("niter", 0, "niter"),
+    ("lambda_eps", 0.0, "lambda_eps"),
+    ("delta_q", 0.2, "delta_q"),
+])
+def test_create_rejects_bad_params(pbf_mod, field, value, msg):
+    from pbf import _lib
+
+    lib = _lib.load()
+    p = pbf_mod.dam_params().replace(**{field: value})
+    rc, h = _create(lib, p)
+    assert rc == _lib.PBF_ERR_INVALID_ARGUMENT and not h
+    assert msg in lib.pbf_last_error(None).decode()
+
+
+def test_create_rejects_bad_box_capacity_layout(pbf_mod):
+    from pbf import _lib
+
+    lib = _lib.load()
+    p = pbf_mod.dam_params()
+    assert _create(lib, p, ulim=(0, 2, 2))[0] == _lib.PBF_ERR_INVALID_ARGUMENT
+    assert _create(lib, p, cap=0)[0] == _lib.PBF_ERR_INVALID_ARGUMENT
+    assert _create(lib, p, layout=5)[0] == _lib.PBF_ERR_INVALID_ARGUMENT
+    assert _create(lib, p, max_cells=10)[0] == _lib.PBF_ERR_CAPACITY
+    assert _create(lib, p, ulim=(1e9, 1e9, 1e9))[0] == _lib.PBF_ERR_CAPACITY
+    rc = lib.pbf_create(ctypes.byref(p.to_c()), _lib.PbfVec3(1, 1, 1), _lib.PbfVec3(0, 0, 0), 10,
+                        0, 4, None)
+    assert rc == _lib.PBF_ERR_INVALID_ARGUMENT
+
+
+def test_null_solver_calls_fail_cleanly(pbf_mod):
+    from pbf import _lib
+
+    lib = _lib.load()
+    assert lib.pbf_step(None, None, None, None, None, None, 10, None) == _lib.PBF_ERR_INVALID_ARGUMENT
+    assert lib.pbf_set_lim(None, _lib.PbfVec3(1, 1, 1), _lib.PbfVec3(0, 0, 0)) != 0
+    assert lib.pbf_set_params(None, None) != 0
+    assert lib.pbf_set_timing(None, 1) != 0
+    lib.pbf_destroy(None)  # accepted
+    assert b"NULL" in lib.pbf_last_error(None)
+
+
+def test_create_without_device_reports_no_device(pbf_mod):
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a HIP device is visible")
+    with pytest.raises(pbf_mod.PBFError) as e:
+        pbf_mod.Simulator(pbf_mod.dam_params(), (4, 2, 2), (0, 0, 0), capacity=100)
+    assert e.value.code == 4
+
+
+def test_python_wrapper_validates_buffers(pbf_mod):
+    from pbf.simulator import _ptr
+
+    import torch
+
+    with pytest.raises(ValueError):
+        _ptr(torch.zeros(8), 4, "pos", 2, True)  # CPU tensor
+    with pytest.raises(TypeError):
+        _ptr([1, 2], 4, "pos", 2, True)
+    assert _ptr(12345, 4, "pos", 2, True) == 12345

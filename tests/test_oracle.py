@@ -1,0 +1,335 @@
+"""Pins the CPU oracle (oracle/pbf_oracle.cpp) before it is trusted as the
+checker.  The reference ships no tests or golden vectors, so every check is
+an independent fact: closed forms of the SPH kernels (Mueller et al. 2003),
+known answers, brute force, and invariants chosen so that a plausible slip
+(dropped self term, swapped i/j, wrong sign, wrong cell order) fails one of
+them.  CPU only."""
+import math
+
+import numpy as np
+import pytest
+from scipy import integrate
+
+from pbf import params as P
+from pbf import scenes
+
+PI_REF = 3.14159265359  # fluids/helper.h:4
+
+
+def w_poly6(h, r):
+    """Textbook poly6, 315/(64 pi h^9)(h^2-r^2)^3 (Mueller et al. 2003)."""
+    return 315.0 / (64.0 * math.pi * h**9) * (h * h - r * r) ** 3 if r < h else 0.0
+
+
+def w_spiky(h, r):
+    """Textbook spiky kernel, 15/(pi h^6)(h-r)^3 (Mueller et al. 2003)."""
+    return 15.0 / (math.pi * h**6) * (h - r) ** 3 if r < h else 0.0
+
+
+# ---- kernels ---------------------------------------------------------------
+
+def test_poly6_known_answer_and_normalisation(orc):
+    h = 0.1
+    assert orc.poly6(h, 0.0) == pytest.approx(1566.68, abs=0.01)  # 315/(64 pi h^3)
+    assert orc.poly6(h, 0.0) == pytest.approx(315.0 / (64 * PI_REF * h**3), rel=1e-14)
+    assert orc.poly6(h, h * h) == 0.0
+    assert orc.poly6(h, 2 * h * h) == 0.0
+    # the textbook kernel integrates to 1; the oracle's matches it pointwise
+    total, _ = integrate.quad(lambda r: 4 * math.pi * r * r * orc.poly6(h, r * r), 0, h)
+    assert total == pytest.approx(1.0, rel=1e-9)
+    for r in (0.01, 0.037, 0.05, 0.099):
+        assert orc.poly6(h, r * r) == pytest.approx(w_poly6(h, r), rel=1e-9)
+
+
+def test_spiky_gradient_is_derivative_of_spiky_kernel(orc):
+    h = 0.1
+    total, _ = integrate.quad(lambda r: 4 * math.pi * r * r * w_spiky(h, r), 0, h)
+    assert total == pytest.approx(1.0, rel=1e-9)  # the test's own kernel is normalised
+    rng = np.random.default_rng(7)
+    for _ in range(20):
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        r = rng.uniform(0.002, 0.098)
+        g = orc.spiky_grad(h, r * d)
+        e = 1e-7
+        dwdr = (w_spiky(h, r + e) - w_spiky(h, r - e)) / (2 * e)
+        np.testing.assert_allclose(g, dwdr * d, rtol=1e-6, atol=1e-6)
+        assert np.dot(g, d) < 0  # points from i towards j: attraction direction of -grad
+
+
+def test_spiky_gradient_cutoffs(orc):
+    h = 0.1
+    x = np.array([1.0, 0.0, 0.0])
+    assert np.all(orc.spiky_grad(h, 0.5e-4 * x) == 0)  # |r| < KERNAL_EPS (helper.h:7)
+    assert orc.spiky_grad(h, 2e-4 * x)[0] < 0
+    assert np.all(orc.spiky_grad(h, h * x) == 0)       # |r| >= h
+    assert orc.spiky_grad(h, 0.0999 * x)[0] < 0
+
+
+def test_boundary_density_pieces(orc):
+    h = 0.1
+    ul, ll = (10.0, 10.0, 10.0), (0.0, 0.0, 0.0)
+    centre = (5.0, 5.0, 5.0)
+    assert orc.boundary_density(ul, ll, h, centre) == 0.0  # every wall farther than h
+    assert orc.boundary_density(ul, ll, h, (5.0, 5.0, 0.0)) == pytest.approx(2 * PI_REF / 3)
+    assert orc.boundary_density(ul, ll, h, (5.0, 5.0, -1.0)) == pytest.approx(2 * PI_REF / 3)
+    d = 0.05
+    want = (2 * PI_REF / 3) * (h - d) ** 2 * (h + d)
+    assert orc.boundary_density(ul, ll, h, (5.0, 5.0, d)) == pytest.approx(want, rel=1e-12)
+    # two walls near a corner add
+    assert orc.boundary_density(ul, ll, h, (d, 5.0, d)) == pytest.approx(2 * want, rel=1e-12)
+
+
+# ---- grid, keys, order ------------------------------------------------------
+
+def test_grid_dims_match_fp32_ceil(orc):
+    assert orc.grid_dims((4.0, 2.0, 2.0), (0, 0, 0), 0.1) == (40, 20, 20)
+    assert orc.grid_dims((25.6, 12.8, 12.8), (0, 0, 0), 0.1) == (256, 128, 128)
+    rng = np.random.default_rng(3)
+    for _ in range(50):
+        lo = rng.uniform(-3, 3, 3).astype(np.float32)
+        hi = (lo + rng.uniform(0.05, 4, 3)).astype(np.float32)
+        h = np.float32(rng.uniform(0.03, 0.2))
+        want = tuple(int(np.ceil((hi[a] - lo[a]) / h)) for a in range(3))  # FP32 numpy
+        assert orc.grid_dims(hi, lo, h) == want
+
+
+def test_cell_keys_match_numpy_fp32(orc):
+    rng = np.random.default_rng(4)
+    ul, ll, h = (4.0, 2.0, 2.0), (0.0, 0.0, 0.0), np.float32(0.1)
+    x = rng.uniform(-0.5, 4.5, size=(5000, 3)).astype(np.float32)  # includes outside the box
+    dims = orc.grid_dims(ul, ll, h)
+    q = (x - np.array(ll, np.float32)) / h
+    c = np.stack([np.clip(np.trunc(q[:, a]), 0, dims[a] - 1) for a in range(3)], 1).astype(np.int64)
+    want = (c[:, 0] * dims[1] + c[:, 1]) * dims[2] + c[:, 2]
+    np.testing.assert_array_equal(orc.cell_keys(x, ul, ll, h), want.astype(np.uint32))
+
+
+def test_stable_order_and_ranges(orc):
+    rng = np.random.default_rng(5)
+    for keys in (rng.integers(0, 50, 3000), np.zeros(100), np.arange(100)[::-1],
+                 rng.integers(0, 1 << 22, 10000)):
+        keys = keys.astype(np.uint32)
+        perm = orc.stable_order(keys)
+        np.testing.assert_array_equal(perm, np.argsort(keys, kind="stable"))
+        sk = keys[perm]
+        cells = int(sk.max()) + 3
+        start, end = orc.cell_ranges(sk, cells)
+        cs = np.arange(cells)
+        lo = np.searchsorted(sk, cs, "left")
+        hi = np.searchsorted(sk, cs, "right")
+        nonempty = hi > lo
+        np.testing.assert_array_equal(start[nonempty], lo[nonempty])
+        np.testing.assert_array_equal(end[nonempty], hi[nonempty])
+        assert np.all(start[~nonempty] == 0) and np.all(end[~nonempty] == 0)
+
+
+# ---- step-level known answers and invariants --------------------------------
+
+def _lattice(n=12, s=0.05, lo=1.0):
+    x = scenes.lattice((n, n, n), (lo, lo, lo), s, 0.0, seed=1)
+    return x
+
+
+def test_interior_lattice_density_known_answer(orc):
+    """rho of an interior particle of an unjittered 0.5h lattice =
+    W(0)(1 + 6*0.421875 + 12*0.125 + 8*0.015625) = 8078.20 (self included)."""
+    p = P.dam_params(niter=1).replace(g=0.0)
+    x = _lattice()
+    n = x.shape[0]
+    r = orc.step(p, (3.0, 3.0, 3.0), (0, 0, 0), x, np.zeros_like(x), np.arange(n, dtype=np.uint32))
+    xs = r["pos"][:, :3]
+    interior = np.all((xs > 1.0 + 0.05 * 3) & (xs < 1.0 + 0.05 * 9), axis=1)
+    assert interior.sum() > 100
+    want = 315.0 / (64 * PI_REF * 0.1**3) * (1 + 6 * 0.421875 + 12 * 0.125 + 8 * 0.015625)
+    assert want == pytest.approx(8078.20, abs=0.01)
+    # positions are FP32-rounded lattice points: 1e-5 relative; a missing
+    # neighbour shell or self term would be off by >= 1.5%
+    np.testing.assert_allclose(r["rho"][interior], want, rtol=1e-5)
+
+
+def test_symmetric_lattice_interior_has_no_correction(orc):
+    """Inside a symmetric lattice every pair term cancels its mirror image, so
+    the first Delta-p sum is ~0 there, while it is large at the faces."""
+    p = P.dam_params(niter=1).replace(g=0.0)
+    x = _lattice()
+    n = x.shape[0]
+    r = orc.step(p, (3.0, 3.0, 3.0), (0, 0, 0), x, np.zeros_like(x), np.arange(n, dtype=np.uint32))
+    xs = r["pos"][:, :3]
+    dp = np.linalg.norm(r["dp_raw_first"], axis=1)
+    interior = np.all((xs > 1.0 + 0.05 * 3) & (xs < 1.0 + 0.05 * 9), axis=1)
+    assert dp[~interior].max() > 1.0
+    assert dp[interior].max() < 1e-4 * dp[~interior].max()
+
+
+def test_first_pass_correction_is_antisymmetric(orc):
+    """With a fresh grid every neighbour pair is mutual and the pair term is
+    antisymmetric, so the Delta-p sums add to zero over all particles."""
+    sc = scenes.block((12, 10, 12), (2.0, 2.0, 2.0), seed=11)
+    for p in (P.dam_params(niter=1), P.reference_defaults().replace(niter=1)):
+        pos, vel = sc.as_layout(4)
+        r = orc.step(p, sc.ulim, sc.llim, pos, vel, sc.iid)
+        dp = r["dp_raw_first"]
+        scale = np.abs(dp).sum()
+        assert scale > 0
+        assert np.abs(dp.sum(0)).max() < 1e-12 * scale
+
+
+def test_xsph_conserves_momentum_with_mutual_pairs(orc):
+    sc = scenes.block((8, 8, 8), (2.0, 2.0, 2.0), seed=12, vel_sigma=0.5)
+    p = P.reference_defaults().replace(niter=2)
+    pos, vel = sc.as_layout(4)
+    r = orc.step(p, sc.ulim, sc.llim, pos, vel, sc.iid, brute_force=True)
+    np.testing.assert_allclose(r["v_final"].sum(0), r["v_pre"].sum(0), rtol=0, atol=1e-10)
+    assert np.abs(r["v_final"] - r["v_pre"]).max() > 1e-3  # XSPH did act
+
+
+def test_isolated_particle_is_ballistic(orc):
+    """No neighbours: lambda = -(W(0)/pho0 - 1)/eps, no correction, so the
+    particle follows x1 = x0 + dt (v0 + g dt) in the FP32 fused form."""
+    p = P.dam_params()
+    x = np.array([[1.0, 1.0, 1.5, 0.0]], np.float32)
+    v = np.array([[0.3, -0.2, 0.1, 0.0]], np.float32)
+    r = orc.step(p, (4.0, 2.0, 2.0), (0, 0, 0), x, v, np.zeros(1, np.uint32))
+    dt = np.float32(p.dt)
+    w0 = 315.0 / (64 * PI_REF * p.h**3)
+    assert r["lambda"][0] == pytest.approx(-(w0 / p.pho0 - 1) / p.lambda_eps, rel=1e-6)
+    want = [float(x[0, 0]) + float(dt) * float(v[0, 0]), float(x[0, 1]) + float(dt) * float(v[0, 1]),
+            float(x[0, 2]) + float(dt) * (float(v[0, 2]) - float(dt) * p.g)]
+    np.testing.assert_allclose(r["x_final"][0], want, atol=2e-7)
+    np.testing.assert_allclose(r["v_final"][0], (r["x_final"][0] - x[0, :3]) / p.dt, atol=1e-4)
+
+
+def test_two_particles_closed_form_and_repulsion(orc):
+    """Two particles 0.06 apart with pho0 below their density: lambda, the
+    correction and the new separation follow the PBF formulas (Macklin and
+    Mueller 2013) with the textbook kernels; they move apart."""
+    h, rsep = 0.1, 0.06
+    p = P.FluidParams(niter=1, pho0=1000.0, g=0.0, h=h, dt=1 / 60, lambda_eps=600.0,
+                      delta_q=0.02, k_corr=0.1, n_corr=4.0, k_boundaryDensity=0.0, c_XSPH=0.0)
+    x = np.array([[1.0, 1.0, 1.0], [1.0 + rsep, 1.0, 1.0]], np.float32)
+    r = orc.step(p, (2.0, 2.0, 2.0), (0, 0, 0), x, np.zeros_like(x), np.arange(2, dtype=np.uint32))
+    rs = float(np.float32(1.0 + rsep)) - 1.0
+    f32 = lambda v: float(np.float32(v))  # the step reads FP32 parameters
+    h, kc, dq = f32(h), f32(p.k_corr), f32(p.delta_q)
+    rho = w_poly6(h, 0.0) + w_poly6(h, rs)
+    dwdr = -45.0 / (math.pi * h**6) * (h - rs) ** 2  # spiky gradient magnitude
+    g = abs(dwdr) / p.pho0
+    lam = -(rho / p.pho0 - 1) / (2 * g * g + p.lambda_eps)
+    np.testing.assert_allclose(r["rho"], rho, rtol=1e-9)
+    np.testing.assert_allclose(r["lambda"], lam, rtol=1e-9)
+    corr = -kc * (w_poly6(h, rs) / w_poly6(h, dq)) ** 4
+    push = (2 * lam + corr) * dwdr / p.pho0  # along +x for the right particle
+    sep = r["x_final"][1, 0] - r["x_final"][0, 0]
+    assert sep == pytest.approx(rs + 2 * push, rel=1e-9)
+    assert sep > rs
+
+
+def test_brute_force_equals_fresh_grid_first_pass(orc):
+    sc = scenes.block((10, 10, 10), (2.0, 2.0, 2.0), seed=13, vel_sigma=0.3)
+    p = P.dam_params(niter=1)
+    pos, vel = sc.as_layout(3)
+    a = orc.step(p, sc.ulim, sc.llim, pos, vel, sc.iid)
+    b = orc.step(p, sc.ulim, sc.llim, pos, vel, sc.iid, brute_force=True)
+    np.testing.assert_allclose(a["rho"], b["rho"], rtol=1e-12)
+    np.testing.assert_allclose(a["lambda"], b["lambda"], rtol=1e-10, atol=1e-16)
+    np.testing.assert_allclose(a["x_final"], b["x_final"], rtol=0, atol=1e-12)
+
+
+def test_step_contract_sorted_outputs(orc):
+    sc = scenes.double_dam()
+    p = P.dam_params()
+    pos, vel = sc.as_layout(4)
+    r = orc.step(p, sc.ulim, sc.llim, pos, vel, sc.iid)
+    perm = r["perm"]
+    np.testing.assert_array_equal(r["iid"], sc.iid[perm])
+    np.testing.assert_array_equal(r["pos"], pos[perm])  # sorted start positions, bit for bit
+    keys = r["keys"]
+    assert np.all(np.diff(keys.astype(np.int64)) >= 0)
+    lo = np.array([0.001] * 3)
+    hi = np.array(sc.ulim) - 0.001
+    assert np.all(r["x_final"] >= lo - 1e-12) and np.all(r["x_final"] <= hi + 1e-12)
+
+
+def test_fp32_mirror_matches_fp64_when_well_conditioned(orc):
+    """The FP32 operation sequence (the GPU's) stays within 1e-4 h of FP64 on
+    the reference's default constants."""
+    sc = scenes.double_dam()
+    p = P.reference_defaults()
+    pos, vel = sc.as_layout(4)
+    a = orc.step(p, sc.ulim, sc.llim, pos, vel, sc.iid)
+    b = orc.step(p, sc.ulim, sc.llim, pos, vel, sc.iid, fp32=True)
+    np.testing.assert_array_equal(a["iid"], b["iid"])
+    assert np.abs(a["x_final"] - b["npos"][:, :3]).max() < 1e-4 * p.h
+    assert np.abs(a["v_final"] - b["nvel"][:, :3]).max() < 1e-4 * p.h / p.dt
+
+
+def _perturb_ulp(x, seed):
+    rng = np.random.default_rng(seed)
+    s = rng.integers(-1, 2, size=x[:, :3].shape)
+    y = x.copy()
+    up = np.nextafter(x[:, :3], np.float32(np.inf))
+    dn = np.nextafter(x[:, :3], np.float32(-np.inf))
+    y[:, :3] = np.where(s > 0, up, np.where(s < 0, dn, x[:, :3]))
+    return y
+
+
+def test_conditioning_of_the_dam_constants(orc):
+    """Characterisation: with pho0=6378, eps=600, k_corr=0.1 on a 0.5h lattice
+    a +-1 ulp change of the FP32 input moves the FP64 result by far more than
+    1e-4 h (s_corr dominates lambda and the MAX_DP clamp fires), so no FP32
+    implementation can meet that bound there; with the reference defaults
+    the same noise moves it by < 1e-5 h."""
+    sc = scenes.double_dam()
+    pos, vel = sc.as_layout(4)
+    for p, ill in ((P.dam_params(), True), (P.reference_defaults(), False)):
+        a = orc.step(p, sc.ulim, sc.llim, pos, vel, sc.iid)
+        b = orc.step(p, sc.ulim, sc.llim, _perturb_ulp(pos, 1), vel, sc.iid)
+        if not np.array_equal(a["iid"], b["iid"]):
+            continue
+        d = np.abs(a["x_final"] - b["x_final"]).max()
+        if ill:
+            assert d > 1e-2
+        else:
+            assert d < 1e-5 * p.h  # a tenth of the 1e-4 h parity bound
+
+
+def _density_run(orc, p, sc, frames, fp32, pos=None):
+    if pos is None:
+        pos, vel = sc.as_layout(4)
+    else:
+        vel = sc.as_layout(4)[1]
+    iid = sc.iid
+    stats = []
+    for _ in range(frames):
+        r = orc.step(p, sc.ulim, sc.llim, pos, vel, iid, fp32=fp32)
+        pos, vel, iid = r["npos"], r["nvel"], r["iid"]
+        stats.append(orc.density_error(p, sc.ulim, sc.llim, pos))
+    return np.array(stats).mean(0)
+
+
+@pytest.mark.slow
+def test_multiframe_density_error_reference_defaults(orc):
+    """200 frames of the double dam: the frame-averaged mean density error of
+    the FP32 sequence (== GPU) is within 1% of FP64."""
+    sc = scenes.double_dam()
+    p = P.reference_defaults()
+    a = _density_run(orc, p, sc, 200, False)
+    b = _density_run(orc, p, sc, 200, True)
+    assert abs(a[0] - b[0]) <= 0.01 * a[0]
+
+
+@pytest.mark.slow
+def test_multiframe_density_error_dam_constants_within_fp64_ensemble(orc):
+    """The dam constants are chaotic over 200 frames: FP64 runs that differ
+    by 1 ulp of input spread by several percent.  The FP32 sequence's
+    frame-averaged mean and max density errors must lie within 1% of the
+    envelope of an FP64 ensemble."""
+    sc = scenes.double_dam()
+    p = P.dam_params()
+    pos, _ = sc.as_layout(4)
+    ens = np.array([_density_run(orc, p, sc, 200, False, _perturb_ulp(pos, s) if s else None)
+                    for s in (0, 1, 2)])
+    b = _density_run(orc, p, sc, 200, True)
+    lo, hi = ens.min(0), ens.max(0)
+    assert np.all(b >= 0.99 * lo) and np.all(b <= 1.01 * hi), (b, ens)

@@ -221,9 +221,12 @@ def test_capacity_and_argument_errors(pbf_mod, device):
     p = P.dam_params()
     sc = scenes.block((4, 4, 4), (1.0, 1.0, 1.0), seed=1)
     run = Run(pbf_mod, sc, p, capacity=64, max_cells=2000)
-    with pytest.raises(pbf_mod.PBFError) as e:
-        run.sim.step(run.a[0], run.b[0], run.a[1], run.b[1], run.iid, 65)
+    raw = [t.data_ptr() for t in (run.a[0], run.b[0], run.a[1], run.b[1], run.iid)]
+    with pytest.raises(pbf_mod.PBFError) as e:  # raw pointers reach the C ABI's own check
+        run.sim.step(*raw, 65)
     assert e.value.code == 2
+    with pytest.raises(ValueError):  # the Python wrapper checks tensor sizes first
+        run.sim.step(run.a[0], run.b[0], run.a[1], run.b[1], run.iid, 65)
     with pytest.raises(pbf_mod.PBFError) as e:
         run.sim.setLim((3.0, 3.0, 3.0), (0, 0, 0))  # 27000 cells > 2000
     assert e.value.code == 2

@@ -188,6 +188,38 @@ inline T dist2(const T* xi, const T* xj, T d[3]) {
   return r2;
 }
 
+// Canonical summation order (shared with the GPU kernels, which must
+// reproduce it bit for bit): a target's candidates, in the reference's walk
+// order, are summed sequentially when there are at most kLightMax of them;
+// otherwise in consecutive chunks of kChunk candidates, each chunk summed
+// sequentially from zero and the chunk sums added in chunk order.  The
+// chunked form lets the GPU spread a crowded target's candidates over a
+// wavefront instead of one lane.
+constexpr size_t kLightMax = 256;
+constexpr size_t kChunk = 64;
+
+template <class T, int K>
+struct Acc {
+  T v[K] = {};
+};
+
+template <class T, int K, class F>
+Acc<T, K> canonical_sum(const std::vector<uint32_t>& cand, F&& add) {
+  Acc<T, K> total;
+  const size_t m = cand.size();
+  if (m <= kLightMax) {
+    for (uint32_t j : cand) add(total, j);
+    return total;
+  }
+  for (size_t c0 = 0; c0 < m; c0 += kChunk) {
+    Acc<T, K> part;
+    const size_t c1 = std::min(m, c0 + kChunk);
+    for (size_t u = c0; u < c1; ++u) add(part, cand[u]);
+    for (int k = 0; k < K; ++k) total.v[k] = total.v[k] + part.v[k];
+  }
+  return total;
+}
+
 bool finite_params(const orc_params* p) {
   const float v[] = {p->pho0, p->g, p->h, p->dt, p->lambda_eps, p->delta_q,
                      p->k_corr, p->n_corr, p->k_boundaryDensity, p->c_XSPH};
@@ -268,13 +300,15 @@ int step_impl(const orc_params* p, const orc_box* box, int n, float* pos, float*
   const Consts<T> c = make_consts<T>(p, box);
   std::vector<T> lam(n, (T)0), rho(n, (T)0);
 
-  auto candidates = [&](const T* xi, auto&& f) {
+  // The candidate list of a target in the reference's walk order.
+  auto collect = [&](const T* xi, std::vector<uint32_t>& cand) {
+    cand.clear();
     if (brute) {
-      for (int j = 0; j < n; ++j) f((uint32_t)j);
+      for (int j = 0; j < n; ++j) cand.push_back((uint32_t)j);
     } else {
       int q[3];
       query_cell(xi, box, p->h, dims, q64, q);
-      for_each_candidate(g, q, f);
+      for_each_candidate(g, q, [&](uint32_t j) { cand.push_back(j); });
     }
   };
 
@@ -283,52 +317,61 @@ int step_impl(const orc_params* p, const orc_box* box, int n, float* pos, float*
     // every candidate, self included (:91); the spiky gradient / pho0 is
     // summed into grad_i (:93) and, for j != i, its squared norm into
     // gradj_l2 (:98-100); wall term (:119-120); lambda (:122-123).
-#pragma omp parallel for schedule(dynamic, 256) num_threads(TH)
-    for (int i = 0; i < n; ++i) {
-      const T* xi = &x[(size_t)3 * i];
-      T r = 0, gx = 0, gy = 0, gz = 0, gl2 = 0;
-      candidates(xi, [&](uint32_t j) {
-        T d[3];
-        const T r2 = dist2(xi, &x[(size_t)3 * j], d);
-        if (r2 < c.h2_outer) {
-          if (r2 < c.h2) {
-            const T t = c.h2 - r2;
-            r = r + ((c.poly6 * t) * t) * t;
-          }
-          const T rl = std::sqrt(r2);
-          if (rl >= c.keps && rl < c.h) {
-            const T t = c.h - rl;
-            const T s = ((c.spiky_rho0 * t) * t) / rl;
-            const T ax = s * d[0], ay = s * d[1], az = s * d[2];
-            gx = gx + ax;
-            gy = gy + ay;
-            gz = gz + az;
-            if (j != (uint32_t)i) {
-              gl2 = fma_(ax, ax, gl2);
-              gl2 = fma_(ay, ay, gl2);
-              gl2 = fma_(az, az, gl2);
+#pragma omp parallel num_threads(TH)
+    {
+      std::vector<uint32_t> cand;
+#pragma omp for schedule(dynamic, 256)
+      for (int i = 0; i < n; ++i) {
+        const T* xi = &x[(size_t)3 * i];
+        collect(xi, cand);
+        // v = {rho, grad_i x, y, z, gradj_l2}
+        Acc<T, 5> a = canonical_sum<T, 5>(cand, [&](Acc<T, 5>& s, uint32_t j) {
+          T d[3];
+          const T r2 = dist2(xi, &x[(size_t)3 * j], d);
+          if (r2 < c.h2_outer) {
+            if (r2 < c.h2) {
+              const T t = c.h2 - r2;
+              s.v[0] = s.v[0] + ((c.poly6 * t) * t) * t;
+            }
+            const T rl = std::sqrt(r2);
+            if (rl >= c.keps && rl < c.h) {
+              const T t = c.h - rl;
+              const T f = ((c.spiky_rho0 * t) * t) / rl;
+              const T ax = f * d[0], ay = f * d[1], az = f * d[2];
+              s.v[1] = s.v[1] + ax;
+              s.v[2] = s.v[2] + ay;
+              s.v[3] = s.v[3] + az;
+              if (j != (uint32_t)i) {
+                s.v[4] = fma_(ax, ax, s.v[4]);
+                s.v[4] = fma_(ay, ay, s.v[4]);
+                s.v[4] = fma_(az, az, s.v[4]);
+              }
             }
           }
-        }
-      });
-      if (c.k_boundary != (T)0) r = r + c.k_boundary * boundary_density(xi, c);
-      T den = fma_(gx, gx, gl2);
-      den = fma_(gy, gy, den);
-      den = fma_(gz, gz, den);
-      den = den + c.lambda_eps;
-      lam[i] = -(r / c.pho0 - (T)1) / den;
-      rho[i] = r;
+        });
+        T r = a.v[0];
+        if (c.k_boundary != (T)0) r = r + c.k_boundary * boundary_density(xi, c);
+        T den = fma_(a.v[1], a.v[1], a.v[4]);
+        den = fma_(a.v[2], a.v[2], den);
+        den = fma_(a.v[3], a.v[3], den);
+        den = den + c.lambda_eps;
+        lam[i] = -(r / c.pho0 - (T)1) / den;
+        rho[i] = r;
+      }
     }
     // computetpos, fluids/Simulator_kernel.cuh:131-194: sum over j != i of
     // (lambda_i + lambda_j + s_corr) * spikyGrad (:163-166), / pho0, clamp to
     // +-MAX_DP (:184), add (:188), clamp into the shrunk box (:190-192).
     // Jacobi: all reads from x, writes to xn, then x <- xn (thrust::copy_n,
     // fluids/Simulator.cu:247-248).
-#pragma omp parallel for schedule(dynamic, 256) num_threads(TH)
+#pragma omp parallel num_threads(TH)
+    {
+      std::vector<uint32_t> cand;
+#pragma omp for schedule(dynamic, 256)
     for (int i = 0; i < n; ++i) {
       const T* xi = &x[(size_t)3 * i];
-      T px = 0, py = 0, pz = 0;
-      candidates(xi, [&](uint32_t j) {
+      collect(xi, cand);
+      Acc<T, 3> a = canonical_sum<T, 3>(cand, [&](Acc<T, 3>& acc, uint32_t j) {
         if (j == (uint32_t)i) return;
         T d[3];
         const T r2 = dist2(xi, &x[(size_t)3 * j], d);
@@ -344,12 +387,13 @@ int step_impl(const orc_params* p, const orc_box* box, int n, float* pos, float*
             const T t = c.h - rl;
             const T gs = ((c.spiky * t) * t) / rl;
             const T s = ((lam[i] + lam[j]) + corr) * gs;
-            px = fma_(s, d[0], px);
-            py = fma_(s, d[1], py);
-            pz = fma_(s, d[2], pz);
+            acc.v[0] = fma_(s, d[0], acc.v[0]);
+            acc.v[1] = fma_(s, d[1], acc.v[1]);
+            acc.v[2] = fma_(s, d[2], acc.v[2]);
           }
         }
       });
+      const T px = a.v[0], py = a.v[1], pz = a.v[2];
       if (it == 0 && diag && diag->dp_raw_first) {
         diag->dp_raw_first[(size_t)3 * i + 0] = (double)px;
         diag->dp_raw_first[(size_t)3 * i + 1] = (double)py;
@@ -364,6 +408,7 @@ int step_impl(const orc_params* p, const orc_box* box, int n, float* pos, float*
         xn[(size_t)3 * i + a] = y;
       }
     }
+    }
     x.swap(xn);
   }
 
@@ -375,27 +420,31 @@ int step_impl(const orc_params* p, const orc_box* box, int n, float* pos, float*
   // computeXSPH, fluids/Simulator_kernel.cuh:196-239 (j == i included, rho
   // of the last lambda pass).  A pair with W = 0 adds exactly 0 to the
   // reference's sum whenever rho_i + rho_j != 0, so it is skipped.
-#pragma omp parallel for schedule(dynamic, 256) num_threads(TH)
-  for (int i = 0; i < n; ++i) {
-    const T* xi = &x[(size_t)3 * i];
-    const T* vi = &v[(size_t)3 * i];
-    T ax = 0, ay = 0, az = 0;
-    candidates(xi, [&](uint32_t j) {
-      T d[3];
-      const T r2 = dist2(xi, &x[(size_t)3 * j], d);
-      if (r2 < c.h2) {
-        const T t = c.h2 - r2;
-        const T w = ((c.poly6 * t) * t) * t;
-        const T* vj = &v[(size_t)3 * j];
-        const T f = ((T)2 * w) / (rho[i] + rho[j]);
-        ax = fma_(vj[0] - vi[0], f, ax);
-        ay = fma_(vj[1] - vi[1], f, ay);
-        az = fma_(vj[2] - vi[2], f, az);
-      }
-    });
-    vf[(size_t)3 * i + 0] = fma_(c.c_xsph, ax, vi[0]);  // :235
-    vf[(size_t)3 * i + 1] = fma_(c.c_xsph, ay, vi[1]);
-    vf[(size_t)3 * i + 2] = fma_(c.c_xsph, az, vi[2]);
+#pragma omp parallel num_threads(TH)
+  {
+    std::vector<uint32_t> cand;
+#pragma omp for schedule(dynamic, 256)
+    for (int i = 0; i < n; ++i) {
+      const T* xi = &x[(size_t)3 * i];
+      const T* vi = &v[(size_t)3 * i];
+      collect(xi, cand);
+      Acc<T, 3> a = canonical_sum<T, 3>(cand, [&](Acc<T, 3>& acc, uint32_t j) {
+        T d[3];
+        const T r2 = dist2(xi, &x[(size_t)3 * j], d);
+        if (r2 < c.h2) {
+          const T t = c.h2 - r2;
+          const T w = ((c.poly6 * t) * t) * t;
+          const T* vj = &v[(size_t)3 * j];
+          const T f = ((T)2 * w) / (rho[i] + rho[j]);
+          acc.v[0] = fma_(vj[0] - vi[0], f, acc.v[0]);
+          acc.v[1] = fma_(vj[1] - vi[1], f, acc.v[1]);
+          acc.v[2] = fma_(vj[2] - vi[2], f, acc.v[2]);
+        }
+      });
+      vf[(size_t)3 * i + 0] = fma_(c.c_xsph, a.v[0], vi[0]);  // :235
+      vf[(size_t)3 * i + 1] = fma_(c.c_xsph, a.v[1], vi[1]);
+      vf[(size_t)3 * i + 2] = fma_(c.c_xsph, a.v[2], vi[2]);
+    }
   }
 
   // Outputs in sorted order, FP32: the contract of Simulator::step.
@@ -452,22 +501,28 @@ int density_impl(const orc_params* p, const orc_box* box, int n, const float* xi
   const Consts<T> c = make_consts<T>(p, box);
   std::vector<double> err(n);
   const int TH = nthreads(threads);
-#pragma omp parallel for schedule(dynamic, 256) num_threads(TH)
-  for (int i = 0; i < n; ++i) {
-    const T* xi = &x[(size_t)3 * i];
-    int q[3];
-    query_cell(xi, box, p->h, dims, false, q);
-    T r = 0;
-    for_each_candidate(g, q, [&](uint32_t j) {
-      T d[3];
-      const T r2 = dist2(xi, &x[(size_t)3 * j], d);
-      if (r2 < c.h2) {
-        const T t = c.h2 - r2;
-        r = r + ((c.poly6 * t) * t) * t;
-      }
-    });
-    if (c.k_boundary != (T)0) r = r + c.k_boundary * boundary_density(xi, c);
-    err[i] = std::fabs((double)r / (double)c.pho0 - 1.0);
+#pragma omp parallel num_threads(TH)
+  {
+    std::vector<uint32_t> cand;
+#pragma omp for schedule(dynamic, 256)
+    for (int i = 0; i < n; ++i) {
+      const T* xi = &x[(size_t)3 * i];
+      int q[3];
+      query_cell(xi, box, p->h, dims, false, q);
+      cand.clear();
+      for_each_candidate(g, q, [&](uint32_t j) { cand.push_back(j); });
+      Acc<T, 1> a = canonical_sum<T, 1>(cand, [&](Acc<T, 1>& acc, uint32_t j) {
+        T d[3];
+        const T r2 = dist2(xi, &x[(size_t)3 * j], d);
+        if (r2 < c.h2) {
+          const T t = c.h2 - r2;
+          acc.v[0] = acc.v[0] + ((c.poly6 * t) * t) * t;
+        }
+      });
+      T r = a.v[0];
+      if (c.k_boundary != (T)0) r = r + c.k_boundary * boundary_density(xi, c);
+      err[i] = std::fabs((double)r / (double)c.pho0 - 1.0);
+    }
   }
   double sum = 0.0, mx = 0.0;
   for (int i = 0; i < n; ++i) {

@@ -68,7 +68,9 @@ struct pbf_solver {
   uint32_t *iid_s = nullptr, *hist = nullptr, *totals = nullptr;
   float4 *xp = nullptr, *x0s = nullptr, *xa = nullptr;
   float* rho = nullptr;
-  double *dens_partial = nullptr, *dens_out = nullptr;
+  uint32_t* counters = nullptr;  // heavy-target work list counters
+  pbf::HeavyLists hl{};
+  double *dens_err = nullptr, *dens_partial = nullptr, *dens_out = nullptr;
   uint2* ranges = nullptr;
 
   Timer timer;
@@ -313,7 +315,10 @@ int pbf_create(const pbf_params* params, pbf_vec3 ulim, pbf_vec3 llim, int capac
 
   const size_t N = (size_t)capacity;
   const size_t tiles = (size_t)pbf::radix_tiles(capacity);
-  const size_t dblk = (size_t)pbf::density_blocks(capacity);
+  const size_t dblk = (size_t)pbf::density_partials(capacity);
+  // chunk items of crowded targets: 2 per particle covers piles holding a
+  // tenth of the particles; beyond that the targets are summed in place
+  const size_t ICAP = 2 * N + 1024;
   size_t off = 0;
   auto carve = [&](size_t bytes) {
     const size_t o = off;
@@ -323,7 +328,9 @@ int pbf_create(const pbf_params* params, pbf_vec3 ulim, pbf_vec3 llim, int capac
   const size_t o_key0 = carve(4 * N), o_val0 = carve(4 * N), o_key1 = carve(4 * N),
                o_val1 = carve(4 * N), o_iid = carve(4 * N), o_hist = carve(4 * 256 * tiles),
                o_tot = carve(4 * 256), o_xp = carve(16 * N), o_x0s = carve(16 * N),
-               o_xa = carve(16 * N), o_rho = carve(4 * N), o_dp = carve(16 * dblk),
+               o_xa = carve(16 * N), o_rho = carve(4 * N), o_recs = carve(16 * N),
+               o_items = carve(8 * ICAP), o_part = carve(4 * pbf::kPartialWords * ICAP),
+               o_cnt = carve(4 * 8), o_de = carve(8 * N), o_dp = carve(16 * dblk),
                o_do = carve(16), o_rng = carve(8 * (size_t)max_cells);
   void* arena = nullptr;
   if ((e = hipMalloc(&arena, off)) != hipSuccess) {
@@ -343,6 +350,13 @@ int pbf_create(const pbf_params* params, pbf_vec3 ulim, pbf_vec3 llim, int capac
   s->x0s = (float4*)(b + o_x0s);
   s->xa = (float4*)(b + o_xa);
   s->rho = (float*)(b + o_rho);
+  s->counters = (uint32_t*)(b + o_cnt);
+  s->hl.counters = s->counters;
+  s->hl.recs = (uint4*)(b + o_recs);
+  s->hl.items = (uint2*)(b + o_items);
+  s->hl.partials = (float*)(b + o_part);
+  s->hl.item_cap = (uint32_t)ICAP;
+  s->dens_err = (double*)(b + o_de);
   s->dens_partial = (double*)(b + o_dp);
   s->dens_out = (double*)(b + o_do);
   s->ranges = (uint2*)(b + o_rng);
@@ -434,7 +448,7 @@ int pbf_step(pbf_solver* s, float* pos, float* npos, float* vel, float* nvel, ui
   tbegin(s, PBF_PHASE_STEP, st);
   // advect() + getGridId (Simulator.cpp:50, Simulator.cu:165-193)
   tbegin(s, PBF_PHASE_PREDICT, st);
-  pbf::launch_predict_key(L, pos, vel, n, c, s->xp, s->key0, s->val0, st);
+  pbf::launch_predict_key(L, pos, vel, n, c, s->xp, s->key0, s->val0, s->counters, st);
   tend(s, PBF_PHASE_PREDICT, st);
   // thrust::sort_by_key (Simulator.cu:196-198)
   tbegin(s, PBF_PHASE_SORT, st);
@@ -452,12 +466,14 @@ int pbf_step(pbf_solver* s, float* pos, float* npos, float* vel, float* nvel, ui
   // niter x correctDensity (Simulator.cpp:62-66, Simulator.cu:213-249);
   // xl (x, lambda) aliases xp, which is dead after the permute.
   float4* xl = s->xp;
+  int pass = 0;
   for (int it = 0; it < s->prm.niter; ++it) {
     tbegin(s, PBF_PHASE_LAMBDA, st);
-    pbf::launch_lambda(s->xa, s->ranges, n, c, xl, it + 1 == s->prm.niter ? s->rho : nullptr, st);
+    pbf::launch_lambda(s->xa, s->ranges, n, c, xl, it + 1 == s->prm.niter ? s->rho : nullptr,
+                       s->hl, pass++, st);
     tend(s, PBF_PHASE_LAMBDA, st);
     tbegin(s, PBF_PHASE_DELTA, st);
-    pbf::launch_delta(xl, s->ranges, n, c, s->xa, st);
+    pbf::launch_delta(xl, s->ranges, n, c, s->xa, s->hl, pass++, st);
     tend(s, PBF_PHASE_DELTA, st);
   }
   // updateVelocity (Simulator.cpp:72, Simulator.cu:267-274); vr aliases xp.
@@ -467,7 +483,7 @@ int pbf_step(pbf_solver* s, float* pos, float* npos, float* vel, float* nvel, ui
   tend(s, PBF_PHASE_VEL, st);
   // correctVelocity (Simulator.cpp:77, Simulator.cu:251-265)
   tbegin(s, PBF_PHASE_XSPH, st);
-  pbf::launch_xsph(L, s->xa, vr, s->ranges, n, c, npos, nvel, st);
+  pbf::launch_xsph(L, s->xa, vr, s->ranges, n, c, npos, nvel, s->hl, pass, st);
   tend(s, PBF_PHASE_XSPH, st);
   tend(s, PBF_PHASE_STEP, st);
 
@@ -493,13 +509,14 @@ int pbf_density_error(pbf_solver* s, const float* pos, int n, void* stream, doub
   DeviceGuard g(s->device);
   if (!g.ok) return fail(s, PBF_ERR_HIP, "hipSetDevice(%d) failed", s->device);
   hipStream_t st = (hipStream_t)stream;
-  pbf::launch_key_only(s->layout, pos, n, c, s->xp, s->key0, s->val0, st);
+  pbf::launch_key_only(s->layout, pos, n, c, s->xp, s->key0, s->val0, s->counters, st);
   const int which = pbf::radix_sort_pairs(s->key0, s->val0, s->key1, s->val1, n,
                                           key_bits_for(c.cells), s->hist, s->totals, st);
   pbf::launch_clear_ranges(s->ranges, c.cells, st);
   pbf::launch_permute_pos(which ? s->key1 : s->key0, which ? s->val1 : s->val0, s->xp, n,
                           c.cells, s->xa, s->ranges, st);
-  pbf::launch_density_error(s->xa, s->ranges, n, c, s->dens_partial, s->dens_out, st);
+  pbf::launch_density_error(s->xa, s->ranges, n, c, s->dens_err, s->dens_partial, s->dens_out,
+                            s->hl, st);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail_hip(s, e, "kernel launch");
   double host[2];

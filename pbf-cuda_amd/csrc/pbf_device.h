@@ -70,39 +70,164 @@ __device__ __forceinline__ uint32_t cell_id(const Cell& q, const StepConsts& c) 
   return (uint32_t)((q.x * c.dy + q.y) * c.dz + q.z);
 }
 
-// Visits the candidate neighbours of a particle whose query cell is q, in
-// the reference's order (fluids/Simulator_kernel.cuh:78-88): dx, then dy,
-// then dz, j ascending.  The three cells (x,y,z-1..z+1) of one column are
+// Candidate neighbours of a particle whose query cell is q, in the
+// reference's order (fluids/Simulator_kernel.cuh:78-88): dx, then dy, then
+// dz, j ascending.  The three cells (x,y,z-1..z+1) of one column are
 // consecutive cell ids, so their particles form one contiguous sorted range:
 // [first start, last end) over the column's non-empty cells.  Walking that
 // range is the same j sequence as walking the three cells in turn.
-template <class F>
-__device__ __forceinline__ void for_each_candidate(const Cell& q, const uint2* __restrict__ ranges,
-                                                   const StepConsts& c, F&& f) {
-  const int z0 = q.z > 0 ? q.z - 1 : 0;
-  const int z1 = q.z + 1 < c.dz ? q.z + 1 : c.dz - 1;
-#pragma unroll 1
-  for (int ox = -1; ox <= 1; ++ox) {
-    const int X = q.x + ox;
-    if (X < 0 || X >= c.dx) continue;
-#pragma unroll 1
-    for (int oy = -1; oy <= 1; ++oy) {
-      const int Y = q.y + oy;
-      if (Y < 0 || Y >= c.dy) continue;
-      const int col = (X * c.dy + Y) * c.dz;
-      uint32_t b = 0xffffffffu, e = 0u;
-      for (int z = z0; z <= z1; ++z) {
+// column_range(q, k) is column k = 3*(dx+1) + (dy+1); empty -> [0,0).
+__device__ __forceinline__ uint2 column_range(const Cell& q, int k,
+                                              const uint2* __restrict__ ranges,
+                                              const StepConsts& c) {
+  const int X = q.x + k / 3 - 1;
+  const int Y = q.y + k % 3 - 1;
+  uint32_t b = 0xffffffffu, e = 0u;
+  if (X >= 0 && X < c.dx && Y >= 0 && Y < c.dy) {
+    const int col = (X * c.dy + Y) * c.dz;
+#pragma unroll
+    for (int oz = -1; oz <= 1; ++oz) {
+      const int z = q.z + oz;
+      if (z >= 0 && z < c.dz) {
         const uint2 r = ranges[col + z];
         if (r.x < r.y) {
           b = r.x < b ? r.x : b;
           e = r.y > e ? r.y : e;
         }
       }
+    }
+  }
+  return b < e ? make_uint2(b, e) : make_uint2(0u, 0u);
+}
+
+// Canonical summation order, shared with the host oracle's FP32 mirror: a
+// target's candidate sequence is summed sequentially when it has at most
+// kLightMax candidates (the "light" kernels, one work-item per target);
+// otherwise in consecutive chunks of kChunk candidates, each summed
+// sequentially from zero, chunk sums added in chunk order (the "heavy"
+// kernels, one wavefront per target, one chunk per lane).  Crowded cells
+// (particles piled into a corner) then cost one wavefront a few chunk
+// lengths instead of one lane thousands of dependent gathers.
+constexpr uint32_t kLightMax = 256;
+constexpr uint32_t kChunk = 64;
+
+// Calls add(j, load(j)) for j in [j0, j1) in order, with the loads of four
+// consecutive candidates issued together so their latencies overlap.
+template <class Load, class Add>
+__device__ __forceinline__ void sweep_range(uint32_t j0, uint32_t j1, Load&& load, Add&& add) {
 #pragma unroll 1
-      for (uint32_t j = b; j < e; ++j) f(j);
+  for (uint32_t j = j0; j < j1; j += 4) {
+    const uint32_t last = j1 - 1;
+    const auto d0 = load(j);
+    const auto d1 = load(j + 1 < j1 ? j + 1 : last);
+    const auto d2 = load(j + 2 < j1 ? j + 2 : last);
+    const auto d3 = load(j + 3 < j1 ? j + 3 : last);
+    add(j, d0);
+    if (j + 1 < j1) add(j + 1, d1);
+    if (j + 2 < j1) add(j + 2, d2);
+    if (j + 3 < j1) add(j + 3, d3);
+  }
+}
+
+// Same visit as sweep_range restricted to the candidates that can
+// contribute: blocks of up to 32 candidates are first screened with the
+// cheap test hit(j, data) (loads four at a time) into a bit mask, then
+// add(j, data) runs only for the set bits, in ascending j, with those
+// candidates reloaded four at a time.  Candidates failing hit() contribute
+// exactly nothing in add(), so the sums are identical to the plain sweep;
+// the expensive part of add() no longer runs for every candidate of every
+// lane whenever any lane of the wavefront has a hit.
+template <class Load, class Hit, class Add>
+__device__ __forceinline__ void sweep_filtered(uint32_t j0, uint32_t j1, Load&& load, Hit&& hit,
+                                               Add&& add) {
+#pragma unroll 1
+  for (uint32_t b = j0; b < j1; b += 32) {
+    const uint32_t e = j1 - b < 32u ? j1 : b + 32u;
+    const uint32_t last = e - 1;
+    uint32_t mask = 0;
+#pragma unroll 1
+    for (uint32_t j = b; j < e; j += 4) {
+      const auto d0 = load(j);
+      const auto d1 = load(j + 1 < e ? j + 1 : last);
+      const auto d2 = load(j + 2 < e ? j + 2 : last);
+      const auto d3 = load(j + 3 < e ? j + 3 : last);
+      const uint32_t u = j - b;
+      if (hit(j, d0)) mask |= 1u << u;
+      if (j + 1 < e && hit(j + 1, d1)) mask |= 2u << u;
+      if (j + 2 < e && hit(j + 2, d2)) mask |= 4u << u;
+      if (j + 3 < e && hit(j + 3, d3)) mask |= 8u << u;
+    }
+#pragma unroll 1
+    while (mask) {
+      const uint32_t t0 = __builtin_ctz(mask);
+      mask &= mask - 1u;
+      uint32_t t1 = t0, t2 = t0, t3 = t0;
+      const bool h1 = mask != 0u;
+      if (h1) {
+        t1 = __builtin_ctz(mask);
+        mask &= mask - 1u;
+      }
+      const bool h2 = mask != 0u;
+      if (h2) {
+        t2 = __builtin_ctz(mask);
+        mask &= mask - 1u;
+      }
+      const bool h3 = mask != 0u;
+      if (h3) {
+        t3 = __builtin_ctz(mask);
+        mask &= mask - 1u;
+      }
+      const auto d0 = load(b + t0);
+      const auto d1 = load(b + t1);
+      const auto d2 = load(b + t2);
+      const auto d3 = load(b + t3);
+      add(b + t0, d0);
+      if (h1) add(b + t1, d1);
+      if (h2) add(b + t2, d2);
+      if (h3) add(b + t3, d3);
     }
   }
 }
+
+// Positions [s, t) of the candidate sequence given the nine column ranges.
+template <class Load, class Hit, class Add>
+__device__ __forceinline__ void sweep_seq(const uint2 (&col)[9], uint32_t s, uint32_t t, Load&& load,
+                                          Hit&& hit, Add&& add) {
+  uint32_t cum = 0;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    const uint32_t len = col[k].y - col[k].x;
+    const uint32_t lo = s > cum ? s - cum : 0u;
+    uint32_t hi = t > cum ? t - cum : 0u;
+    hi = hi < len ? hi : len;
+    if (lo < hi) sweep_filtered(col[k].x + lo, col[k].x + hi, load, hit, add);
+    cum += len;
+  }
+}
+
+// Exclusive prefix sum of v over the active lanes of the wavefront; *total
+// receives the sum.  All active lanes must call it.
+__device__ __forceinline__ uint32_t wave_exclusive_sum(uint32_t v, uint32_t* total) {
+  const int lane = threadIdx.x & 63;
+  uint32_t incl = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t t = __shfl_up(incl, off);
+    if (lane >= off) incl += t;
+  }
+  *total = __shfl(incl, 63);
+  return incl - v;
+}
+
+// Work lists of the heavy targets of one neighbour pass.
+struct HeavyLists {
+  uint32_t* counters;  // [2][4]: per pass parity {records, chunk items, -, -}
+  uint4* recs;         // per heavy target: (i, first item, chunks, 0)
+  uint2* items;        // per chunk: (i, chunk index); i = ~0u marks an unused slot
+  float* partials;     // kPartialWords floats per chunk item
+  uint32_t item_cap;   // capacity of items / partials
+};
+constexpr int kPartialWords = 8;
 
 // x^n for the s_corr exponent: repeated multiplication for small integer n
 // (the reference default n=4), powf otherwise.

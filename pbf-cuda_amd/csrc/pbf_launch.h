@@ -18,11 +18,14 @@ int radix_tiles(int n);
 int radix_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, int n, int key_bits,
                      uint32_t* hist, uint32_t* totals, hipStream_t st);
 
-// step.hip
+// step.hip.  Neighbour passes take the heavy-target work lists; `pass`
+// numbers the neighbour passes of one step from 0 (predict / key_only clear
+// the parity-0 counters, each light kernel clears the other parity's).
 void launch_predict_key(int layout, const float* pos, const float* vel, int n, const StepConsts& c,
-                        float4* xp, uint32_t* key, uint32_t* val, hipStream_t st);
+                        float4* xp, uint32_t* key, uint32_t* val, uint32_t* counters,
+                        hipStream_t st);
 void launch_key_only(int layout, const float* pos, int n, const StepConsts& c, float4* xp,
-                     uint32_t* key, uint32_t* val, hipStream_t st);
+                     uint32_t* key, uint32_t* val, uint32_t* counters, hipStream_t st);
 void launch_clear_ranges(uint2* ranges, long long cells, hipStream_t st);
 void launch_permute(int layout, const uint32_t* skey, const uint32_t* sval, const float* pos,
                     const float4* xp, const uint32_t* iid, int n, int cells, float4* x0s,
@@ -30,17 +33,19 @@ void launch_permute(int layout, const uint32_t* skey, const uint32_t* sval, cons
 void launch_permute_pos(const uint32_t* skey, const uint32_t* sval, const float4* xp, int n,
                         int cells, float4* xa, uint2* ranges, hipStream_t st);
 void launch_lambda(const float4* xa, const uint2* ranges, int n, const StepConsts& c, float4* xl,
-                   float* rho, hipStream_t st);
+                   float* rho, const HeavyLists& hl, int pass, hipStream_t st);
 void launch_delta(const float4* xl, const uint2* ranges, int n, const StepConsts& c, float4* xa,
-                  hipStream_t st);
+                  const HeavyLists& hl, int pass, hipStream_t st);
 void launch_velocity(int layout, const float4* xa, const float4* x0s, const uint32_t* iid_s,
                      const float* rho, int n, const StepConsts& c, float4* vr, float* pos,
                      float* vel, uint32_t* iid, hipStream_t st);
 void launch_xsph(int layout, const float4* xa, const float4* vr, const uint2* ranges, int n,
-                 const StepConsts& c, float* npos, float* nvel, hipStream_t st);
-// partial holds 2*density_blocks(n) doubles; out (device) 2 doubles.
-int density_blocks(int n);
+                 const StepConsts& c, float* npos, float* nvel, const HeavyLists& hl, int pass,
+                 hipStream_t st);
+// err holds n doubles, partial 2*density_partials(n), out (device) 2.
+int density_partials(int n);
 void launch_density_error(const float4* xa, const uint2* ranges, int n, const StepConsts& c,
-                          double* partial, double* out, hipStream_t st);
+                          double* err, double* partial, double* out, const HeavyLists& hl,
+                          hipStream_t st);
 
 }  // namespace pbf

@@ -4,16 +4,21 @@
 //   xp   float4  predicted position, input order (dead after k_permute)
 //   x0s  float4  start-of-frame position
 //   xa   float4  current position iterate (w unused)
-//   xl   float4  (position, lambda): written by k_lambda, gathered by
-//                k_delta, so one 16-byte load brings a neighbour's position
-//                and lambda (aliases xp)
-//   vr   float4  (velocity before XSPH, rho): gathered by k_xsph (aliases xp)
+//   xl   float4  (position, lambda): written by the lambda pass, gathered by
+//                the delta-p pass, so one 16-byte load brings a neighbour's
+//                position and lambda (aliases xp)
+//   vr   float4  (velocity before XSPH, rho): gathered by XSPH (aliases xp)
 //   rho  float   density of the last lambda pass
 //   ranges uint2 per cell: [start, end) of its particles, 0/0 when empty
-// The neighbour kernels are one work-item per particle over the cell-sorted
-// order; consecutive work-items share neighbour cells, so their gathers hit
-// the same cache lines, and workgroups are remapped so each XCD's L2 serves
-// one contiguous range of particles.
+//
+// Each neighbour pass (lambda, delta-p, XSPH, density) is three kernels:
+//   light  one work-item per target over the cell-sorted order; workgroups
+//          remapped so each XCD's L2 serves one contiguous particle range; a
+//          target with more than kLightMax candidates is appended to the
+//          heavy list instead;
+//   chunks one work-item per kChunk-candidate chunk of the heavy targets;
+//   combine one work-item per heavy target, adding its chunk sums in order.
+// Both follow the canonical summation order of pbf_device.h.
 #include "pbf_device.h"
 #include "pbf_launch.h"
 
@@ -22,20 +27,426 @@ namespace pbf {
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kWavesPerBlock = kBlock / 64;
 
 inline int blocks_for(long long n) { return (int)((n + kBlock - 1) / kBlock); }
+
+// ---------------------------------------------------------------------------
+// Pass policies: what a target loads, what a candidate contributes, how the
+// sums finish.  add() is the body of the reference's candidate loop.
+// ---------------------------------------------------------------------------
+
+// computeLambda (fluids/Simulator_kernel.cuh:52-129): rho sums poly6 over
+// every candidate including i itself; the spiky gradient / pho0 is summed
+// into grad_i and, for j != i, its squared norm into gradj_l2; then the wall
+// term and lambda = -(rho/pho0 - 1) / (gradj_l2 + |grad_i|^2 + eps).
+struct LambdaPass {
+  struct Args {
+    const float4* __restrict__ x;
+    float4* __restrict__ xl;
+    float* __restrict__ rho;  // written only when non-null (last iteration)
+  };
+  struct Target {
+    float4 x;
+    uint32_t i;
+  };
+  struct Acc {
+    float rho = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f, gl2 = 0.0f;
+  };
+  static __device__ __forceinline__ Target target(const Args& a, uint32_t i, const StepConsts&) {
+    return Target{a.x[i], i};
+  }
+  static __device__ __forceinline__ float4 load(const Args& a, uint32_t j) { return a.x[j]; }
+  static __device__ __forceinline__ bool hit(const Target& t, uint32_t, const float4& xj,
+                                             const StepConsts& c) {
+    float dx, dy, dz;
+    return dist2(t.x, xj, dx, dy, dz) < c.h2_outer;
+  }
+  static __device__ __forceinline__ void add(Acc& s, const Target& t, uint32_t j, const float4& xj,
+                                             const StepConsts& c) {
+    float dx, dy, dz;
+    const float r2 = dist2(t.x, xj, dx, dy, dz);
+    if (r2 < c.h2_outer) {
+      if (r2 < c.h2) s.rho = s.rho + poly6_of(r2, c);
+      const float r = sqrtf(r2);
+      if (r >= c.keps && r < c.h) {
+        const float u = c.h - r;
+        const float f = ((c.spiky_rho0 * u) * u) / r;
+        const float ax = f * dx, ay = f * dy, az = f * dz;
+        s.gx = s.gx + ax;
+        s.gy = s.gy + ay;
+        s.gz = s.gz + az;
+        if (j != t.i) {
+          s.gl2 = fmaf(ax, ax, s.gl2);
+          s.gl2 = fmaf(ay, ay, s.gl2);
+          s.gl2 = fmaf(az, az, s.gl2);
+        }
+      }
+    }
+  }
+  static __device__ __forceinline__ void accumulate(Acc& tot, const Acc& p) {
+    tot.rho = tot.rho + p.rho;
+    tot.gx = tot.gx + p.gx;
+    tot.gy = tot.gy + p.gy;
+    tot.gz = tot.gz + p.gz;
+    tot.gl2 = tot.gl2 + p.gl2;
+  }
+  static __device__ __forceinline__ void finish(const Args& a, const Target& t, const Acc& s,
+                                                const StepConsts& c) {
+    float rho = s.rho;
+    if (c.k_boundary != 0.0f) rho = rho + c.k_boundary * boundary_density(t.x.x, t.x.y, t.x.z, c);
+    float den = fmaf(s.gx, s.gx, s.gl2);
+    den = fmaf(s.gy, s.gy, den);
+    den = fmaf(s.gz, s.gz, den);
+    den = den + c.lambda_eps;
+    const float lam = -(rho / c.pho0 - 1.0f) / den;
+    a.xl[t.i] = make_float4(t.x.x, t.x.y, t.x.z, lam);
+    if (a.rho) a.rho[t.i] = rho;
+  }
+};
+
+// computetpos (fluids/Simulator_kernel.cuh:131-194): sum over j != i of
+// (lambda_i + lambda_j + s_corr) * spikyGrad, / pho0, clamped to +-MAX_DP per
+// axis, added, then clamped into the box shrunk by LIM_EPS.  Reads
+// (x, lambda) from xl, writes the new iterate to xa (Jacobi; the ping-pong
+// replaces thrust::copy_n at fluids/Simulator.cu:247-248).
+struct DeltaPass {
+  struct Args {
+    const float4* __restrict__ xl;
+    float4* __restrict__ xa;
+  };
+  struct Target {
+    float4 x;  // w = lambda_i
+    uint32_t i;
+  };
+  struct Acc {
+    float px = 0.0f, py = 0.0f, pz = 0.0f;
+  };
+  static __device__ __forceinline__ Target target(const Args& a, uint32_t i, const StepConsts&) {
+    return Target{a.xl[i], i};
+  }
+  static __device__ __forceinline__ float4 load(const Args& a, uint32_t j) { return a.xl[j]; }
+  static __device__ __forceinline__ bool hit(const Target& t, uint32_t j, const float4& xj,
+                                             const StepConsts& c) {
+    float dx, dy, dz;
+    return j != t.i && dist2(t.x, xj, dx, dy, dz) < c.h2_outer;
+  }
+  static __device__ __forceinline__ void add(Acc& s, const Target& t, uint32_t j, const float4& xj,
+                                             const StepConsts& c) {
+    if (j == t.i) return;
+    float dx, dy, dz;
+    const float r2 = dist2(t.x, xj, dx, dy, dz);
+    if (r2 < c.h2_outer) {
+      const float r = sqrtf(r2);
+      if (r >= c.keps && r < c.h) {
+        const float w = r2 < c.h2 ? poly6_of(r2, c) : 0.0f;
+        const float corr = c.coef_corr * pow_corr(w, c);
+        const float u = c.h - r;
+        const float gs = ((c.spiky * u) * u) / r;
+        const float f = ((t.x.w + xj.w) + corr) * gs;
+        s.px = fmaf(f, dx, s.px);
+        s.py = fmaf(f, dy, s.py);
+        s.pz = fmaf(f, dz, s.pz);
+      }
+    }
+  }
+  static __device__ __forceinline__ void accumulate(Acc& tot, const Acc& p) {
+    tot.px = tot.px + p.px;
+    tot.py = tot.py + p.py;
+    tot.pz = tot.pz + p.pz;
+  }
+  static __device__ __forceinline__ void finish(const Args& a, const Target& t, const Acc& s,
+                                                const StepConsts& c) {
+    const float px = fminf(fmaxf(s.px / c.pho0, -c.max_dp), c.max_dp);
+    const float py = fminf(fmaxf(s.py / c.pho0, -c.max_dp), c.max_dp);
+    const float pz = fminf(fmaxf(s.pz / c.pho0, -c.max_dp), c.max_dp);
+    const float x = fmaxf(fminf(t.x.x + px, c.hix), c.lox);
+    const float y = fmaxf(fminf(t.x.y + py, c.hiy), c.loy);
+    const float z = fmaxf(fminf(t.x.z + pz, c.hiz), c.loz);
+    a.xa[t.i] = make_float4(x, y, z, 0.0f);
+  }
+};
+
+// computeXSPH (fluids/Simulator_kernel.cuh:196-239): nvel_i = v_i +
+// c * sum_j 2 (v_j - v_i) W(|x_i - x_j|^2) / (rho_i + rho_j), j == i
+// included (its term is 0), rho of the last lambda pass.  Pairs with W == 0
+// are skipped: their reference term is exactly 0.  Also stores the final
+// positions into the caller's npos.
+template <int L>
+struct XsphPass {
+  struct Args {
+    const float4* __restrict__ xa;
+    const float4* __restrict__ vr;  // (v, rho)
+    float* __restrict__ npos;
+    float* __restrict__ nvel;
+  };
+  struct Target {
+    float4 x, v;
+    uint32_t i;
+  };
+  struct Acc {
+    float ax = 0.0f, ay = 0.0f, az = 0.0f;
+  };
+  // Positions are batch-loaded for every candidate; a neighbour's (v, rho)
+  // only for pairs inside h.
+  struct Nb {
+    float4 x;
+    const float4* __restrict__ vr;
+  };
+  static __device__ __forceinline__ Target target(const Args& a, uint32_t i, const StepConsts&) {
+    return Target{a.xa[i], a.vr[i], i};
+  }
+  static __device__ __forceinline__ Nb load(const Args& a, uint32_t j) {
+    return Nb{a.xa[j], a.vr};
+  }
+  static __device__ __forceinline__ bool hit(const Target& t, uint32_t, const Nb& nb,
+                                             const StepConsts& c) {
+    float dx, dy, dz;
+    return dist2(t.x, nb.x, dx, dy, dz) < c.h2;
+  }
+  static __device__ __forceinline__ void add(Acc& s, const Target& t, uint32_t j, const Nb& nb,
+                                             const StepConsts& c) {
+    float dx, dy, dz;
+    const float r2 = dist2(t.x, nb.x, dx, dy, dz);
+    if (r2 < c.h2) {
+      const float4 vj = nb.vr[j];
+      const float w = poly6_of(r2, c);
+      const float f = (2.0f * w) / (t.v.w + vj.w);
+      s.ax = fmaf(vj.x - t.v.x, f, s.ax);
+      s.ay = fmaf(vj.y - t.v.y, f, s.ay);
+      s.az = fmaf(vj.z - t.v.z, f, s.az);
+    }
+  }
+  static __device__ __forceinline__ void accumulate(Acc& tot, const Acc& p) {
+    tot.ax = tot.ax + p.ax;
+    tot.ay = tot.ay + p.ay;
+    tot.az = tot.az + p.az;
+  }
+  static __device__ __forceinline__ void finish(const Args& a, const Target& t, const Acc& s,
+                                                const StepConsts& c) {
+    store_vec<L>(a.nvel, t.i, fmaf(c.c_xsph, s.ax, t.v.x), fmaf(c.c_xsph, s.ay, t.v.y),
+                 fmaf(c.c_xsph, s.az, t.v.z));
+    store_vec<L>(a.npos, t.i, t.x.x, t.x.y, t.x.z);
+  }
+};
+
+// rho on a fresh grid (the lambda pass's density term), for the density
+// error statistic: |rho/pho0 - 1| per particle in FP64.
+struct DensityPass {
+  struct Args {
+    const float4* __restrict__ x;
+    double* __restrict__ err;
+  };
+  struct Target {
+    float4 x;
+    uint32_t i;
+  };
+  struct Acc {
+    float rho = 0.0f;
+  };
+  static __device__ __forceinline__ Target target(const Args& a, uint32_t i, const StepConsts&) {
+    return Target{a.x[i], i};
+  }
+  static __device__ __forceinline__ float4 load(const Args& a, uint32_t j) { return a.x[j]; }
+  static __device__ __forceinline__ bool hit(const Target& t, uint32_t, const float4& xj,
+                                             const StepConsts& c) {
+    float dx, dy, dz;
+    return dist2(t.x, xj, dx, dy, dz) < c.h2;
+  }
+  static __device__ __forceinline__ void add(Acc& s, const Target& t, uint32_t, const float4& xj,
+                                             const StepConsts& c) {
+    float dx, dy, dz;
+    const float r2 = dist2(t.x, xj, dx, dy, dz);
+    if (r2 < c.h2) s.rho = s.rho + poly6_of(r2, c);
+  }
+  static __device__ __forceinline__ void accumulate(Acc& tot, const Acc& p) {
+    tot.rho = tot.rho + p.rho;
+  }
+  static __device__ __forceinline__ void finish(const Args& a, const Target& t, const Acc& s,
+                                                const StepConsts& c) {
+    float rho = s.rho;
+    if (c.k_boundary != 0.0f) rho = rho + c.k_boundary * boundary_density(t.x.x, t.x.y, t.x.z, c);
+    a.err[t.i] = fabs((double)rho / (double)c.pho0 - 1.0);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Generic neighbour-pass kernels over a pass policy.
+// ---------------------------------------------------------------------------
+
+// Light: one work-item per target.  A target with more than kLightMax
+// candidates is heavy: it gets one chunk item per kChunk candidates and a
+// record, appended with one atomic per wavefront each.  counters[pass & 1]
+// belong to this pass; the kernel clears the other parity's counters for the
+// next pass (their previous users finished earlier on the stream).  If the
+// chunk list is full the lane sums its target itself, still chunk by chunk,
+// so the canonical order holds either way.
+template <class P>
+__global__ __launch_bounds__(kBlock) void k_light(typename P::Args a, int n, StepConsts c,
+                                                  const uint2* __restrict__ ranges, HeavyLists hl,
+                                                  int pass) {
+  __shared__ uint2 cols[9][kBlock];
+  uint32_t* cnt = hl.counters + 4 * (pass & 1);
+  if (blockIdx.x == 0 && threadIdx.x < 4) hl.counters[4 * ((pass + 1) & 1) + threadIdx.x] = 0u;
+  const int tid = threadIdx.x;
+  const int i = xcd_block_index() * kBlock + tid;
+  const bool active = i < n;
+  typename P::Target t{};
+  uint32_t m = 0;
+  if (active) {
+    t = P::target(a, (uint32_t)i, c);
+    const Cell q = cell_of(t.x.x, t.x.y, t.x.z, c);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      const uint2 r = column_range(q, k, ranges, c);
+      cols[k][tid] = r;
+      m += r.y - r.x;
+    }
+  }
+  const bool is_heavy = active && m > kLightMax;
+  const uint32_t nch = is_heavy ? (m + kChunk - 1) / kChunk : 0u;
+  if (__ballot(is_heavy) != 0ull) {
+    // wave-aggregated allocation of records and chunk items
+    uint32_t rec_total, item_total;
+    const uint32_t rec_off = wave_exclusive_sum(is_heavy ? 1u : 0u, &rec_total);
+    const uint32_t item_off = wave_exclusive_sum(nch, &item_total);
+    uint32_t rec_base = 0, item_base = 0;
+    if ((threadIdx.x & 63) == 0) {
+      item_base = atomicAdd(&cnt[1], item_total);
+    }
+    item_base = __shfl(item_base, 0);
+    const uint32_t first = item_base + item_off;
+    const bool fits = is_heavy && first + nch <= hl.item_cap;
+    if (is_heavy) {
+      const uint32_t stop = first + nch < hl.item_cap ? first + nch : hl.item_cap;
+      for (uint32_t u = first; u < stop; ++u)
+        hl.items[u] = fits ? make_uint2((uint32_t)i, u - first) : make_uint2(~0u, 0u);
+    }
+    const uint64_t fm = __ballot(fits);
+    if (fm != 0ull) {
+      if ((threadIdx.x & 63) == 0) rec_base = atomicAdd(&cnt[0], (uint32_t)__popcll(fm));
+      rec_base = __shfl(rec_base, 0);
+      if (fits)
+        hl.recs[rec_base + (uint32_t)__popcll(fm & ((1ull << (threadIdx.x & 63)) - 1ull))] =
+            make_uint4((uint32_t)i, first, nch, 0u);
+    }
+    (void)rec_off;
+    (void)rec_total;
+    if (fits) return;
+  }
+  if (!active) return;
+  typename P::Acc s;
+  if (!is_heavy) {
+#pragma unroll 1
+    for (int k = 0; k < 9; ++k) {
+      const uint2 r = cols[k][tid];
+      sweep_filtered(
+          r.x, r.y, [&](uint32_t j) { return P::load(a, j); },
+          [&](uint32_t j, const auto& d) { return P::hit(t, j, d, c); },
+          [&](uint32_t j, const auto& d) { P::add(s, t, j, d, c); });
+    }
+  } else {
+    // chunk list full: this lane sums its chunks in order
+    uint2 col[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) col[k] = cols[k][tid];
+    for (uint32_t ch = 0; ch < nch; ++ch) {
+      typename P::Acc part;
+      const uint32_t s0 = ch * kChunk;
+      const uint32_t s1 = s0 + kChunk < m ? s0 + kChunk : m;
+      sweep_seq(
+          col, s0, s1, [&](uint32_t j) { return P::load(a, j); },
+          [&](uint32_t j, const auto& d) { return P::hit(t, j, d, c); },
+          [&](uint32_t j, const auto& d) { P::add(part, t, j, d, c); });
+      P::accumulate(s, part);
+    }
+  }
+  P::finish(a, t, s, c);
+}
+
+// One work-item per chunk item: the partial sum of kChunk consecutive
+// candidates of a heavy target.
+template <class P>
+__global__ __launch_bounds__(kBlock) void k_chunks(typename P::Args a, StepConsts c,
+                                                   const uint2* __restrict__ ranges, HeavyLists hl,
+                                                   int pass) {
+  const uint32_t* cnt = hl.counters + 4 * (pass & 1);
+  const uint32_t total = cnt[1] < hl.item_cap ? cnt[1] : hl.item_cap;
+  const uint32_t stride = gridDim.x * kBlock;
+  for (uint32_t u = (uint32_t)xcd_block_index() * kBlock + threadIdx.x; u < total; u += stride) {
+    const uint2 it = hl.items[u];
+    if (it.x == ~0u) continue;
+    const typename P::Target t = P::target(a, it.x, c);
+    const Cell q = cell_of(t.x.x, t.x.y, t.x.z, c);
+    uint2 col[9];
+    uint32_t m = 0;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      col[k] = column_range(q, k, ranges, c);
+      m += col[k].y - col[k].x;
+    }
+    typename P::Acc part;
+    const uint32_t s0 = it.y * kChunk;
+    const uint32_t s1 = s0 + kChunk < m ? s0 + kChunk : m;
+    sweep_seq(
+        col, s0, s1, [&](uint32_t j) { return P::load(a, j); },
+        [&](uint32_t j, const auto& d) { return P::hit(t, j, d, c); },
+        [&](uint32_t j, const auto& d) { P::add(part, t, j, d, c); });
+    static_assert(sizeof(part) <= kPartialWords * sizeof(float), "partial too large");
+    *reinterpret_cast<typename P::Acc*>(hl.partials + (size_t)u * kPartialWords) = part;
+  }
+}
+
+// One work-item per heavy target: chunk partials added in chunk order.
+template <class P>
+__global__ __launch_bounds__(kBlock) void k_combine(typename P::Args a, StepConsts c,
+                                                    HeavyLists hl, int pass) {
+  const uint32_t* cnt = hl.counters + 4 * (pass & 1);
+  const uint32_t nrec = cnt[0];
+  const uint32_t stride = gridDim.x * kBlock;
+  for (uint32_t r = blockIdx.x * kBlock + threadIdx.x; r < nrec; r += stride) {
+    const uint4 rec = hl.recs[r];
+    const typename P::Target t = P::target(a, rec.x, c);
+    typename P::Acc s;
+    for (uint32_t k = 0; k < rec.z; ++k)
+      P::accumulate(
+          s, *reinterpret_cast<const typename P::Acc*>(hl.partials +
+                                                       (size_t)(rec.y + k) * kPartialWords));
+    P::finish(a, t, s, c);
+  }
+}
+
+int grid_for_items(long long items, int max_blocks) {
+  const long long b = (items + kBlock - 1) / kBlock;
+  return (int)(b < 1 ? 1 : (b > max_blocks ? max_blocks : b));
+}
+
+template <class P>
+void launch_pass(const typename P::Args& a, int n, const StepConsts& c, const uint2* ranges,
+                 const HeavyLists& hl, int pass, hipStream_t st) {
+  k_light<P><<<blocks_for(n), kBlock, 0, st>>>(a, n, c, ranges, hl, pass);
+  k_chunks<P><<<grid_for_items(hl.item_cap, 2048), kBlock, 0, st>>>(a, c, ranges, hl, pass);
+  k_combine<P><<<grid_for_items(n, 512), kBlock, 0, st>>>(a, c, hl, pass);
+}
+
+// ---------------------------------------------------------------------------
+// Streaming kernels.
+// ---------------------------------------------------------------------------
 
 // advect_kernel (fluids/Simulator_kernel.cuh:7-19) fused with getGridId
 // (fluids/Simulator.cu:56-73).  vel += dt*g and npos = pos + dt*vel as fused
 // multiply-adds; the updated velocity itself is not stored, since the
 // reference overwrites it before any read (fluids/Simulator.cu:267-274).
+// Also clears the heavy counter of the first neighbour pass.
 template <int L>
 __global__ __launch_bounds__(kBlock) void k_predict_key(const float* __restrict__ pos,
                                                         const float* __restrict__ vel, int n,
                                                         StepConsts c, float4* __restrict__ xp,
                                                         uint32_t* __restrict__ key,
-                                                        uint32_t* __restrict__ val) {
+                                                        uint32_t* __restrict__ val,
+                                                        uint32_t* __restrict__ counters) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < 4) counters[i] = 0u;
   if (i >= n) return;
   const float4 p = load_vec<L>(pos, i);
   const float4 v = load_vec<L>(vel, i);
@@ -55,8 +466,10 @@ template <int L>
 __global__ __launch_bounds__(kBlock) void k_key_only(const float* __restrict__ pos, int n,
                                                      StepConsts c, float4* __restrict__ xp,
                                                      uint32_t* __restrict__ key,
-                                                     uint32_t* __restrict__ val) {
+                                                     uint32_t* __restrict__ val,
+                                                     uint32_t* __restrict__ counters) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < 4) counters[i] = 0u;
   if (i >= n) return;
   const float4 p = load_vec<L>(pos, i);
   xp[i] = make_float4(p.x, p.y, p.z, 0.0f);
@@ -77,10 +490,25 @@ __global__ __launch_bounds__(kBlock) void k_clear_ranges(uint2* __restrict__ ran
   if ((cells & 1) && blockIdx.x == 0 && threadIdx.x == 0) ranges[cells - 1] = make_uint2(0u, 0u);
 }
 
+// computeGridRange (fluids/Simulator_kernel.cuh:21-50): a slot whose key
+// differs from the previous slot's opens its cell and closes the previous
+// one.  Keys < cells and slots < n hold for any sorted key array; the guards
+// keep a corrupted one from reaching out of bounds.
+__device__ __forceinline__ void mark_range(const uint32_t* __restrict__ skey, int s, int n,
+                                           int cells, uint2* __restrict__ ranges) {
+  const uint32_t k = skey[s];
+  const uint32_t prev = s > 0 ? skey[s - 1] : 0xffffffffu;
+  if (k < (uint32_t)cells) {
+    if (s == 0 || prev != k) {
+      ranges[k].x = (uint32_t)s;
+      if (s > 0 && prev < (uint32_t)cells) ranges[prev].y = (uint32_t)s;
+    }
+    if (s == n - 1) ranges[k].y = (uint32_t)n;
+  }
+}
+
 // Gather of the sorted payload (the zip permutation of
-// fluids/Simulator.cu:196-198) fused with computeGridRange
-// (fluids/Simulator_kernel.cuh:21-50): a slot whose key differs from the
-// previous slot's opens its cell and closes the previous one.
+// fluids/Simulator.cu:196-198) fused with the cell ranges.
 template <int L>
 __global__ __launch_bounds__(kBlock) void k_permute(
     const uint32_t* __restrict__ skey, const uint32_t* __restrict__ sval,
@@ -89,17 +517,7 @@ __global__ __launch_bounds__(kBlock) void k_permute(
     uint32_t* __restrict__ iid_s, uint2* __restrict__ ranges) {
   const int s = blockIdx.x * kBlock + threadIdx.x;
   if (s >= n) return;
-  const uint32_t k = skey[s];
-  const uint32_t prev = s > 0 ? skey[s - 1] : 0xffffffffu;
-  // keys < cells and indices < n hold for any sorted key/index array; the
-  // guards keep a corrupted one from reaching out of bounds
-  if (k < (uint32_t)cells) {
-    if (s == 0 || prev != k) {
-      ranges[k].x = (uint32_t)s;
-      if (s > 0 && prev < (uint32_t)cells) ranges[prev].y = (uint32_t)s;
-    }
-    if (s == n - 1) ranges[k].y = (uint32_t)n;
-  }
+  mark_range(skey, s, n, cells, ranges);
   const uint32_t src = sval[s] < (uint32_t)n ? sval[s] : 0u;
   x0s[s] = load_vec<L>(pos, src);
   xa[s] = xp[src];
@@ -113,101 +531,8 @@ __global__ __launch_bounds__(kBlock) void k_permute_pos(const uint32_t* __restri
                                                         uint2* __restrict__ ranges) {
   const int s = blockIdx.x * kBlock + threadIdx.x;
   if (s >= n) return;
-  const uint32_t k = skey[s];
-  const uint32_t prev = s > 0 ? skey[s - 1] : 0xffffffffu;
-  if (k < (uint32_t)cells) {
-    if (s == 0 || prev != k) {
-      ranges[k].x = (uint32_t)s;
-      if (s > 0 && prev < (uint32_t)cells) ranges[prev].y = (uint32_t)s;
-    }
-    if (s == n - 1) ranges[k].y = (uint32_t)n;
-  }
+  mark_range(skey, s, n, cells, ranges);
   xa[s] = xp[sval[s] < (uint32_t)n ? sval[s] : 0u];
-}
-
-// computeLambda (fluids/Simulator_kernel.cuh:52-129).  Query cell from the
-// current position over the grid of this frame; rho sums poly6 over every
-// candidate including i itself; the spiky gradient / pho0 is summed into
-// grad_i and, for j != i, its squared norm into gradj_l2.
-__global__ __launch_bounds__(kBlock) void k_lambda(const float4* __restrict__ xa,
-                                                   const uint2* __restrict__ ranges, int n,
-                                                   StepConsts c, float4* __restrict__ xl,
-                                                   float* __restrict__ rho_out) {
-  const int i = xcd_block_index() * kBlock + threadIdx.x;
-  if (i >= n) return;
-  const float4 xi = xa[i];
-  const Cell q = cell_of(xi.x, xi.y, xi.z, c);
-  float rho = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f, gl2 = 0.0f;
-  for_each_candidate(q, ranges, c, [&](uint32_t j) {
-    float dx, dy, dz;
-    const float r2 = dist2(xi, xa[j], dx, dy, dz);
-    if (r2 < c.h2_outer) {
-      if (r2 < c.h2) rho = rho + poly6_of(r2, c);
-      const float r = sqrtf(r2);
-      if (r >= c.keps && r < c.h) {
-        const float t = c.h - r;
-        const float s = ((c.spiky_rho0 * t) * t) / r;
-        const float ax = s * dx, ay = s * dy, az = s * dz;
-        gx = gx + ax;
-        gy = gy + ay;
-        gz = gz + az;
-        if (j != (uint32_t)i) {
-          gl2 = fmaf(ax, ax, gl2);
-          gl2 = fmaf(ay, ay, gl2);
-          gl2 = fmaf(az, az, gl2);
-        }
-      }
-    }
-  });
-  if (c.k_boundary != 0.0f) rho = rho + c.k_boundary * boundary_density(xi.x, xi.y, xi.z, c);
-  float denom = fmaf(gx, gx, gl2);
-  denom = fmaf(gy, gy, denom);
-  denom = fmaf(gz, gz, denom);
-  denom = denom + c.lambda_eps;
-  const float lam = -(rho / c.pho0 - 1.0f) / denom;
-  xl[i] = make_float4(xi.x, xi.y, xi.z, lam);
-  if (rho_out) rho_out[i] = rho;
-}
-
-// computetpos (fluids/Simulator_kernel.cuh:131-194): sum over j != i of
-// (lambda_i + lambda_j + s_corr) * spikyGrad, divided by pho0, clamped to
-// +-MAX_DP per axis, added, then clamped into the box shrunk by LIM_EPS.
-// Reads (x, lambda) from xl and writes the new iterate to xa (Jacobi; the
-// ping-pong replaces thrust::copy_n at fluids/Simulator.cu:247-248).
-__global__ __launch_bounds__(kBlock) void k_delta(const float4* __restrict__ xl,
-                                                  const uint2* __restrict__ ranges, int n,
-                                                  StepConsts c, float4* __restrict__ xa) {
-  const int i = xcd_block_index() * kBlock + threadIdx.x;
-  if (i >= n) return;
-  const float4 xi = xl[i];
-  const Cell q = cell_of(xi.x, xi.y, xi.z, c);
-  float px = 0.0f, py = 0.0f, pz = 0.0f;
-  for_each_candidate(q, ranges, c, [&](uint32_t j) {
-    if (j == (uint32_t)i) return;
-    const float4 xj = xl[j];
-    float dx, dy, dz;
-    const float r2 = dist2(xi, xj, dx, dy, dz);
-    if (r2 < c.h2_outer) {
-      const float r = sqrtf(r2);
-      if (r >= c.keps && r < c.h) {
-        const float w = r2 < c.h2 ? poly6_of(r2, c) : 0.0f;
-        const float corr = c.coef_corr * pow_corr(w, c);
-        const float t = c.h - r;
-        const float gs = ((c.spiky * t) * t) / r;
-        const float s = ((xi.w + xj.w) + corr) * gs;
-        px = fmaf(s, dx, px);
-        py = fmaf(s, dy, py);
-        pz = fmaf(s, dz, pz);
-      }
-    }
-  });
-  px = fminf(fmaxf(px / c.pho0, -c.max_dp), c.max_dp);
-  py = fminf(fmaxf(py / c.pho0, -c.max_dp), c.max_dp);
-  pz = fminf(fmaxf(pz / c.pho0, -c.max_dp), c.max_dp);
-  const float x = fmaxf(fminf(xi.x + px, c.hix), c.lox);
-  const float y = fmaxf(fminf(xi.y + py, c.hiy), c.loy);
-  const float z = fmaxf(fminf(xi.z + pz, c.hiz), c.loz);
-  xa[i] = make_float4(x, y, z, 0.0f);
 }
 
 // h_updateVelocity (fluids/Simulator.cu:127-137, :267-274):
@@ -233,66 +558,28 @@ __global__ __launch_bounds__(kBlock) void k_velocity(
   iid[s] = iid_s[s];
 }
 
-// computeXSPH (fluids/Simulator_kernel.cuh:196-239): nvel_i = v_i +
-// c * sum_j 2 (v_j - v_i) W(|x_i - x_j|^2) / (rho_i + rho_j), j == i
-// included (its term is 0), rho of the last lambda pass.  Pairs with W == 0
-// are skipped: their reference term is exactly 0.
-template <int L>
-__global__ __launch_bounds__(kBlock) void k_xsph(const float4* __restrict__ xa,
-                                                 const float4* __restrict__ vr,
-                                                 const uint2* __restrict__ ranges, int n,
-                                                 StepConsts c, float* __restrict__ npos,
-                                                 float* __restrict__ nvel) {
-  const int i = xcd_block_index() * kBlock + threadIdx.x;
-  if (i >= n) return;
-  const float4 xi = xa[i];
-  const float4 vi = vr[i];
-  const Cell q = cell_of(xi.x, xi.y, xi.z, c);
-  float ax = 0.0f, ay = 0.0f, az = 0.0f;
-  for_each_candidate(q, ranges, c, [&](uint32_t j) {
-    float dx, dy, dz;
-    const float r2 = dist2(xi, xa[j], dx, dy, dz);
-    if (r2 < c.h2) {
-      const float w = poly6_of(r2, c);
-      const float4 vj = vr[j];
-      const float f = (2.0f * w) / (vi.w + vj.w);
-      ax = fmaf(vj.x - vi.x, f, ax);
-      ay = fmaf(vj.y - vi.y, f, ay);
-      az = fmaf(vj.z - vi.z, f, az);
+constexpr int kRedBlock = 256;
+constexpr int kRedItems = 16;  // particles per work-item in the first reduction
+
+// Deterministic two-level reduction of the per-particle errors: blocks of
+// kRedBlock * kRedItems particles to (sum, max) partials, then one block.
+__global__ __launch_bounds__(kRedBlock) void k_reduce_err(const double* __restrict__ err, int n,
+                                                          double* __restrict__ partial) {
+  __shared__ double ssum[kRedBlock];
+  __shared__ double smax[kRedBlock];
+  const int base = blockIdx.x * kRedBlock * kRedItems;
+  double s = 0.0, m = 0.0;
+  for (int k = 0; k < kRedItems; ++k) {
+    const int i = base + k * kRedBlock + (int)threadIdx.x;
+    if (i < n) {
+      s += err[i];
+      m = fmax(m, err[i]);
     }
-  });
-  store_vec<L>(nvel, i, fmaf(c.c_xsph, ax, vi.x), fmaf(c.c_xsph, ay, vi.y),
-               fmaf(c.c_xsph, az, vi.z));
-  store_vec<L>(npos, i, xi.x, xi.y, xi.z);
-}
-
-constexpr int kDensBlock = 256;
-
-// |rho_i/pho0 - 1| on a fresh grid, block partial sums and maxima in FP64.
-__global__ __launch_bounds__(kDensBlock) void k_density_error(const float4* __restrict__ xa,
-                                                              const uint2* __restrict__ ranges,
-                                                              int n, StepConsts c,
-                                                              double* __restrict__ partial) {
-  __shared__ double ssum[kDensBlock];
-  __shared__ double smax[kDensBlock];
-  const int i = blockIdx.x * kDensBlock + threadIdx.x;
-  double e = 0.0;
-  if (i < n) {
-    const float4 xi = xa[i];
-    const Cell q = cell_of(xi.x, xi.y, xi.z, c);
-    float rho = 0.0f;
-    for_each_candidate(q, ranges, c, [&](uint32_t j) {
-      float dx, dy, dz;
-      const float r2 = dist2(xi, xa[j], dx, dy, dz);
-      if (r2 < c.h2) rho = rho + poly6_of(r2, c);
-    });
-    if (c.k_boundary != 0.0f) rho = rho + c.k_boundary * boundary_density(xi.x, xi.y, xi.z, c);
-    e = fabs((double)rho / (double)c.pho0 - 1.0);
   }
-  ssum[threadIdx.x] = e;
-  smax[threadIdx.x] = e;
+  ssum[threadIdx.x] = s;
+  smax[threadIdx.x] = m;
   __syncthreads();
-  for (int off = kDensBlock / 2; off > 0; off >>= 1) {
+  for (int off = kRedBlock / 2; off > 0; off >>= 1) {
     if ((int)threadIdx.x < off) {
       ssum[threadIdx.x] += ssum[threadIdx.x + off];
       smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + off]);
@@ -305,21 +592,20 @@ __global__ __launch_bounds__(kDensBlock) void k_density_error(const float4* __re
   }
 }
 
-// Fixed-order reduction of the block partials (deterministic).
-__global__ __launch_bounds__(kDensBlock) void k_reduce_partials(const double* __restrict__ partial,
-                                                                int nb, int n,
-                                                                double* __restrict__ out) {
-  __shared__ double ssum[kDensBlock];
-  __shared__ double smax[kDensBlock];
+__global__ __launch_bounds__(kRedBlock) void k_reduce_partials(const double* __restrict__ partial,
+                                                               int nb, int n,
+                                                               double* __restrict__ out) {
+  __shared__ double ssum[kRedBlock];
+  __shared__ double smax[kRedBlock];
   double s = 0.0, m = 0.0;
-  for (int b = threadIdx.x; b < nb; b += kDensBlock) {
+  for (int b = threadIdx.x; b < nb; b += kRedBlock) {
     s += partial[2 * b];
     m = fmax(m, partial[2 * b + 1]);
   }
   ssum[threadIdx.x] = s;
   smax[threadIdx.x] = m;
   __syncthreads();
-  for (int off = kDensBlock / 2; off > 0; off >>= 1) {
+  for (int off = kRedBlock / 2; off > 0; off >>= 1) {
     if ((int)threadIdx.x < off) {
       ssum[threadIdx.x] += ssum[threadIdx.x + off];
       smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + off]);
@@ -335,19 +621,20 @@ __global__ __launch_bounds__(kDensBlock) void k_reduce_partials(const double* __
 }  // namespace
 
 void launch_predict_key(int layout, const float* pos, const float* vel, int n, const StepConsts& c,
-                        float4* xp, uint32_t* key, uint32_t* val, hipStream_t st) {
+                        float4* xp, uint32_t* key, uint32_t* val, uint32_t* counters,
+                        hipStream_t st) {
   if (layout == 3)
-    k_predict_key<3><<<blocks_for(n), kBlock, 0, st>>>(pos, vel, n, c, xp, key, val);
+    k_predict_key<3><<<blocks_for(n), kBlock, 0, st>>>(pos, vel, n, c, xp, key, val, counters);
   else
-    k_predict_key<4><<<blocks_for(n), kBlock, 0, st>>>(pos, vel, n, c, xp, key, val);
+    k_predict_key<4><<<blocks_for(n), kBlock, 0, st>>>(pos, vel, n, c, xp, key, val, counters);
 }
 
 void launch_key_only(int layout, const float* pos, int n, const StepConsts& c, float4* xp,
-                     uint32_t* key, uint32_t* val, hipStream_t st) {
+                     uint32_t* key, uint32_t* val, uint32_t* counters, hipStream_t st) {
   if (layout == 3)
-    k_key_only<3><<<blocks_for(n), kBlock, 0, st>>>(pos, n, c, xp, key, val);
+    k_key_only<3><<<blocks_for(n), kBlock, 0, st>>>(pos, n, c, xp, key, val, counters);
   else
-    k_key_only<4><<<blocks_for(n), kBlock, 0, st>>>(pos, n, c, xp, key, val);
+    k_key_only<4><<<blocks_for(n), kBlock, 0, st>>>(pos, n, c, xp, key, val, counters);
 }
 
 void launch_clear_ranges(uint2* ranges, long long cells, hipStream_t st) {
@@ -374,13 +661,13 @@ void launch_permute_pos(const uint32_t* skey, const uint32_t* sval, const float4
 }
 
 void launch_lambda(const float4* xa, const uint2* ranges, int n, const StepConsts& c, float4* xl,
-                   float* rho, hipStream_t st) {
-  k_lambda<<<blocks_for(n), kBlock, 0, st>>>(xa, ranges, n, c, xl, rho);
+                   float* rho, const HeavyLists& hl, int pass, hipStream_t st) {
+  launch_pass<LambdaPass>(LambdaPass::Args{xa, xl, rho}, n, c, ranges, hl, pass, st);
 }
 
 void launch_delta(const float4* xl, const uint2* ranges, int n, const StepConsts& c, float4* xa,
-                  hipStream_t st) {
-  k_delta<<<blocks_for(n), kBlock, 0, st>>>(xl, ranges, n, c, xa);
+                  const HeavyLists& hl, int pass, hipStream_t st) {
+  launch_pass<DeltaPass>(DeltaPass::Args{xl, xa}, n, c, ranges, hl, pass, st);
 }
 
 void launch_velocity(int layout, const float4* xa, const float4* x0s, const uint32_t* iid_s,
@@ -393,20 +680,23 @@ void launch_velocity(int layout, const float4* xa, const float4* x0s, const uint
 }
 
 void launch_xsph(int layout, const float4* xa, const float4* vr, const uint2* ranges, int n,
-                 const StepConsts& c, float* npos, float* nvel, hipStream_t st) {
+                 const StepConsts& c, float* npos, float* nvel, const HeavyLists& hl, int pass,
+                 hipStream_t st) {
   if (layout == 3)
-    k_xsph<3><<<blocks_for(n), kBlock, 0, st>>>(xa, vr, ranges, n, c, npos, nvel);
+    launch_pass<XsphPass<3>>(XsphPass<3>::Args{xa, vr, npos, nvel}, n, c, ranges, hl, pass, st);
   else
-    k_xsph<4><<<blocks_for(n), kBlock, 0, st>>>(xa, vr, ranges, n, c, npos, nvel);
+    launch_pass<XsphPass<4>>(XsphPass<4>::Args{xa, vr, npos, nvel}, n, c, ranges, hl, pass, st);
 }
 
-int density_blocks(int n) { return (n + kDensBlock - 1) / kDensBlock; }
+int density_partials(int n) { return (n + kRedBlock * kRedItems - 1) / (kRedBlock * kRedItems); }
 
 void launch_density_error(const float4* xa, const uint2* ranges, int n, const StepConsts& c,
-                          double* partial, double* out, hipStream_t st) {
-  const int nb = density_blocks(n);
-  k_density_error<<<nb, kDensBlock, 0, st>>>(xa, ranges, n, c, partial);
-  k_reduce_partials<<<1, kDensBlock, 0, st>>>(partial, nb, n, out);
+                          double* err, double* partial, double* out, const HeavyLists& hl,
+                          hipStream_t st) {
+  launch_pass<DensityPass>(DensityPass::Args{xa, err}, n, c, ranges, hl, 0, st);
+  const int nb = density_partials(n);
+  k_reduce_err<<<nb, kRedBlock, 0, st>>>(err, n, partial);
+  k_reduce_partials<<<1, kRedBlock, 0, st>>>(partial, nb, n, out);
 }
 
 }  // namespace pbf

@@ -123,6 +123,37 @@ int pbf_step(pbf_solver* s, float* pos, float* npos, float* vel, float* nvel,
              uint32_t* iid, int n, void* stream);
 
 /*
+ * Sweeping wall of the reference's FluidSystem (fluids/FluidSystem.cpp:36-38,
+ * :104-110): for frame f >= start_frame,
+ *   phi = sinf(w * (f - start_frame)),
+ *   ulim = ulim0 + a_ulim * phi,  llim = llim0 + a_llim * phi  (FP32);
+ * before start_frame the box is (ulim0, llim0).
+ */
+typedef struct pbf_sweep {
+  pbf_vec3 ulim0, llim0; /* resting box */
+  pbf_vec3 a_ulim, a_llim; /* amplitudes */
+  float w;                 /* radians per frame */
+  int start_frame;
+} pbf_sweep;
+
+/* The box of frame `frame` under `sweep`.  Host only (no device call). */
+int pbf_sweep_box(const pbf_sweep* sweep, int frame, pbf_vec3* ulim, pbf_vec3* llim);
+
+/*
+ * FluidSystem::stepSimulate repeated for `nframes` frames
+ * (fluids/FluidSystem.cpp:99-120): per frame, when sweep != NULL the box is
+ * set to pbf_sweep_box(sweep, frame) (as pbf_set_lim), then one pbf_step
+ * with tic-toc buffers: frames first_frame + 2k step (a -> b), the others
+ * (b -> a).  *which receives 0 if the newest state is in (pos_a, vel_a),
+ * 1 if in (pos_b, vel_b).  Stream-ordered, does not synchronise.  Stops at
+ * the first failing frame (its status is returned; *which then names the
+ * newest completed state).
+ */
+int pbf_run_frames(pbf_solver* s, float* pos_a, float* vel_a, float* pos_b, float* vel_b,
+                   uint32_t* iid, int n, int first_frame, int nframes, const pbf_sweep* sweep,
+                   void* stream, int* which);
+
+/*
  * Density error of a particle set: mean and max over particles of
  * |rho_i/pho0 - 1|, with rho_i recomputed on `pos` by a fresh grid (self
  * included, same kernel and wall term as the lambda pass,

@@ -71,7 +71,10 @@ struct pbf_solver {
   uint32_t* counters = nullptr;  // heavy-target work list counters
   pbf::HeavyLists hl{};
   double *dens_err = nullptr, *dens_partial = nullptr, *dens_out = nullptr;
-  uint2* ranges = nullptr;
+  uint32_t* cnt = nullptr;       // particles per cell, all zero between steps
+  uint32_t* cs = nullptr;        // cell-start table (max_cells + 1)
+  uint32_t* tile_sum = nullptr;  // scan partials
+  bool cnt_dirty = false;        // a step failed between counting and the scan
 
   Timer timer;
   std::string err;
@@ -331,7 +334,9 @@ int pbf_create(const pbf_params* params, pbf_vec3 ulim, pbf_vec3 llim, int capac
                o_xa = carve(16 * N), o_rho = carve(4 * N), o_recs = carve(16 * N),
                o_items = carve(8 * ICAP), o_part = carve(4 * pbf::kPartialWords * ICAP),
                o_cnt = carve(4 * 8), o_de = carve(8 * N), o_dp = carve(16 * dblk),
-               o_do = carve(16), o_rng = carve(8 * (size_t)max_cells);
+               o_do = carve(16), o_cnt2 = carve(4 * (size_t)max_cells),
+               o_cs = carve(4 * ((size_t)max_cells + 1)),
+               o_ts = carve(4 * ((size_t)pbf::scan_tiles(max_cells) + 1));
   void* arena = nullptr;
   if ((e = hipMalloc(&arena, off)) != hipSuccess) {
     delete s;
@@ -359,7 +364,16 @@ int pbf_create(const pbf_params* params, pbf_vec3 ulim, pbf_vec3 llim, int capac
   s->dens_err = (double*)(b + o_de);
   s->dens_partial = (double*)(b + o_dp);
   s->dens_out = (double*)(b + o_do);
-  s->ranges = (uint2*)(b + o_rng);
+  s->cnt = (uint32_t*)(b + o_cnt2);
+  s->cs = (uint32_t*)(b + o_cs);
+  s->tile_sum = (uint32_t*)(b + o_ts);
+  // the per-cell counts start (and stay, between steps) all zero
+  pbf::launch_fill_u32(s->cnt, max_cells, 0u, nullptr);
+  if ((e = hipDeviceSynchronize()) != hipSuccess) {
+    (void)hipFree(arena);
+    delete s;
+    return fail_hip(nullptr, e, "clearing the cell counts");
+  }
   *out = s;
   return PBF_OK;
 }
@@ -448,20 +462,20 @@ int pbf_step(pbf_solver* s, float* pos, float* npos, float* vel, float* nvel, ui
   tbegin(s, PBF_PHASE_STEP, st);
   // advect() + getGridId (Simulator.cpp:50, Simulator.cu:165-193)
   tbegin(s, PBF_PHASE_PREDICT, st);
-  pbf::launch_predict_key(L, pos, vel, n, c, s->xp, s->key0, s->val0, s->counters, st);
+  if (s->cnt_dirty) pbf::launch_fill_u32(s->cnt, s->max_cells, 0u, st);
+  s->cnt_dirty = true;
+  pbf::launch_predict_key(L, pos, vel, n, c, s->xp, s->key0, s->val0, s->cnt, s->counters, st);
   tend(s, PBF_PHASE_PREDICT, st);
   // thrust::sort_by_key (Simulator.cu:196-198)
   tbegin(s, PBF_PHASE_SORT, st);
   const int which = pbf::radix_sort_pairs(s->key0, s->val0, s->key1, s->val1, n,
                                           key_bits_for(c.cells), s->hist, s->totals, st);
   tend(s, PBF_PHASE_SORT, st);
-  const uint32_t* skey = which ? s->key1 : s->key0;
   const uint32_t* sval = which ? s->val1 : s->val0;
   // memset + computeGridRange (Simulator.cu:200-204), fused with the payload gather
   tbegin(s, PBF_PHASE_GRID, st);
-  pbf::launch_clear_ranges(s->ranges, c.cells, st);
-  pbf::launch_permute(L, skey, sval, pos, s->xp, iid, n, c.cells, s->x0s, s->xa, s->iid_s,
-                      s->ranges, st);
+  pbf::launch_cell_starts(s->cnt, c.cells, s->tile_sum, s->cs, st);
+  pbf::launch_permute(L, sval, pos, s->xp, iid, n, s->x0s, s->xa, s->iid_s, st);
   tend(s, PBF_PHASE_GRID, st);
   // niter x correctDensity (Simulator.cpp:62-66, Simulator.cu:213-249);
   // xl (x, lambda) aliases xp, which is dead after the permute.
@@ -469,11 +483,11 @@ int pbf_step(pbf_solver* s, float* pos, float* npos, float* vel, float* nvel, ui
   int pass = 0;
   for (int it = 0; it < s->prm.niter; ++it) {
     tbegin(s, PBF_PHASE_LAMBDA, st);
-    pbf::launch_lambda(s->xa, s->ranges, n, c, xl, it + 1 == s->prm.niter ? s->rho : nullptr,
+    pbf::launch_lambda(s->xa, s->cs, n, c, xl, it + 1 == s->prm.niter ? s->rho : nullptr,
                        s->hl, pass++, st);
     tend(s, PBF_PHASE_LAMBDA, st);
     tbegin(s, PBF_PHASE_DELTA, st);
-    pbf::launch_delta(xl, s->ranges, n, c, s->xa, s->hl, pass++, st);
+    pbf::launch_delta(xl, s->cs, n, c, s->xa, s->hl, pass++, st);
     tend(s, PBF_PHASE_DELTA, st);
   }
   // updateVelocity (Simulator.cpp:72, Simulator.cu:267-274); vr aliases xp.
@@ -483,12 +497,55 @@ int pbf_step(pbf_solver* s, float* pos, float* npos, float* vel, float* nvel, ui
   tend(s, PBF_PHASE_VEL, st);
   // correctVelocity (Simulator.cpp:77, Simulator.cu:251-265)
   tbegin(s, PBF_PHASE_XSPH, st);
-  pbf::launch_xsph(L, s->xa, vr, s->ranges, n, c, npos, nvel, s->hl, pass, st);
+  pbf::launch_xsph(L, s->xa, vr, s->cs, n, c, npos, nvel, s->hl, pass, st);
   tend(s, PBF_PHASE_XSPH, st);
   tend(s, PBF_PHASE_STEP, st);
 
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail_hip(s, e, "kernel launch");
+  s->cnt_dirty = false;
+  return PBF_OK;
+}
+
+int pbf_sweep_box(const pbf_sweep* sw, int frame, pbf_vec3* ulim, pbf_vec3* llim) {
+  if (!sw || !ulim || !llim) return fail(nullptr, PBF_ERR_INVALID_ARGUMENT, "NULL argument");
+  float phi = 0.0f;
+  if (frame >= sw->start_frame) {
+    // FluidSystem::stepSimulate (fluids/FluidSystem.cpp:107-109), FP32
+    const float t = sw->w * (float)(frame - sw->start_frame);
+    phi = std::sin(t);
+  }
+  ulim->x = sw->ulim0.x + sw->a_ulim.x * phi;
+  ulim->y = sw->ulim0.y + sw->a_ulim.y * phi;
+  ulim->z = sw->ulim0.z + sw->a_ulim.z * phi;
+  llim->x = sw->llim0.x + sw->a_llim.x * phi;
+  llim->y = sw->llim0.y + sw->a_llim.y * phi;
+  llim->z = sw->llim0.z + sw->a_llim.z * phi;
+  return PBF_OK;
+}
+
+int pbf_run_frames(pbf_solver* s, float* pos_a, float* vel_a, float* pos_b, float* vel_b,
+                   uint32_t* iid, int n, int first_frame, int nframes, const pbf_sweep* sweep,
+                   void* stream, int* which) {
+  if (!s) return fail(nullptr, PBF_ERR_INVALID_ARGUMENT, "solver is NULL");
+  if (!which) return fail(s, PBF_ERR_INVALID_ARGUMENT, "which is NULL");
+  *which = 0;
+  if (nframes < 0) return fail(s, PBF_ERR_INVALID_ARGUMENT, "nframes must be >= 0");
+  for (int k = 0; k < nframes; ++k) {
+    const int frame = first_frame + k;
+    if (sweep) {
+      pbf_vec3 ul, ll;
+      pbf_sweep_box(sweep, frame, &ul, &ll);
+      const int rc = pbf_set_lim(s, ul, ll);
+      if (rc) return rc;
+    }
+    // tic-toc of fluids/FluidSystem.cpp:112-117
+    const bool tic = (k & 1) == 0;
+    const int rc = tic ? pbf_step(s, pos_a, pos_b, vel_a, vel_b, iid, n, stream)
+                       : pbf_step(s, pos_b, pos_a, vel_b, vel_a, iid, n, stream);
+    if (rc) return rc;
+    *which = tic ? 1 : 0;
+  }
   return PBF_OK;
 }
 
@@ -509,16 +566,18 @@ int pbf_density_error(pbf_solver* s, const float* pos, int n, void* stream, doub
   DeviceGuard g(s->device);
   if (!g.ok) return fail(s, PBF_ERR_HIP, "hipSetDevice(%d) failed", s->device);
   hipStream_t st = (hipStream_t)stream;
-  pbf::launch_key_only(s->layout, pos, n, c, s->xp, s->key0, s->val0, s->counters, st);
+  if (s->cnt_dirty) pbf::launch_fill_u32(s->cnt, s->max_cells, 0u, st);
+  s->cnt_dirty = true;
+  pbf::launch_key_only(s->layout, pos, n, c, s->xp, s->key0, s->val0, s->cnt, s->counters, st);
   const int which = pbf::radix_sort_pairs(s->key0, s->val0, s->key1, s->val1, n,
                                           key_bits_for(c.cells), s->hist, s->totals, st);
-  pbf::launch_clear_ranges(s->ranges, c.cells, st);
-  pbf::launch_permute_pos(which ? s->key1 : s->key0, which ? s->val1 : s->val0, s->xp, n,
-                          c.cells, s->xa, s->ranges, st);
-  pbf::launch_density_error(s->xa, s->ranges, n, c, s->dens_err, s->dens_partial, s->dens_out,
+  pbf::launch_cell_starts(s->cnt, c.cells, s->tile_sum, s->cs, st);
+  pbf::launch_permute_pos(which ? s->val1 : s->val0, s->xp, n, s->xa, st);
+  pbf::launch_density_error(s->xa, s->cs, n, c, s->dens_err, s->dens_partial, s->dens_out,
                             s->hl, st);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail_hip(s, e, "kernel launch");
+  s->cnt_dirty = false;
   double host[2];
   if ((e = hipMemcpyAsync(host, s->dens_out, sizeof(host), hipMemcpyDeviceToHost, st)) != hipSuccess)
     return fail_hip(s, e, "hipMemcpyAsync");

@@ -73,31 +73,23 @@ __device__ __forceinline__ uint32_t cell_id(const Cell& q, const StepConsts& c) 
 // Candidate neighbours of a particle whose query cell is q, in the
 // reference's order (fluids/Simulator_kernel.cuh:78-88): dx, then dy, then
 // dz, j ascending.  The three cells (x,y,z-1..z+1) of one column are
-// consecutive cell ids, so their particles form one contiguous sorted range:
-// [first start, last end) over the column's non-empty cells.  Walking that
-// range is the same j sequence as walking the three cells in turn.
-// column_range(q, k) is column k = 3*(dx+1) + (dy+1); empty -> [0,0).
+// consecutive cell ids, so their particles form one contiguous sorted range,
+// read from the cell-start table cs (cells + 1 entries; cs[c] = first sorted
+// slot whose cell id is >= c): [cs[first cell], cs[last cell + 1]).  Walking
+// that range is the same j sequence as walking the three cells in turn, and
+// costs two neighbouring loads instead of three cell lookups.
+// column_range(q, k) is column k = 3*(dx+1) + (dy+1); outside the grid or
+// empty, it is an empty range.
 __device__ __forceinline__ uint2 column_range(const Cell& q, int k,
-                                              const uint2* __restrict__ ranges,
+                                              const uint32_t* __restrict__ cs,
                                               const StepConsts& c) {
   const int X = q.x + k / 3 - 1;
   const int Y = q.y + k % 3 - 1;
-  uint32_t b = 0xffffffffu, e = 0u;
-  if (X >= 0 && X < c.dx && Y >= 0 && Y < c.dy) {
-    const int col = (X * c.dy + Y) * c.dz;
-#pragma unroll
-    for (int oz = -1; oz <= 1; ++oz) {
-      const int z = q.z + oz;
-      if (z >= 0 && z < c.dz) {
-        const uint2 r = ranges[col + z];
-        if (r.x < r.y) {
-          b = r.x < b ? r.x : b;
-          e = r.y > e ? r.y : e;
-        }
-      }
-    }
-  }
-  return b < e ? make_uint2(b, e) : make_uint2(0u, 0u);
+  if (X < 0 || X >= c.dx || Y < 0 || Y >= c.dy) return make_uint2(0u, 0u);
+  const int col = (X * c.dy + Y) * c.dz;
+  const int z0 = q.z > 0 ? q.z - 1 : 0;
+  const int z1 = q.z + 1 < c.dz ? q.z + 1 : c.dz - 1;
+  return make_uint2(cs[col + z0], cs[col + z1 + 1]);
 }
 
 // Canonical summation order, shared with the host oracle's FP32 mirror: a
@@ -189,19 +181,116 @@ __device__ __forceinline__ void sweep_filtered(uint32_t j0, uint32_t j1, Load&& 
   }
 }
 
-// Positions [s, t) of the candidate sequence given the nine column ranges.
-template <class Load, class Hit, class Add>
-__device__ __forceinline__ void sweep_seq(const uint2 (&col)[9], uint32_t s, uint32_t t, Load&& load,
-                                          Hit&& hit, Add&& add) {
-  uint32_t cum = 0;
+// A work-item's candidate sequence as a compact column table in LDS: entry
+// k < nk is (first slot of the k-th non-empty column, sequence position of
+// its first candidate); entry nk is (0, m).  `tab` points at entry 0 of this
+// work-item and `stride` separates entries (LDS laid out [entry][lane] so
+// lanes touch consecutive words).
+struct SeqTable {
+  uint2* tab;
+  int stride;
+  __device__ __forceinline__ uint2 at(int k) const { return tab[k * stride]; }
+};
+
+// Builds the table from the nine column ranges around query cell q and
+// returns the candidate count m.
+__device__ __forceinline__ uint32_t build_seq(const SeqTable& st, const Cell& q,
+                                              const uint32_t* __restrict__ cs,
+                                              const StepConsts& c) {
+  uint32_t m = 0;
+  int nk = 0;
 #pragma unroll
   for (int k = 0; k < 9; ++k) {
-    const uint32_t len = col[k].y - col[k].x;
-    const uint32_t lo = s > cum ? s - cum : 0u;
-    uint32_t hi = t > cum ? t - cum : 0u;
-    hi = hi < len ? hi : len;
-    if (lo < hi) sweep_filtered(col[k].x + lo, col[k].x + hi, load, hit, add);
-    cum += len;
+    const uint2 r = column_range(q, k, cs, c);
+    if (r.y > r.x) {
+      st.tab[nk * st.stride] = make_uint2(r.x, m);
+      ++nk;
+      m += r.y - r.x;
+    }
+  }
+  st.tab[nk * st.stride] = make_uint2(0u, m);
+  return m;
+}
+
+// Maps increasing sequence positions to sorted slots (u must be < m and
+// must not decrease between calls).
+struct SeqCursor {
+  SeqTable st;
+  int k;
+  uint32_t b, cum, next;
+  __device__ __forceinline__ void init(const SeqTable& t, uint32_t u) {
+    st = t;
+    k = 0;
+    uint2 e0 = st.at(0), e1 = st.at(1);
+    while (u >= e1.y) {  // only when starting inside the sequence
+      ++k;
+      e0 = e1;
+      e1 = st.at(k + 1);
+    }
+    b = e0.x;
+    cum = e0.y;
+    next = e1.y;
+  }
+  __device__ __forceinline__ uint32_t slot(uint32_t u) {
+    while (u >= next) {
+      ++k;
+      const uint2 e0 = st.at(k);
+      b = e0.x;
+      cum = e0.y;
+      next = st.at(k + 1).y;
+    }
+    return b + (u - cum);
+  }
+};
+
+// Visits sequence positions [u0, u1) of a candidate sequence in order,
+// calling add() only for candidates passing the cheap test hit().  Blocks of
+// 32 positions are screened eight loads at a time (across column
+// boundaries, so a sparse neighbourhood costs a few memory round trips, not
+// one or two per column), then the hits are reloaded four at a time and
+// accumulated in ascending order.  Candidates failing hit() contribute
+// exactly nothing in add(), so the sums equal the plain in-order sum.
+template <class LoadS, class LoadF, class Hit, class Add>
+__device__ __forceinline__ void sweep_seq(const SeqTable& st, uint32_t u0, uint32_t u1,
+                                          LoadS&& load_s, LoadF&& load_f, Hit&& hit, Add&& add) {
+  if (u0 >= u1) return;
+  SeqCursor c1, c2;
+  c1.init(st, u0);
+  c2 = c1;
+#pragma unroll 1
+  for (uint32_t bb = u0; bb < u1; bb += 32) {
+    const uint32_t be = u1 - bb < 32u ? u1 : bb + 32u;
+    uint32_t mask = 0;
+#pragma unroll 1
+    for (uint32_t u = bb; u < be; u += 8) {
+      uint32_t j[8];
+#pragma unroll
+      for (int t = 0; t < 8; ++t) j[t] = c1.slot(u + t < be ? u + t : be - 1);
+      decltype(load_s(0u)) d[8];
+#pragma unroll
+      for (int t = 0; t < 8; ++t) d[t] = load_s(j[t]);
+#pragma unroll
+      for (int t = 0; t < 8; ++t)
+        if (u + t < be && hit(j[t], d[t])) mask |= 1u << (u - bb + t);
+    }
+#pragma unroll 1
+    while (mask) {
+      uint32_t j[4];
+      bool h[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        h[t] = mask != 0u;
+        const uint32_t bit = h[t] ? (uint32_t)__builtin_ctz(mask) : 0u;
+        mask &= mask - 1u;
+        j[t] = h[t] ? c2.slot(bb + bit) : 0u;
+      }
+      decltype(load_f(0u)) d[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) d[t] = load_f(h[t] ? j[t] : j[0]);
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+        if (h[t]) add(j[t], d[t]);
+    }
   }
 }
 

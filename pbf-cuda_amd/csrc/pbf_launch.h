@@ -22,29 +22,35 @@ int radix_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, int
 // numbers the neighbour passes of one step from 0 (predict / key_only clear
 // the parity-0 counters, each light kernel clears the other parity's).
 void launch_predict_key(int layout, const float* pos, const float* vel, int n, const StepConsts& c,
-                        float4* xp, uint32_t* key, uint32_t* val, uint32_t* counters,
-                        hipStream_t st);
+                        float4* xp, uint32_t* key, uint32_t* val, uint32_t* cnt,
+                        uint32_t* counters, hipStream_t st);
 void launch_key_only(int layout, const float* pos, int n, const StepConsts& c, float4* xp,
-                     uint32_t* key, uint32_t* val, uint32_t* counters, hipStream_t st);
-void launch_clear_ranges(uint2* ranges, long long cells, hipStream_t st);
-void launch_permute(int layout, const uint32_t* skey, const uint32_t* sval, const float* pos,
-                    const float4* xp, const uint32_t* iid, int n, int cells, float4* x0s,
-                    float4* xa, uint32_t* iid_s, uint2* ranges, hipStream_t st);
-void launch_permute_pos(const uint32_t* skey, const uint32_t* sval, const float4* xp, int n,
-                        int cells, float4* xa, uint2* ranges, hipStream_t st);
-void launch_lambda(const float4* xa, const uint2* ranges, int n, const StepConsts& c, float4* xl,
+                     uint32_t* key, uint32_t* val, uint32_t* cnt, uint32_t* counters,
+                     hipStream_t st);
+void launch_fill_u32(uint32_t* a, long long n, uint32_t v, hipStream_t st);
+// Cell-start table from the per-cell counts (which it clears): cs holds
+// cells + 1 entries, tile_sum scan_tiles(cells) entries.
+long long scan_tiles(long long cells);
+void launch_cell_starts(uint32_t* cnt, long long cells, uint32_t* tile_sum, uint32_t* cs,
+                        hipStream_t st);
+void launch_permute(int layout, const uint32_t* sval, const float* pos, const float4* xp,
+                    const uint32_t* iid, int n, float4* x0s, float4* xa, uint32_t* iid_s,
+                    hipStream_t st);
+void launch_permute_pos(const uint32_t* sval, const float4* xp, int n, float4* xa,
+                        hipStream_t st);
+void launch_lambda(const float4* xa, const uint32_t* cs, int n, const StepConsts& c, float4* xl,
                    float* rho, const HeavyLists& hl, int pass, hipStream_t st);
-void launch_delta(const float4* xl, const uint2* ranges, int n, const StepConsts& c, float4* xa,
+void launch_delta(const float4* xl, const uint32_t* cs, int n, const StepConsts& c, float4* xa,
                   const HeavyLists& hl, int pass, hipStream_t st);
 void launch_velocity(int layout, const float4* xa, const float4* x0s, const uint32_t* iid_s,
                      const float* rho, int n, const StepConsts& c, float4* vr, float* pos,
                      float* vel, uint32_t* iid, hipStream_t st);
-void launch_xsph(int layout, const float4* xa, const float4* vr, const uint2* ranges, int n,
+void launch_xsph(int layout, const float4* xa, const float4* vr, const uint32_t* cs, int n,
                  const StepConsts& c, float* npos, float* nvel, const HeavyLists& hl, int pass,
                  hipStream_t st);
 // err holds n doubles, partial 2*density_partials(n), out (device) 2.
 int density_partials(int n);
-void launch_density_error(const float4* xa, const uint2* ranges, int n, const StepConsts& c,
+void launch_density_error(const float4* xa, const uint32_t* cs, int n, const StepConsts& c,
                           double* err, double* partial, double* out, const HeavyLists& hl,
                           hipStream_t st);
 

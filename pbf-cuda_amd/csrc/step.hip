@@ -9,7 +9,9 @@
 //                position and lambda (aliases xp)
 //   vr   float4  (velocity before XSPH, rho): gathered by XSPH (aliases xp)
 //   rho  float   density of the last lambda pass
-//   ranges uint2 per cell: [start, end) of its particles, 0/0 when empty
+//   cnt  uint32 per cell: particles per cell (kept all-zero between steps)
+//   cs   uint32 per cell + 1: first sorted slot of each cell (exclusive scan
+//        of cnt); a cell's particles are [cs[c], cs[c+1])
 //
 // Each neighbour pass (lambda, delta-p, XSPH, density) is three kernels:
 //   light  one work-item per target over the cell-sorted order; workgroups
@@ -56,7 +58,8 @@ struct LambdaPass {
   static __device__ __forceinline__ Target target(const Args& a, uint32_t i, const StepConsts&) {
     return Target{a.x[i], i};
   }
-  static __device__ __forceinline__ float4 load(const Args& a, uint32_t j) { return a.x[j]; }
+  static __device__ __forceinline__ float4 load_s(const Args& a, uint32_t j) { return a.x[j]; }
+  static __device__ __forceinline__ float4 load_f(const Args& a, uint32_t j) { return a.x[j]; }
   static __device__ __forceinline__ bool hit(const Target& t, uint32_t, const float4& xj,
                                              const StepConsts& c) {
     float dx, dy, dz;
@@ -125,7 +128,8 @@ struct DeltaPass {
   static __device__ __forceinline__ Target target(const Args& a, uint32_t i, const StepConsts&) {
     return Target{a.xl[i], i};
   }
-  static __device__ __forceinline__ float4 load(const Args& a, uint32_t j) { return a.xl[j]; }
+  static __device__ __forceinline__ float4 load_s(const Args& a, uint32_t j) { return a.xl[j]; }
+  static __device__ __forceinline__ float4 load_f(const Args& a, uint32_t j) { return a.xl[j]; }
   static __device__ __forceinline__ bool hit(const Target& t, uint32_t j, const float4& xj,
                                              const StepConsts& c) {
     float dx, dy, dz;
@@ -187,29 +191,29 @@ struct XsphPass {
   struct Acc {
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
   };
-  // Positions are batch-loaded for every candidate; a neighbour's (v, rho)
-  // only for pairs inside h.
+  // Screening loads positions only; a neighbour's (v, rho) is loaded with
+  // its position for the pairs inside h.
   struct Nb {
-    float4 x;
-    const float4* __restrict__ vr;
+    float4 x, v;
   };
   static __device__ __forceinline__ Target target(const Args& a, uint32_t i, const StepConsts&) {
     return Target{a.xa[i], a.vr[i], i};
   }
-  static __device__ __forceinline__ Nb load(const Args& a, uint32_t j) {
-    return Nb{a.xa[j], a.vr};
+  static __device__ __forceinline__ float4 load_s(const Args& a, uint32_t j) { return a.xa[j]; }
+  static __device__ __forceinline__ Nb load_f(const Args& a, uint32_t j) {
+    return Nb{a.xa[j], a.vr[j]};
   }
-  static __device__ __forceinline__ bool hit(const Target& t, uint32_t, const Nb& nb,
+  static __device__ __forceinline__ bool hit(const Target& t, uint32_t, const float4& xj,
                                              const StepConsts& c) {
     float dx, dy, dz;
-    return dist2(t.x, nb.x, dx, dy, dz) < c.h2;
+    return dist2(t.x, xj, dx, dy, dz) < c.h2;
   }
-  static __device__ __forceinline__ void add(Acc& s, const Target& t, uint32_t j, const Nb& nb,
+  static __device__ __forceinline__ void add(Acc& s, const Target& t, uint32_t, const Nb& nb,
                                              const StepConsts& c) {
     float dx, dy, dz;
     const float r2 = dist2(t.x, nb.x, dx, dy, dz);
     if (r2 < c.h2) {
-      const float4 vj = nb.vr[j];
+      const float4 vj = nb.v;
       const float w = poly6_of(r2, c);
       const float f = (2.0f * w) / (t.v.w + vj.w);
       s.ax = fmaf(vj.x - t.v.x, f, s.ax);
@@ -247,7 +251,8 @@ struct DensityPass {
   static __device__ __forceinline__ Target target(const Args& a, uint32_t i, const StepConsts&) {
     return Target{a.x[i], i};
   }
-  static __device__ __forceinline__ float4 load(const Args& a, uint32_t j) { return a.x[j]; }
+  static __device__ __forceinline__ float4 load_s(const Args& a, uint32_t j) { return a.x[j]; }
+  static __device__ __forceinline__ float4 load_f(const Args& a, uint32_t j) { return a.x[j]; }
   static __device__ __forceinline__ bool hit(const Target& t, uint32_t, const float4& xj,
                                              const StepConsts& c) {
     float dx, dy, dz;
@@ -283,37 +288,29 @@ struct DensityPass {
 // so the canonical order holds either way.
 template <class P>
 __global__ __launch_bounds__(kBlock) void k_light(typename P::Args a, int n, StepConsts c,
-                                                  const uint2* __restrict__ ranges, HeavyLists hl,
+                                                  const uint32_t* __restrict__ cs, HeavyLists hl,
                                                   int pass) {
-  __shared__ uint2 cols[9][kBlock];
+  __shared__ uint2 seq[10][kBlock];
   uint32_t* cnt = hl.counters + 4 * (pass & 1);
   if (blockIdx.x == 0 && threadIdx.x < 4) hl.counters[4 * ((pass + 1) & 1) + threadIdx.x] = 0u;
   const int tid = threadIdx.x;
   const int i = xcd_block_index() * kBlock + tid;
   const bool active = i < n;
+  const SeqTable st{&seq[0][tid], kBlock};
   typename P::Target t{};
   uint32_t m = 0;
   if (active) {
     t = P::target(a, (uint32_t)i, c);
-    const Cell q = cell_of(t.x.x, t.x.y, t.x.z, c);
-#pragma unroll
-    for (int k = 0; k < 9; ++k) {
-      const uint2 r = column_range(q, k, ranges, c);
-      cols[k][tid] = r;
-      m += r.y - r.x;
-    }
+    m = build_seq(st, cell_of(t.x.x, t.x.y, t.x.z, c), cs, c);
   }
   const bool is_heavy = active && m > kLightMax;
   const uint32_t nch = is_heavy ? (m + kChunk - 1) / kChunk : 0u;
   if (__ballot(is_heavy) != 0ull) {
-    // wave-aggregated allocation of records and chunk items
-    uint32_t rec_total, item_total;
-    const uint32_t rec_off = wave_exclusive_sum(is_heavy ? 1u : 0u, &rec_total);
+    // wave-aggregated allocation of chunk items and records
+    uint32_t item_total;
     const uint32_t item_off = wave_exclusive_sum(nch, &item_total);
-    uint32_t rec_base = 0, item_base = 0;
-    if ((threadIdx.x & 63) == 0) {
-      item_base = atomicAdd(&cnt[1], item_total);
-    }
+    uint32_t item_base = 0, rec_base = 0;
+    if ((threadIdx.x & 63) == 0) item_base = atomicAdd(&cnt[1], item_total);
     item_base = __shfl(item_base, 0);
     const uint32_t first = item_base + item_off;
     const bool fits = is_heavy && first + nch <= hl.item_cap;
@@ -330,34 +327,23 @@ __global__ __launch_bounds__(kBlock) void k_light(typename P::Args a, int n, Ste
         hl.recs[rec_base + (uint32_t)__popcll(fm & ((1ull << (threadIdx.x & 63)) - 1ull))] =
             make_uint4((uint32_t)i, first, nch, 0u);
     }
-    (void)rec_off;
-    (void)rec_total;
     if (fits) return;
   }
   if (!active) return;
+  auto ls = [&](uint32_t j) { return P::load_s(a, j); };
+  auto lf = [&](uint32_t j) { return P::load_f(a, j); };
+  auto hit = [&](uint32_t j, const auto& d) { return P::hit(t, j, d, c); };
   typename P::Acc s;
   if (!is_heavy) {
-#pragma unroll 1
-    for (int k = 0; k < 9; ++k) {
-      const uint2 r = cols[k][tid];
-      sweep_filtered(
-          r.x, r.y, [&](uint32_t j) { return P::load(a, j); },
-          [&](uint32_t j, const auto& d) { return P::hit(t, j, d, c); },
-          [&](uint32_t j, const auto& d) { P::add(s, t, j, d, c); });
-    }
+    sweep_seq(st, 0u, m, ls, lf, hit, [&](uint32_t j, const auto& d) { P::add(s, t, j, d, c); });
   } else {
     // chunk list full: this lane sums its chunks in order
-    uint2 col[9];
-#pragma unroll
-    for (int k = 0; k < 9; ++k) col[k] = cols[k][tid];
     for (uint32_t ch = 0; ch < nch; ++ch) {
       typename P::Acc part;
       const uint32_t s0 = ch * kChunk;
       const uint32_t s1 = s0 + kChunk < m ? s0 + kChunk : m;
-      sweep_seq(
-          col, s0, s1, [&](uint32_t j) { return P::load(a, j); },
-          [&](uint32_t j, const auto& d) { return P::hit(t, j, d, c); },
-          [&](uint32_t j, const auto& d) { P::add(part, t, j, d, c); });
+      sweep_seq(st, s0, s1, ls, lf, hit,
+                [&](uint32_t j, const auto& d) { P::add(part, t, j, d, c); });
       P::accumulate(s, part);
     }
   }
@@ -368,28 +354,24 @@ __global__ __launch_bounds__(kBlock) void k_light(typename P::Args a, int n, Ste
 // candidates of a heavy target.
 template <class P>
 __global__ __launch_bounds__(kBlock) void k_chunks(typename P::Args a, StepConsts c,
-                                                   const uint2* __restrict__ ranges, HeavyLists hl,
+                                                   const uint32_t* __restrict__ cs, HeavyLists hl,
                                                    int pass) {
+  __shared__ uint2 seq[10][kBlock];
   const uint32_t* cnt = hl.counters + 4 * (pass & 1);
   const uint32_t total = cnt[1] < hl.item_cap ? cnt[1] : hl.item_cap;
   const uint32_t stride = gridDim.x * kBlock;
+  const SeqTable st{&seq[0][threadIdx.x], kBlock};
   for (uint32_t u = (uint32_t)xcd_block_index() * kBlock + threadIdx.x; u < total; u += stride) {
     const uint2 it = hl.items[u];
     if (it.x == ~0u) continue;
     const typename P::Target t = P::target(a, it.x, c);
-    const Cell q = cell_of(t.x.x, t.x.y, t.x.z, c);
-    uint2 col[9];
-    uint32_t m = 0;
-#pragma unroll
-    for (int k = 0; k < 9; ++k) {
-      col[k] = column_range(q, k, ranges, c);
-      m += col[k].y - col[k].x;
-    }
+    const uint32_t m = build_seq(st, cell_of(t.x.x, t.x.y, t.x.z, c), cs, c);
     typename P::Acc part;
     const uint32_t s0 = it.y * kChunk;
     const uint32_t s1 = s0 + kChunk < m ? s0 + kChunk : m;
     sweep_seq(
-        col, s0, s1, [&](uint32_t j) { return P::load(a, j); },
+        st, s0, s1, [&](uint32_t j) { return P::load_s(a, j); },
+        [&](uint32_t j) { return P::load_f(a, j); },
         [&](uint32_t j, const auto& d) { return P::hit(t, j, d, c); },
         [&](uint32_t j, const auto& d) { P::add(part, t, j, d, c); });
     static_assert(sizeof(part) <= kPartialWords * sizeof(float), "partial too large");
@@ -422,10 +404,10 @@ int grid_for_items(long long items, int max_blocks) {
 }
 
 template <class P>
-void launch_pass(const typename P::Args& a, int n, const StepConsts& c, const uint2* ranges,
+void launch_pass(const typename P::Args& a, int n, const StepConsts& c, const uint32_t* cs,
                  const HeavyLists& hl, int pass, hipStream_t st) {
-  k_light<P><<<blocks_for(n), kBlock, 0, st>>>(a, n, c, ranges, hl, pass);
-  k_chunks<P><<<grid_for_items(hl.item_cap, 2048), kBlock, 0, st>>>(a, c, ranges, hl, pass);
+  k_light<P><<<blocks_for(n), kBlock, 0, st>>>(a, n, c, cs, hl, pass);
+  k_chunks<P><<<grid_for_items(hl.item_cap, 2048), kBlock, 0, st>>>(a, c, cs, hl, pass);
   k_combine<P><<<grid_for_items(n, 512), kBlock, 0, st>>>(a, c, hl, pass);
 }
 
@@ -433,31 +415,55 @@ void launch_pass(const typename P::Args& a, int n, const StepConsts& c, const ui
 // Streaming kernels.
 // ---------------------------------------------------------------------------
 
+// cnt[k] += 1 for every valid lane, with one atomic per run of equal keys
+// among consecutive lanes (particles arrive in the previous frame's cell
+// order, so runs are common and piled cells would otherwise serialise
+// thousands of atomics on one address).  All lanes of the wave call it.
+__device__ __forceinline__ void count_cell(uint32_t k, bool valid, uint32_t* __restrict__ cnt) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t prev = __shfl_up(k, 1);
+  const uint64_t vm = __ballot(valid);
+  const bool head = valid && (lane == 0 || prev != k);
+  const uint64_t hm = __ballot(head);
+  if (head) {
+    // run = [lane, next head or end of the valid lanes)
+    const uint64_t later = lane == 63 ? 0ull : (hm >> (lane + 1)) << (lane + 1);
+    const int end = later ? __ffsll((unsigned long long)later) - 1 : 64 - __clzll(vm);
+    atomicAdd(&cnt[k], (uint32_t)(end - lane));
+  }
+}
+
 // advect_kernel (fluids/Simulator_kernel.cuh:7-19) fused with getGridId
 // (fluids/Simulator.cu:56-73).  vel += dt*g and npos = pos + dt*vel as fused
 // multiply-adds; the updated velocity itself is not stored, since the
 // reference overwrites it before any read (fluids/Simulator.cu:267-274).
-// Also clears the heavy counter of the first neighbour pass.
+// Also counts particles per cell (the input of the cell-start scan) and
+// clears the heavy counters of the first neighbour pass.
 template <int L>
 __global__ __launch_bounds__(kBlock) void k_predict_key(const float* __restrict__ pos,
                                                         const float* __restrict__ vel, int n,
                                                         StepConsts c, float4* __restrict__ xp,
                                                         uint32_t* __restrict__ key,
                                                         uint32_t* __restrict__ val,
+                                                        uint32_t* __restrict__ cnt,
                                                         uint32_t* __restrict__ counters) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i < 4) counters[i] = 0u;
-  if (i >= n) return;
-  const float4 p = load_vec<L>(pos, i);
-  const float4 v = load_vec<L>(vel, i);
+  if (blockIdx.x * kBlock >= n) return;  // whole workgroup past the end
+  const int ii = i < n ? i : n - 1;      // idle lanes repeat the last particle
+  const float4 p = load_vec<L>(pos, ii);
+  const float4 v = load_vec<L>(vel, ii);
   const float vx = fmaf(c.dt, 0.0f, v.x);
   const float vy = fmaf(c.dt, 0.0f, v.y);
   const float vz = fmaf(c.dt, c.gz, v.z);
   const float x = fmaf(c.dt, vx, p.x);
   const float y = fmaf(c.dt, vy, p.y);
   const float z = fmaf(c.dt, vz, p.z);
+  const uint32_t k = cell_id(cell_of(x, y, z, c), c);
+  count_cell(k, i < n, cnt);
+  if (i >= n) return;
   xp[i] = make_float4(x, y, z, 0.0f);
-  key[i] = cell_id(cell_of(x, y, z, c), c);
+  key[i] = k;
   val[i] = (uint32_t)i;
 }
 
@@ -467,71 +473,146 @@ __global__ __launch_bounds__(kBlock) void k_key_only(const float* __restrict__ p
                                                      StepConsts c, float4* __restrict__ xp,
                                                      uint32_t* __restrict__ key,
                                                      uint32_t* __restrict__ val,
+                                                     uint32_t* __restrict__ cnt,
                                                      uint32_t* __restrict__ counters) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i < 4) counters[i] = 0u;
+  if (blockIdx.x * kBlock >= n) return;
+  const int ii = i < n ? i : n - 1;
+  const float4 p = load_vec<L>(pos, ii);
+  const uint32_t k = cell_id(cell_of(p.x, p.y, p.z, c), c);
+  count_cell(k, i < n, cnt);
   if (i >= n) return;
-  const float4 p = load_vec<L>(pos, i);
   xp[i] = make_float4(p.x, p.y, p.z, 0.0f);
-  key[i] = cell_id(cell_of(p.x, p.y, p.z, c), c);
+  key[i] = k;
   val[i] = (uint32_t)i;
 }
 
-// cudaMemset of gridStart/gridEnd (fluids/Simulator.cu:201-203) as a fill
-// kernel (a memset node must not sit in a captured graph), two cells per
-// 16-byte store.
-__global__ __launch_bounds__(kBlock) void k_clear_ranges(uint2* __restrict__ ranges,
-                                                         long long cells) {
-  const long long pairs = cells >> 1;
-  uint4* r4 = reinterpret_cast<uint4*>(ranges);
+// Fill of the per-cell counts (solver creation, or after a step that failed
+// between counting and the scan).  A fill kernel, not a memset node.
+__global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t* __restrict__ a, long long n,
+                                                     uint32_t v) {
   const long long stride = (long long)gridDim.x * kBlock;
-  for (long long k = (long long)blockIdx.x * kBlock + threadIdx.x; k < pairs; k += stride)
-    r4[k] = make_uint4(0u, 0u, 0u, 0u);
-  if ((cells & 1) && blockIdx.x == 0 && threadIdx.x == 0) ranges[cells - 1] = make_uint2(0u, 0u);
+  for (long long k = (long long)blockIdx.x * kBlock + threadIdx.x; k < n; k += stride) a[k] = v;
 }
 
-// computeGridRange (fluids/Simulator_kernel.cuh:21-50): a slot whose key
-// differs from the previous slot's opens its cell and closes the previous
-// one.  Keys < cells and slots < n hold for any sorted key array; the guards
-// keep a corrupted one from reaching out of bounds.
-__device__ __forceinline__ void mark_range(const uint32_t* __restrict__ skey, int s, int n,
-                                           int cells, uint2* __restrict__ ranges) {
-  const uint32_t k = skey[s];
-  const uint32_t prev = s > 0 ? skey[s - 1] : 0xffffffffu;
-  if (k < (uint32_t)cells) {
-    if (s == 0 || prev != k) {
-      ranges[k].x = (uint32_t)s;
-      if (s > 0 && prev < (uint32_t)cells) ranges[prev].y = (uint32_t)s;
+// ---- cell-start table: exclusive scan of the per-cell counts -------------
+// Replaces cudaMemset + computeGridRange (fluids/Simulator.cu:201-204,
+// fluids/Simulator_kernel.cuh:21-50): cs[c] = number of particles in cells
+// < c, which is the first sorted slot of cell c (empty cells: the next
+// non-empty cell's start).  Three kernels: tile sums, scan of the tile sums,
+// tile-local scan + offset (which also clears the counts for the next step).
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kBlock * kScanItems;
+
+__device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t* sh) {
+  // wave sums by shuffles, then the four wave sums
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  const uint32_t t = sh[0] + sh[1] + sh[2] + sh[3];
+  __syncthreads();
+  return t;
+}
+
+// Exclusive scan over the 256 threads; *total gets the block sum.
+__device__ __forceinline__ uint32_t block_exclusive(uint32_t v, uint32_t* sh, uint32_t* total) {
+  uint32_t tot;
+  const uint32_t ex = wave_exclusive_sum(v, &tot);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) sh[w] = tot;
+  __syncthreads();
+  uint32_t base = 0, all = 0;
+#pragma unroll
+  for (int k = 0; k < kBlock / 64; ++k) {
+    base += k < w ? sh[k] : 0u;
+    all += sh[k];
+  }
+  __syncthreads();
+  *total = all;
+  return base + ex;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_tiles(const uint32_t* __restrict__ cnt,
+                                                       long long cells,
+                                                       uint32_t* __restrict__ tile_sum) {
+  __shared__ uint32_t sh[kBlock / 64];
+  const long long base = (long long)blockIdx.x * kScanTile;
+  uint32_t v = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    const long long c = base + (long long)k * kBlock + threadIdx.x;
+    if (c < cells) v += cnt[c];
+  }
+  const uint32_t t = block_sum(v, sh);
+  if (threadIdx.x == 0) tile_sum[blockIdx.x] = t;
+}
+
+// One workgroup: exclusive scan of the tile sums in place; cs[cells] = total.
+__global__ __launch_bounds__(kBlock) void k_scan_tile_sums(uint32_t* __restrict__ tile_sum,
+                                                           int ntiles, long long cells,
+                                                           uint32_t* __restrict__ cs) {
+  __shared__ uint32_t sh[kBlock / 64];
+  uint32_t carry = 0;
+  for (int t0 = 0; t0 < ntiles; t0 += kBlock) {
+    const int t = t0 + (int)threadIdx.x;
+    const uint32_t v = t < ntiles ? tile_sum[t] : 0u;
+    uint32_t total;
+    const uint32_t ex = block_exclusive(v, sh, &total);
+    if (t < ntiles) tile_sum[t] = carry + ex;
+    carry += total;
+  }
+  if (threadIdx.x == 0) cs[cells] = carry;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_cells(uint32_t* __restrict__ cnt, long long cells,
+                                                       const uint32_t* __restrict__ tile_off,
+                                                       uint32_t* __restrict__ cs) {
+  __shared__ uint32_t sh[kBlock / 64];
+  // thread t owns cells base + t*kScanItems .. + kScanItems-1 (consecutive)
+  const long long base = (long long)blockIdx.x * kScanTile + (long long)threadIdx.x * kScanItems;
+  uint32_t v[kScanItems];
+  uint32_t sum = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    const long long c = base + k;
+    v[k] = c < cells ? cnt[c] : 0u;
+    sum += v[k];
+  }
+  uint32_t total;
+  uint32_t run = tile_off[blockIdx.x] + block_exclusive(sum, sh, &total);
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    const long long c = base + k;
+    if (c < cells) {
+      cs[c] = run;
+      cnt[c] = 0u;
     }
-    if (s == n - 1) ranges[k].y = (uint32_t)n;
+    run += v[k];
   }
 }
 
 // Gather of the sorted payload (the zip permutation of
-// fluids/Simulator.cu:196-198) fused with the cell ranges.
+// fluids/Simulator.cu:196-198).  Indices < n hold for any sorted index
+// array; the guard keeps a corrupted one from reaching out of bounds.
 template <int L>
 __global__ __launch_bounds__(kBlock) void k_permute(
-    const uint32_t* __restrict__ skey, const uint32_t* __restrict__ sval,
-    const float* __restrict__ pos, const float4* __restrict__ xp, const uint32_t* __restrict__ iid,
-    int n, int cells, float4* __restrict__ x0s, float4* __restrict__ xa,
-    uint32_t* __restrict__ iid_s, uint2* __restrict__ ranges) {
+    const uint32_t* __restrict__ sval, const float* __restrict__ pos,
+    const float4* __restrict__ xp, const uint32_t* __restrict__ iid, int n,
+    float4* __restrict__ x0s, float4* __restrict__ xa, uint32_t* __restrict__ iid_s) {
   const int s = blockIdx.x * kBlock + threadIdx.x;
   if (s >= n) return;
-  mark_range(skey, s, n, cells, ranges);
   const uint32_t src = sval[s] < (uint32_t)n ? sval[s] : 0u;
   x0s[s] = load_vec<L>(pos, src);
   xa[s] = xp[src];
   iid_s[s] = iid[src];
 }
 
-__global__ __launch_bounds__(kBlock) void k_permute_pos(const uint32_t* __restrict__ skey,
-                                                        const uint32_t* __restrict__ sval,
+__global__ __launch_bounds__(kBlock) void k_permute_pos(const uint32_t* __restrict__ sval,
                                                         const float4* __restrict__ xp, int n,
-                                                        int cells, float4* __restrict__ xa,
-                                                        uint2* __restrict__ ranges) {
+                                                        float4* __restrict__ xa) {
   const int s = blockIdx.x * kBlock + threadIdx.x;
   if (s >= n) return;
-  mark_range(skey, s, n, cells, ranges);
   xa[s] = xp[sval[s] < (uint32_t)n ? sval[s] : 0u];
 }
 
@@ -621,53 +702,62 @@ __global__ __launch_bounds__(kRedBlock) void k_reduce_partials(const double* __r
 }  // namespace
 
 void launch_predict_key(int layout, const float* pos, const float* vel, int n, const StepConsts& c,
-                        float4* xp, uint32_t* key, uint32_t* val, uint32_t* counters,
-                        hipStream_t st) {
+                        float4* xp, uint32_t* key, uint32_t* val, uint32_t* cnt,
+                        uint32_t* counters, hipStream_t st) {
   if (layout == 3)
-    k_predict_key<3><<<blocks_for(n), kBlock, 0, st>>>(pos, vel, n, c, xp, key, val, counters);
+    k_predict_key<3><<<blocks_for(n), kBlock, 0, st>>>(pos, vel, n, c, xp, key, val, cnt, counters);
   else
-    k_predict_key<4><<<blocks_for(n), kBlock, 0, st>>>(pos, vel, n, c, xp, key, val, counters);
+    k_predict_key<4><<<blocks_for(n), kBlock, 0, st>>>(pos, vel, n, c, xp, key, val, cnt, counters);
 }
 
 void launch_key_only(int layout, const float* pos, int n, const StepConsts& c, float4* xp,
-                     uint32_t* key, uint32_t* val, uint32_t* counters, hipStream_t st) {
+                     uint32_t* key, uint32_t* val, uint32_t* cnt, uint32_t* counters,
+                     hipStream_t st) {
   if (layout == 3)
-    k_key_only<3><<<blocks_for(n), kBlock, 0, st>>>(pos, n, c, xp, key, val, counters);
+    k_key_only<3><<<blocks_for(n), kBlock, 0, st>>>(pos, n, c, xp, key, val, cnt, counters);
   else
-    k_key_only<4><<<blocks_for(n), kBlock, 0, st>>>(pos, n, c, xp, key, val, counters);
+    k_key_only<4><<<blocks_for(n), kBlock, 0, st>>>(pos, n, c, xp, key, val, cnt, counters);
 }
 
-void launch_clear_ranges(uint2* ranges, long long cells, hipStream_t st) {
-  long long blocks = ((cells >> 1) + kBlock - 1) / kBlock;
+void launch_fill_u32(uint32_t* a, long long n, uint32_t v, hipStream_t st) {
+  long long blocks = (n + kBlock - 1) / kBlock;
   if (blocks < 1) blocks = 1;
   if (blocks > 8192) blocks = 8192;
-  k_clear_ranges<<<(int)blocks, kBlock, 0, st>>>(ranges, cells);
+  k_fill_u32<<<(int)blocks, kBlock, 0, st>>>(a, n, v);
 }
 
-void launch_permute(int layout, const uint32_t* skey, const uint32_t* sval, const float* pos,
-                    const float4* xp, const uint32_t* iid, int n, int cells, float4* x0s,
-                    float4* xa, uint32_t* iid_s, uint2* ranges, hipStream_t st) {
+long long scan_tiles(long long cells) { return (cells + kScanTile - 1) / kScanTile; }
+
+void launch_cell_starts(uint32_t* cnt, long long cells, uint32_t* tile_sum, uint32_t* cs,
+                        hipStream_t st) {
+  const int ntiles = (int)scan_tiles(cells);
+  k_scan_tiles<<<ntiles, kBlock, 0, st>>>(cnt, cells, tile_sum);
+  k_scan_tile_sums<<<1, kBlock, 0, st>>>(tile_sum, ntiles, cells, cs);
+  k_scan_cells<<<ntiles, kBlock, 0, st>>>(cnt, cells, tile_sum, cs);
+}
+
+void launch_permute(int layout, const uint32_t* sval, const float* pos, const float4* xp,
+                    const uint32_t* iid, int n, float4* x0s, float4* xa, uint32_t* iid_s,
+                    hipStream_t st) {
   if (layout == 3)
-    k_permute<3><<<blocks_for(n), kBlock, 0, st>>>(skey, sval, pos, xp, iid, n, cells, x0s, xa,
-                                                   iid_s, ranges);
+    k_permute<3><<<blocks_for(n), kBlock, 0, st>>>(sval, pos, xp, iid, n, x0s, xa, iid_s);
   else
-    k_permute<4><<<blocks_for(n), kBlock, 0, st>>>(skey, sval, pos, xp, iid, n, cells, x0s, xa,
-                                                   iid_s, ranges);
+    k_permute<4><<<blocks_for(n), kBlock, 0, st>>>(sval, pos, xp, iid, n, x0s, xa, iid_s);
 }
 
-void launch_permute_pos(const uint32_t* skey, const uint32_t* sval, const float4* xp, int n,
-                        int cells, float4* xa, uint2* ranges, hipStream_t st) {
-  k_permute_pos<<<blocks_for(n), kBlock, 0, st>>>(skey, sval, xp, n, cells, xa, ranges);
+void launch_permute_pos(const uint32_t* sval, const float4* xp, int n, float4* xa,
+                        hipStream_t st) {
+  k_permute_pos<<<blocks_for(n), kBlock, 0, st>>>(sval, xp, n, xa);
 }
 
-void launch_lambda(const float4* xa, const uint2* ranges, int n, const StepConsts& c, float4* xl,
+void launch_lambda(const float4* xa, const uint32_t* cs, int n, const StepConsts& c, float4* xl,
                    float* rho, const HeavyLists& hl, int pass, hipStream_t st) {
-  launch_pass<LambdaPass>(LambdaPass::Args{xa, xl, rho}, n, c, ranges, hl, pass, st);
+  launch_pass<LambdaPass>(LambdaPass::Args{xa, xl, rho}, n, c, cs, hl, pass, st);
 }
 
-void launch_delta(const float4* xl, const uint2* ranges, int n, const StepConsts& c, float4* xa,
+void launch_delta(const float4* xl, const uint32_t* cs, int n, const StepConsts& c, float4* xa,
                   const HeavyLists& hl, int pass, hipStream_t st) {
-  launch_pass<DeltaPass>(DeltaPass::Args{xl, xa}, n, c, ranges, hl, pass, st);
+  launch_pass<DeltaPass>(DeltaPass::Args{xl, xa}, n, c, cs, hl, pass, st);
 }
 
 void launch_velocity(int layout, const float4* xa, const float4* x0s, const uint32_t* iid_s,
@@ -679,21 +769,21 @@ void launch_velocity(int layout, const float4* xa, const float4* x0s, const uint
     k_velocity<4><<<blocks_for(n), kBlock, 0, st>>>(xa, x0s, iid_s, rho, n, c, vr, pos, vel, iid);
 }
 
-void launch_xsph(int layout, const float4* xa, const float4* vr, const uint2* ranges, int n,
+void launch_xsph(int layout, const float4* xa, const float4* vr, const uint32_t* cs, int n,
                  const StepConsts& c, float* npos, float* nvel, const HeavyLists& hl, int pass,
                  hipStream_t st) {
   if (layout == 3)
-    launch_pass<XsphPass<3>>(XsphPass<3>::Args{xa, vr, npos, nvel}, n, c, ranges, hl, pass, st);
+    launch_pass<XsphPass<3>>(XsphPass<3>::Args{xa, vr, npos, nvel}, n, c, cs, hl, pass, st);
   else
-    launch_pass<XsphPass<4>>(XsphPass<4>::Args{xa, vr, npos, nvel}, n, c, ranges, hl, pass, st);
+    launch_pass<XsphPass<4>>(XsphPass<4>::Args{xa, vr, npos, nvel}, n, c, cs, hl, pass, st);
 }
 
 int density_partials(int n) { return (n + kRedBlock * kRedItems - 1) / (kRedBlock * kRedItems); }
 
-void launch_density_error(const float4* xa, const uint2* ranges, int n, const StepConsts& c,
+void launch_density_error(const float4* xa, const uint32_t* cs, int n, const StepConsts& c,
                           double* err, double* partial, double* out, const HeavyLists& hl,
                           hipStream_t st) {
-  launch_pass<DensityPass>(DensityPass::Args{xa, err}, n, c, ranges, hl, 0, st);
+  launch_pass<DensityPass>(DensityPass::Args{xa, err}, n, c, cs, hl, 0, st);
   const int nb = density_partials(n);
   k_reduce_err<<<nb, kRedBlock, 0, st>>>(err, n, partial);
   k_reduce_partials<<<1, kRedBlock, 0, st>>>(partial, nb, n, out);

@@ -49,6 +49,13 @@ class PbfVec3(ctypes.Structure):
     _fields_ = [("x", ctypes.c_float), ("y", ctypes.c_float), ("z", ctypes.c_float)]
 
 
+class PbfSweep(ctypes.Structure):
+    """struct pbf_sweep: the sweeping wall of fluids/FluidSystem.cpp:36-38, :104-110."""
+
+    _fields_ = [("ulim0", PbfVec3), ("llim0", PbfVec3), ("a_ulim", PbfVec3), ("a_llim", PbfVec3),
+                ("w", ctypes.c_float), ("start_frame", ctypes.c_int)]
+
+
 class PBFError(RuntimeError):
     """A non-zero status from libpbf.so, with the library's message."""
 
@@ -71,6 +78,11 @@ _SIGNATURES = [
     ("pbf_get_lim", ctypes.c_int, [_P, ctypes.POINTER(PbfVec3), ctypes.POINTER(PbfVec3)]),
     ("pbf_step", ctypes.c_int, [_P, _P, _P, _P, _P, _P, ctypes.c_int, _P]),
     ("pbf_density_error", ctypes.c_int, [_P, _P, ctypes.c_int, _P, ctypes.POINTER(ctypes.c_double)]),
+    ("pbf_sweep_box", ctypes.c_int,
+     [ctypes.POINTER(PbfSweep), ctypes.c_int, ctypes.POINTER(PbfVec3), ctypes.POINTER(PbfVec3)]),
+    ("pbf_run_frames", ctypes.c_int,
+     [_P, _P, _P, _P, _P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(PbfSweep),
+      _P, ctypes.POINTER(ctypes.c_int)]),
     ("pbf_set_timing", ctypes.c_int, [_P, ctypes.c_int]),
     ("pbf_get_timing", ctypes.c_int,
      [_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_longlong)]),

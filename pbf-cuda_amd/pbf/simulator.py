@@ -12,7 +12,7 @@ import ctypes
 from typing import Optional, Sequence
 
 from . import _lib
-from ._lib import PBFError, PbfParams, PbfVec3
+from ._lib import PBFError, PbfParams, PbfSweep, PbfVec3
 from .params import FluidParams
 
 
@@ -43,6 +43,24 @@ def _ptr(buf, layout: int, what: str, n: int, is_vec: bool) -> int:
         if buf.numel() < n:
             raise ValueError(f"{what}: holds {buf.numel()} ids, need {n}")
     return buf.data_ptr()
+
+
+class Sweep:
+    """Sweeping wall (fluids/FluidSystem.cpp:36-38, :104-110): from frame
+    start_frame on, ulim = ulim0 + a_ulim * sin(w * (frame - start_frame)),
+    llim likewise; w in radians per frame."""
+
+    def __init__(self, ulim0, llim0, a_ulim=(0.0, 0.0, 0.0), a_llim=(0.0, 0.0, 0.0),
+                 w: float = 0.0, start_frame: int = 0):
+        self.c = PbfSweep(_vec3(ulim0), _vec3(llim0), _vec3(a_ulim), _vec3(a_llim), float(w),
+                          int(start_frame))
+
+    def box(self, frame: int):
+        """(ulim, llim) of `frame`, computed by the library (host only)."""
+        u, l = PbfVec3(), PbfVec3()
+        _lib.check(_lib.load().pbf_sweep_box(ctypes.byref(self.c), int(frame), ctypes.byref(u),
+                                             ctypes.byref(l)), None)
+        return (u.x, u.y, u.z), (l.x, l.y, l.z)
 
 
 def _stream_handle(stream) -> Optional[int]:
@@ -142,6 +160,23 @@ class Simulator:
         return (u.x, u.y, u.z), (l.x, l.y, l.z)
 
     # -- additions --------------------------------------------------------
+    def run_frames(self, pos_a, vel_a, pos_b, vel_b, iid, nparticle: int, first_frame: int,
+                   nframes: int, sweep: Optional[Sweep] = None, stream=None) -> int:
+        """FluidSystem::stepSimulate for nframes frames (tic-toc buffers, optional
+        sweeping wall).  Returns 0 if the newest state is in (pos_a, vel_a),
+        1 if in (pos_b, vel_b).  Stream-ordered; does not synchronise."""
+        n = int(nparticle)
+        L = self.layout
+        args = (_ptr(pos_a, L, "pos_a", n, True), _ptr(vel_a, L, "vel_a", n, True),
+                _ptr(pos_b, L, "pos_b", n, True), _ptr(vel_b, L, "vel_b", n, True),
+                _ptr(iid, L, "iid", n, False))
+        which = ctypes.c_int(0)
+        h = self._handle()
+        _lib.check(self._lib.pbf_run_frames(h, *args, n, int(first_frame), int(nframes),
+                                            ctypes.byref(sweep.c) if sweep else None,
+                                            _stream_handle(stream), ctypes.byref(which)), h)
+        return int(which.value)
+
     def density_error(self, pos, nparticle: int, stream=None):
         """(mean, max) of |rho_i/pho0 - 1| on a fresh grid over pos; synchronises."""
         n = int(nparticle)

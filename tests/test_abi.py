@@ -111,3 +111,21 @@ def test_python_wrapper_validates_buffers(pbf_mod):
     with pytest.raises(TypeError):
         _ptr([1, 2], 4, "pos", 2, True)
     assert _ptr(12345, 4, "pos", 2, True) == 12345
+
+
+def test_sweep_box_matches_fluidsystem_formula(pbf_mod):
+    """pbf_sweep_box: ulim = ulim0 + A_u sin(w (f - start)), llim likewise, in
+    FP32, resting box before start (fluids/FluidSystem.cpp:104-110)."""
+    import numpy as np
+
+    sw = pbf_mod.Sweep((16.0, 6.4, 16.0), (0.0, 0.0, 0.0), a_ulim=(2.0, 0.0, 0.0),
+                       a_llim=(0.0, -0.5, 0.0), w=2 * np.pi / 120, start_frame=5)
+    assert sw.box(0) == (pytest.approx((16.0, 6.4, 16.0)), pytest.approx((0.0, 0.0, 0.0)))
+    assert sw.box(4)[0] == pytest.approx((16.0, 6.4, 16.0))
+    for f in (5, 20, 35, 65, 200):
+        t = np.float32(np.float32(2 * np.pi / 120) * np.float32(f - 5))
+        phi = float(np.sin(np.float64(t)))
+        u, l = sw.box(f)
+        assert u[0] == pytest.approx(16.0 + 2.0 * phi, abs=2e-6)
+        assert u[1] == pytest.approx(6.4, abs=1e-6) and l[1] == pytest.approx(-0.5 * phi, abs=1e-6)
+    assert sw.box(35)[0][0] == pytest.approx(18.0, abs=1e-5)  # quarter period: peak

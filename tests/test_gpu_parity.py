@@ -257,3 +257,29 @@ def test_phase_timing(pbf_mod, device):
     assert t["lambda"][1] == 2 * p.niter and t["delta"][1] == 2 * p.niter
     assert t["step"][1] == 2 and t["step"][0] > 0
     assert run.sim.get_timing()["step"][1] == 0  # reset
+
+
+def test_run_frames_sweeping_wall_bitexact(pbf_mod, orc, device):
+    """The native multi-frame driver (tic-toc + sweeping wall,
+    fluids/FluidSystem.cpp:99-120) equals stepping the FP32 mirror frame by
+    frame with the same boxes."""
+    p = P.dam_params()
+    sc = scenes.block((32, 16, 32), (3.0, 1.6, 2.0), seed=41)
+    sw = pbf_mod.Sweep(sc.ulim, sc.llim, a_ulim=(0.6, 0.0, 0.0), w=2 * np.pi / 20, start_frame=2)
+    run = Run(pbf_mod, sc, p, max_cells=4 * 40 * 16 * 20)
+    which = run.sim.run_frames(run.a[0], run.a[1], run.b[0], run.b[1], run.iid, sc.n, 0, 15, sw)
+    torch.cuda.synchronize()
+    assert which == 1  # 15 frames: odd count ends in the b buffers
+    out = run.b if which else run.a
+    pos, vel = sc.as_layout(4)
+    iid = sc.iid
+    for f in range(15):
+        ul, ll = sw.box(f)
+        m = orc.step(p, ul, ll, pos, vel, iid, fp32=True)
+        pos, vel, iid = m["npos"], m["nvel"], m["iid"]
+    assert run.sim.getLim() == sw.box(14)
+    g_pos = from_dev(out[0]).reshape(-1, 4)
+    g_vel = from_dev(out[1]).reshape(-1, 4)
+    np.testing.assert_array_equal(from_dev(run.iid, dtype=np.uint32), iid)
+    np.testing.assert_array_equal(g_pos.view(np.uint32), pos.view(np.uint32))
+    np.testing.assert_array_equal(g_vel.view(np.uint32), vel.view(np.uint32))

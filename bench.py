@@ -8,8 +8,12 @@ iterations, FP32), advanced frame after frame with ping-pong buffers as the
 reference's FluidSystem does.  W frames warm up, K frames are timed.
 
 value = N * niter / (time per step): particle-solver-iterations per second,
-inputs resident in HBM, torch's current stream, bracketed by
-torch.cuda.synchronize().  One JSON line on stdout.
+inputs resident in HBM, bracketed by torch.cuda.synchronize(), timed with
+HIP events on the launch stream.  The launch shapes come from the library's
+autotuner run on the warmed-up state; the timed steps replay a recorded HIP
+graph (the eager figure is reported beside it, from a second simulation
+over the same frames).  Also measured: the 64-scene batched rate.  One JSON
+line on stdout.
 """
 import argparse
 import json
@@ -36,6 +40,9 @@ def parse_args():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-graph", action="store_true", help="time eager steps only")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batched rate")
     return ap.parse_args()
 
 
@@ -74,6 +81,72 @@ def cpu_baseline(p, sc, pos_np, vel_np, iid_np, steps):
                       f"{dt:.2f} s per step"}
 
 
+class SingleRun:
+    """The single dam on the GPU with ping-pong buffers, from its initial state."""
+
+    def __init__(self, pbf, p, sc, dev):
+        import numpy as np
+        import torch
+
+        self.n = sc.n
+        pos, vel = sc.as_layout(4)
+        self.bufs = [[torch.from_numpy(pos.ravel()).to(dev), torch.from_numpy(vel.ravel()).to(dev)],
+                     [torch.zeros(self.n * 4, device=dev), torch.zeros(self.n * 4, device=dev)]]
+        self.iid = torch.from_numpy(sc.iid.astype(np.int32)).to(dev)
+        self.sim = pbf.Simulator(p, sc.ulim, sc.llim, capacity=self.n)
+        self.f = 0
+
+    def step(self, stream):
+        a, b = self.bufs[self.f % 2], self.bufs[(self.f + 1) % 2]
+        self.sim.step(a[0], b[0], a[1], b[1], self.iid, self.n, stream)
+        self.f += 1
+
+    def current(self):
+        return self.bufs[self.f % 2]
+
+
+class BatchRun:
+    """The 64-scene batch on the GPU with ping-pong buffers."""
+
+    def __init__(self, pbf, p, scs, dev):
+        import numpy as np
+        import torch
+
+        from pbf import scenes
+
+        pos, vel, iid, counts, ul, ll, _ = scenes.concat(scs, 4)
+        self.n = int(sum(counts))
+        self.bufs = [[torch.from_numpy(pos.ravel()).to(dev), torch.from_numpy(vel.ravel()).to(dev)],
+                     [torch.zeros(self.n * 4, device=dev), torch.zeros(self.n * 4, device=dev)]]
+        self.iid = torch.from_numpy(iid.astype(np.int32)).to(dev)
+        self.sim = pbf.BatchedSimulator(p, ul, ll, counts)
+        self.f = 0
+
+    def step(self, stream):
+        a, b = self.bufs[self.f % 2], self.bufs[(self.f + 1) % 2]
+        self.sim.step_batched(a[0], b[0], a[1], b[1], self.iid, stream)
+        self.f += 1
+
+    def current(self):
+        return self.bufs[self.f % 2]
+
+
+def timed(run, steps, stream):
+    """ms per step over `steps` steps, bracketed by synchronisation, HIP
+    events on the launch stream."""
+    import torch
+
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(steps):
+        run.step(stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
 def main():
     args = parse_args()
     if args.gpus != 1:
@@ -91,50 +164,79 @@ def main():
     p = pbf.dam_params(niter=4)
     sc = scenes.single_dam()
     n = sc.n
-    pos, vel = sc.as_layout(4)
-    bufs = [[torch.from_numpy(pos.ravel()).to(dev), torch.from_numpy(vel.ravel()).to(dev)],
-            [torch.zeros(n * 4, device=dev), torch.zeros(n * 4, device=dev)]]
-    iid = torch.from_numpy(sc.iid.astype(np.int32)).to(dev)
-    sim = pbf.Simulator(p, sc.ulim, sc.llim, capacity=n)
-    stream = torch.cuda.current_stream()
+    stream = torch.cuda.Stream()  # a capturable (non-null) stream for graph mode
 
-    def step(f):
-        a, b = bufs[f % 2], bufs[(f + 1) % 2]
-        sim.step(a[0], b[0], a[1], b[1], iid, n, stream)
+    # eager run: warm up, tune the launch shapes on the warmed-up state, then
+    # time with per-phase events
+    eager = SingleRun(pbf, p, sc, dev)
+    with torch.cuda.stream(stream):
+        for _ in range(args.warmup):
+            eager.step(stream)
+        shapes = eager.sim.get_shapes()
+        if not args.no_tune:
+            cur = eager.current()
+            shapes, _ = eager.sim.autotune(cur[0], cur[1], eager.iid, n, reps=3, stream=stream)
+        eager.sim.set_timing(True)
+        eager.sim.get_timing()  # reset
+        ms_eager = timed(eager, args.steps, stream)
+        phases = eager.sim.get_timing()
+        eager.sim.set_timing(False)
+    print(f"eager: {ms_eager:.4f} ms/step, shapes {shapes}", file=sys.stderr, flush=True)
 
-    for f in range(args.warmup):
-        step(f)
-    torch.cuda.synchronize()
-    sim.set_timing(True)
-    sim.get_timing()  # reset
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    e0.record(stream)
-    for f in range(args.warmup, args.warmup + args.steps):
-        step(f)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / args.steps
-    phases = sim.get_timing()
-    sim.set_timing(False)
+    # graph run: same frames from the same initial state, one graph launch per step
+    ms_graph = None
+    if not args.no_graph:
+        graph = SingleRun(pbf, p, sc, dev)
+        graph.sim.set_shapes(*shapes)
+        graph.sim.set_graph(True)
+        with torch.cuda.stream(stream):
+            for _ in range(args.warmup):
+                graph.step(stream)
+            ms_graph = timed(graph, args.steps, stream)
+        print(f"graph: {ms_graph:.4f} ms/step", file=sys.stderr, flush=True)
+        del graph
+    ms = ms_graph if ms_graph is not None else ms_eager
     value = n * p.niter / (ms * 1e-3)
 
     phase_ms = {k: round(t / c, 4) for k, (t, c) in phases.items() if c}
     d_ms = phases["delta"][0] / phases["delta"][1]
     achieved = DELTA_BYTES_PER_PARTICLE * n / (d_ms * 1e-3) / 1e9
     traffic = measured_traffic()
-    roofline = {"bound": "hbm", "kernel": "k_delta", "achieved": round(achieved, 3),
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                "traffic": traffic, "launch_us": round(d_ms * 1e3, 2),
-                "algorithmic_bytes": DELTA_BYTES_PER_PARTICLE * n}
+    roofline = {"bound": "hbm", "kernel": "delta-p pass (k_light + k_chunks + k_combine)",
+                "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                "launch_us": round(d_ms * 1e3, 2), "algorithmic_bytes": DELTA_BYTES_PER_PARTICLE * n}
+
+    # batched: 64 scenes of uneven size in one launch sequence, graph mode
+    batched = None
+    if not args.no_batch:
+        scs = scenes.batch_scenes()
+        br = BatchRun(pbf, p, scs, dev)
+        with torch.cuda.stream(stream):
+            for _ in range(args.warmup):
+                br.step(stream)
+            bshapes = br.sim.get_shapes()
+            if not args.no_tune:
+                cur = br.current()
+                bshapes, _ = br.sim.autotune(cur[0], cur[1], br.iid, br.n, reps=3, stream=stream)
+            if not args.no_graph:
+                br.sim.set_graph(True)
+            ms_b = timed(br, args.steps, stream)
+        rate = br.n * p.niter / (ms_b * 1e-3)
+        batched = {"value": rate, "unit": "particle-solver-iters/s", "ms_per_step": ms_b,
+                   "scenes": len(scs), "particles": br.n, "frac_of_single": rate / value,
+                   "shapes": list(bshapes), "graph": not args.no_graph,
+                   "workload": "64 box scenes, counts log-uniform in [4096, 262144] (seeded), "
+                               "fill 0.2-0.5, one jittered lattice block each; same constants"}
+        print(f"batched: {ms_b:.4f} ms/step, {rate:.4g} /s", file=sys.stderr, flush=True)
+        del br
 
     cpu = None
     if not args.no_cpu:
-        last = (args.warmup + args.steps) % 2
-        P4 = bufs[last][0].cpu().numpy().reshape(n, 4)
-        V4 = bufs[last][1].cpu().numpy().reshape(n, 4)
-        I = iid.cpu().numpy().astype(np.uint32)
+        cur = eager.current()
+        P4 = cur[0].cpu().numpy().reshape(n, 4)
+        V4 = cur[1].cpu().numpy().reshape(n, 4)
+        I = eager.iid.cpu().numpy().astype(np.uint32)
         cpu = cpu_baseline(p, sc, P4, V4, I, args.cpu_steps)
 
     out = {
@@ -155,11 +257,15 @@ def main():
                         "SplitMix64), 256x128x128 cells, h=0.1, pho0=6378, eps=600, k_corr=0.1, "
                         "n_corr=4, dq=0.2h, g=9.8, dt=1/60, c_XSPH=0.5, 4 iterations; frames "
                         f"{args.warmup}..{args.warmup + args.steps - 1} of the running simulation",
-            "particles": n, "niter": p.niter, "parallelism": "single GPU"},
+            "particles": n, "niter": p.niter, "parallelism": "single GPU",
+            "mode": "hip graph replay" if ms_graph is not None else "eager",
+            "shapes": list(shapes)},
+        "eager_ms_per_step": ms_eager,
+        "graph_ms_per_step": ms_graph,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "phase_ms": phase_ms,
-        "batched_rate": None,
+        "batched_rate": batched,
         "backward_forward_ratio": None,
     }
     print(json.dumps(out), flush=True)

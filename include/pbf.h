@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define PBF_ABI_VERSION 1
+#define PBF_ABI_VERSION 2
 
 /* status codes */
 #define PBF_OK 0
@@ -182,6 +182,73 @@ int pbf_set_timing(pbf_solver* s, int enable);
  * since the last call; resets the accumulators. */
 int pbf_get_timing(pbf_solver* s, double ms_total[PBF_NUM_PHASES],
                    long long count[PBF_NUM_PHASES]);
+
+/*
+ * Batched solver: `nscenes` independent scenes in one set of particle
+ * arrays, stepped together by one launch sequence.  No counterpart in the
+ * reference (one scene per Simulator).  Scene k owns the index range
+ * [off_k, off_k + counts[k]), off_k = counts[0] + ... + counts[k-1], of every
+ * array, and its own box (ulim[k], llim[k]); all scenes share `params`.
+ * Each scene's result equals, bit for bit, stepping that scene alone with
+ * pbf_step in a single-scene solver, and each scene's arrays are left in
+ * that scene's own sorted order inside its range.  Batched boxes are fixed
+ * (pbf_set_lim / pbf_get_lim / pbf_run_frames with a sweep return
+ * PBF_ERR_STATE); pbf_set_params applies to every scene.  `max_cells` bounds
+ * the cells of all scenes together (<= 0: 4x the initial total).
+ * 1 <= nscenes <= 65535; counts[k] >= 0; total <= 2^30.  Synchronises.
+ */
+int pbf_create_batched(const pbf_params* params, int nscenes, const pbf_vec3* ulim,
+                       const pbf_vec3* llim, const int* counts, long long max_cells, int layout,
+                       pbf_solver** out);
+
+/* Scenes of a solver: *nscenes = 1 for pbf_create; counts (optional, host,
+ * nscenes entries) receives each scene's particle count (for a single-scene
+ * solver: its capacity). */
+int pbf_get_scenes(const pbf_solver* s, int* nscenes, int* counts);
+
+/* One frame of every scene of a batched solver (n = the total count).
+ * pbf_step on a batched solver is the same call with n == the total. */
+int pbf_step_batched(pbf_solver* s, float* pos, float* npos, float* vel, float* nvel,
+                     uint32_t* iid, void* stream);
+
+/*
+ * Graph mode (no counterpart in the reference).  With enable != 0, pbf_step
+ * records its launch sequence into a HIP graph the first time it sees an
+ * (arrays, n, stream) combination, and replays that graph on later calls
+ * with the same combination: one graph launch per step instead of ~30
+ * kernel launches.  Up to 4 graphs are kept (a tic-toc needs 2);
+ * pbf_set_params, pbf_set_lim, pbf_set_shapes and pbf_set_graph drop them.
+ * Steps on the null stream, or on a stream the caller is capturing, run
+ * eagerly.  With timing enabled, graph steps record PBF_PHASE_STEP only.
+ * Results are identical to eager steps.
+ */
+int pbf_set_graph(pbf_solver* s, int enable);
+
+/*
+ * Launch shapes of the neighbour passes (no counterpart in the reference):
+ * light_block = workgroup size of the one-target-per-lane kernels (64, 128
+ * or 256), chunk_blocks = grid cap of the crowded-cell chunk kernels
+ * (64..65536).  Every shape sums in the same order, so results never depend
+ * on them; only the speed does.  Defaults: 256, 2048.
+ */
+typedef struct pbf_shapes {
+  int light_block;
+  int chunk_blocks;
+} pbf_shapes;
+int pbf_set_shapes(pbf_solver* s, const pbf_shapes* shapes);
+int pbf_get_shapes(const pbf_solver* s, pbf_shapes* out);
+
+/*
+ * Autotuner: times one step from the given state for each candidate shape
+ * (light_block in {64, 128, 256} x chunk_blocks in {512, 1024, 2048, 4096}),
+ * `reps` times each (median), on private copies of pos / vel / iid (the
+ * caller's arrays are not modified), checks that every shape produced the
+ * same bits, and installs the fastest.  *best and *best_ms (optional)
+ * receive it and its median step time.  Allocates and synchronises.
+ * n = the particle count (the total for a batched solver).
+ */
+int pbf_autotune(pbf_solver* s, const float* pos, const float* vel, const uint32_t* iid, int n,
+                 int reps, void* stream, pbf_shapes* best, double* best_ms);
 
 /* Message of the last failure on `s` (or of the last pbf_create failure on
  * this thread when s is NULL).  Never NULL. */

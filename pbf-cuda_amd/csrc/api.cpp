@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <new>
 #include <string>
@@ -31,6 +32,8 @@ constexpr float kKernelEps = 1e-4f;
 constexpr float kMaxDp = 0.1f;
 constexpr int kMaxIter = 1 << 16;
 constexpr long long kMaxCellsLimit = 1ll << 31;
+constexpr int kMaxParticles = 1 << 30;
+constexpr int kMaxScenes = 65535;
 
 thread_local std::string g_create_error;
 
@@ -55,6 +58,17 @@ struct Timer {
 
 }  // namespace
 
+// A recorded step: the launch sequence of one (arrays, n, stream)
+// combination as an instantiated HIP graph.
+struct GraphEntry {
+  const void* key[6] = {};
+  int n = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  unsigned long long used = 0;
+};
+constexpr int kMaxGraphs = 4;
+
 struct pbf_solver {
   int device = 0;
   pbf_params prm{};
@@ -62,6 +76,19 @@ struct pbf_solver {
   int capacity = 0;
   int layout = PBF_LAYOUT_FLOAT4;
   long long max_cells = 0;
+
+  // batch of scenes (nscenes == 0: single-scene solver, box in ulim/llim)
+  int nscenes = 0;
+  std::vector<int> counts;
+  std::vector<pbf_vec3> sc_ulim, sc_llim;
+  long long batch_cells = 0;
+  pbf::SceneGeo* geo_d = nullptr;  // per-scene geometry (device)
+  uint16_t* sid_d = nullptr;       // scene of each index (device)
+
+  pbf::Shapes shapes;
+  bool graph_on = false;
+  std::vector<GraphEntry> graphs;
+  unsigned long long graph_clock = 0;
 
   void* arena = nullptr;
   uint32_t *key0 = nullptr, *val0 = nullptr, *key1 = nullptr, *val1 = nullptr;
@@ -132,11 +159,11 @@ int validate_params(pbf_solver* s, const pbf_params* p) {
   return PBF_OK;
 }
 
-// Grid dims and every per-step constant, computed on the host exactly once
-// per step (the reference recomputes the dims in every buildGridHash,
-// fluids/Simulator.cu:187-188, because the box may move).
-int make_consts(pbf_solver* s, const pbf_params& p, pbf_vec3 ul, pbf_vec3 ll,
-                pbf::StepConsts& c) {
+// Grid dims and clamp bounds of one box (the reference recomputes the dims
+// in every buildGridHash, fluids/Simulator.cu:187-188, because the box may
+// move); `base` is the box's first global cell id.
+int make_geo(pbf_solver* s, const pbf_params& p, pbf_vec3 ul, pbf_vec3 ll, long long base,
+             pbf::SceneGeo& g, long long& cells_out) {
   const float ulv[3] = {ul.x, ul.y, ul.z};
   const float llv[3] = {ll.x, ll.y, ll.z};
   int d[3];
@@ -151,18 +178,40 @@ int make_consts(pbf_solver* s, const pbf_params& p, pbf_vec3 ul, pbf_vec3 ll,
     d[a] = (int)q;
   }
   const long long cells = (long long)d[0] * d[1] * d[2];
-  if (cells > kMaxCellsLimit) return fail(s, PBF_ERR_CAPACITY, "grid has %lld cells (> 2^31)", cells);
+  if (cells > kMaxCellsLimit || base + cells > kMaxCellsLimit)
+    return fail(s, PBF_ERR_CAPACITY, "grid has %lld cells (> 2^31)", base + cells);
+  std::memset(&g, 0, sizeof(g));
+  g.llx = ll.x;
+  g.lly = ll.y;
+  g.llz = ll.z;
+  g.ulx = ul.x;
+  g.uly = ul.y;
+  g.ulz = ul.z;
+  // Box clamp of computetpos (fluids/Simulator_kernel.cuh:190-192): the bound
+  // ulim - LIM_EPS is formed in double there; rounding it to FP32 once gives
+  // the same clamped FP32 value.
+  g.lox = (float)((double)ll.x + kLimEps);
+  g.loy = (float)((double)ll.y + kLimEps);
+  g.loz = (float)((double)ll.z + kLimEps);
+  g.hix = (float)((double)ul.x - kLimEps);
+  g.hiy = (float)((double)ul.y - kLimEps);
+  g.hiz = (float)((double)ul.z - kLimEps);
+  g.dx = d[0];
+  g.dy = d[1];
+  g.dz = d[2];
+  g.base = (uint32_t)base;
+  cells_out = cells;
+  return PBF_OK;
+}
 
+// Every per-step constant, computed on the host once per step, for the box
+// (ul, ll) (single scene; a batch overrides cells and ignores g).
+int make_consts(pbf_solver* s, const pbf_params& p, pbf_vec3 ul, pbf_vec3 ll,
+                pbf::StepConsts& c) {
   std::memset(&c, 0, sizeof(c));
-  c.llx = ll.x;
-  c.lly = ll.y;
-  c.llz = ll.z;
-  c.ulx = ul.x;
-  c.uly = ul.y;
-  c.ulz = ul.z;
-  c.dx = d[0];
-  c.dy = d[1];
-  c.dz = d[2];
+  long long cells = 0;
+  const int rc = make_geo(s, p, ul, ll, 0, c.g, cells);
+  if (rc) return rc;
   c.cells = (int)cells;
   c.h = p.h;
   c.h2 = p.h * p.h;
@@ -192,15 +241,6 @@ int make_consts(pbf_solver* s, const pbf_params& p, pbf_vec3 ul, pbf_vec3 ll,
                      : -1;
   c.k_boundary = p.k_boundaryDensity;
   c.wall_c = (float)(2.0 * kPi / 3.0);
-  // Box clamp of computetpos (fluids/Simulator_kernel.cuh:190-192): the bound
-  // ulim - LIM_EPS is formed in double there; rounding it to FP32 once gives
-  // the same clamped FP32 value.
-  c.lox = (float)((double)ll.x + kLimEps);
-  c.loy = (float)((double)ll.y + kLimEps);
-  c.loz = (float)((double)ll.z + kLimEps);
-  c.hix = (float)((double)ul.x - kLimEps);
-  c.hiy = (float)((double)ul.y - kLimEps);
-  c.hiz = (float)((double)ul.z - kLimEps);
   c.max_dp = kMaxDp;
   c.dt = p.dt;
   c.inv_dt = 1.0f / p.dt;  // h_updateVelocity, fluids/Simulator.cu:129
@@ -258,46 +298,171 @@ int check_vec_ptr(pbf_solver* s, const void* p, const char* name) {
   return PBF_OK;
 }
 
-}  // namespace
-
-extern "C" {
-
-int pbf_abi_version(void) { return PBF_ABI_VERSION; }
-
-const char* pbf_status_string(int code) {
-  switch (code) {
-    case PBF_OK: return "PBF_OK";
-    case PBF_ERR_INVALID_ARGUMENT: return "PBF_ERR_INVALID_ARGUMENT";
-    case PBF_ERR_CAPACITY: return "PBF_ERR_CAPACITY";
-    case PBF_ERR_HIP: return "PBF_ERR_HIP";
-    case PBF_ERR_NO_DEVICE: return "PBF_ERR_NO_DEVICE";
-    case PBF_ERR_STATE: return "PBF_ERR_STATE";
-    default: return "PBF_ERR_UNKNOWN";
+// Geometry of every scene of a batch under parameters p: scene k's cell ids
+// follow scene k-1's.
+int make_batch_geo(pbf_solver* s, const pbf_params& p, std::vector<pbf::SceneGeo>& geo,
+                   long long& total) {
+  geo.assign((size_t)s->nscenes, pbf::SceneGeo{});
+  total = 0;
+  for (int k = 0; k < s->nscenes; ++k) {
+    long long cells = 0;
+    const int rc = make_geo(s, p, s->sc_ulim[k], s->sc_llim[k], total, geo[k], cells);
+    if (rc) return rc;
+    total += cells;
   }
+  return PBF_OK;
 }
 
-const char* pbf_last_error(const pbf_solver* s) {
-  return s ? s->err.c_str() : g_create_error.c_str();
+void drop_graphs(pbf_solver* s) {
+  for (auto& g : s->graphs) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (g.graph) (void)hipGraphDestroy(g.graph);
+  }
+  s->graphs.clear();
 }
 
-int pbf_create(const pbf_params* params, pbf_vec3 ulim, pbf_vec3 llim, int capacity,
-               long long max_cells, int layout, pbf_solver** out) {
-  if (!out) return fail(nullptr, PBF_ERR_INVALID_ARGUMENT, "out is NULL");
-  *out = nullptr;
-  int rc = validate_params(nullptr, params);
+// Caller arrays of one step.
+struct StepArrays {
+  float *pos, *npos, *vel, *nvel;
+  uint32_t* iid;
+  int n;
+};
+
+pbf::Scenes scenes_of(const pbf_solver* s) {
+  pbf::Scenes sc;
+  if (s->nscenes) {
+    sc.geo = s->geo_d;
+    sc.sid = s->sid_d;
+  }
+  return sc;
+}
+
+// Constants of the next step, with its cell count checked against max_cells.
+int step_consts(pbf_solver* s, pbf::StepConsts& c) {
+  const int rc = make_consts(s, s->prm, s->ulim, s->llim, c);
   if (rc) return rc;
-  if (capacity < 1 || capacity > (1 << 30))
-    return fail(nullptr, PBF_ERR_INVALID_ARGUMENT, "capacity must be in [1, 2^30], got %d", capacity);
+  if (s->nscenes) c.cells = (int)s->batch_cells;
+  if (c.cells > s->max_cells)
+    return fail(s, PBF_ERR_CAPACITY, "box needs %d cells > max_cells %lld", c.cells, s->max_cells);
+  return PBF_OK;
+}
+
+// The launch sequence of one frame, on validated arguments.  timed: record
+// the phase events (never inside a graph capture).
+void enqueue_step(pbf_solver* s, const StepArrays& a, const pbf::StepConsts& c, hipStream_t st,
+                  bool timed) {
+  auto tb = [&](int ph) {
+    if (timed) tbegin(s, ph, st);
+  };
+  auto te = [&](int ph) {
+    if (timed) tend(s, ph, st);
+  };
+  const int L = s->layout;
+  const int n = a.n;
+  const pbf::Scenes scn = scenes_of(s);
+  const pbf::Shapes& sh = s->shapes;
+  tb(PBF_PHASE_STEP);
+  // advect() + getGridId (Simulator.cpp:50, Simulator.cu:165-193)
+  tb(PBF_PHASE_PREDICT);
+  pbf::launch_predict_key(L, a.pos, a.vel, n, c, scn, s->xp, s->key0, s->val0, s->cnt,
+                          s->counters, st);
+  te(PBF_PHASE_PREDICT);
+  // thrust::sort_by_key (Simulator.cu:196-198)
+  tb(PBF_PHASE_SORT);
+  const int which = pbf::radix_sort_pairs(s->key0, s->val0, s->key1, s->val1, n,
+                                          key_bits_for(c.cells), s->hist, s->totals, st);
+  te(PBF_PHASE_SORT);
+  const uint32_t* sval = which ? s->val1 : s->val0;
+  // memset + computeGridRange (Simulator.cu:200-204), fused with the payload gather
+  tb(PBF_PHASE_GRID);
+  pbf::launch_cell_starts(s->cnt, c.cells, s->tile_sum, s->cs, st);
+  pbf::launch_permute(L, sval, a.pos, s->xp, a.iid, n, s->x0s, s->xa, s->iid_s, st);
+  te(PBF_PHASE_GRID);
+  // niter x correctDensity (Simulator.cpp:62-66, Simulator.cu:213-249);
+  // xl (x, lambda) aliases xp, which is dead after the permute.
+  float4* xl = s->xp;
+  int pass = 0;
+  for (int it = 0; it < s->prm.niter; ++it) {
+    tb(PBF_PHASE_LAMBDA);
+    pbf::launch_lambda(s->xa, s->cs, n, c, scn, xl, it + 1 == s->prm.niter ? s->rho : nullptr,
+                       s->hl, pass++, sh, st);
+    te(PBF_PHASE_LAMBDA);
+    tb(PBF_PHASE_DELTA);
+    pbf::launch_delta(xl, s->cs, n, c, scn, s->xa, s->hl, pass++, sh, st);
+    te(PBF_PHASE_DELTA);
+  }
+  // updateVelocity (Simulator.cpp:72, Simulator.cu:267-274); vr aliases xp.
+  float4* vr = s->xp;
+  tb(PBF_PHASE_VEL);
+  pbf::launch_velocity(L, s->xa, s->x0s, s->iid_s, s->rho, n, c, vr, a.pos, a.vel, a.iid, st);
+  te(PBF_PHASE_VEL);
+  // correctVelocity (Simulator.cpp:77, Simulator.cu:251-265)
+  tb(PBF_PHASE_XSPH);
+  pbf::launch_xsph(L, s->xa, vr, s->cs, n, c, scn, a.npos, a.nvel, s->hl, pass, sh, st);
+  te(PBF_PHASE_XSPH);
+  te(PBF_PHASE_STEP);
+}
+
+// The recorded graph of this (arrays, n, stream) combination, recording it
+// (stream capture of enqueue_step) when there is none; the least recently
+// used of kMaxGraphs entries makes room.
+int find_or_record(pbf_solver* s, const StepArrays& a, const pbf::StepConsts& c, hipStream_t st,
+                   GraphEntry** out) {
+  const void* key[6] = {a.pos, a.npos, a.vel, a.nvel, a.iid, (const void*)st};
+  for (auto& g : s->graphs) {
+    if (g.n == a.n && std::memcmp(g.key, key, sizeof(key)) == 0) {
+      g.used = ++s->graph_clock;
+      *out = &g;
+      return PBF_OK;
+    }
+  }
+  if ((int)s->graphs.size() >= kMaxGraphs) {
+    size_t lru = 0;
+    for (size_t k = 1; k < s->graphs.size(); ++k)
+      if (s->graphs[k].used < s->graphs[lru].used) lru = k;
+    (void)hipGraphExecDestroy(s->graphs[lru].exec);
+    (void)hipGraphDestroy(s->graphs[lru].graph);
+    s->graphs.erase(s->graphs.begin() + (long)lru);
+  }
+  (void)hipGetLastError();
+  hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return fail_hip(s, e, "hipStreamBeginCapture");
+  enqueue_step(s, a, c, st, false);
+  const hipError_t le = hipGetLastError();
+  hipGraph_t graph = nullptr;
+  e = hipStreamEndCapture(st, &graph);
+  if (le != hipSuccess || e != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return fail_hip(s, le != hipSuccess ? le : e, "recording the step graph");
+  }
+  hipGraphExec_t exec = nullptr;
+  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(graph);
+    return fail_hip(s, e, "hipGraphInstantiate");
+  }
+  GraphEntry g;
+  std::memcpy(g.key, key, sizeof(key));
+  g.n = a.n;
+  g.graph = graph;
+  g.exec = exec;
+  g.used = ++s->graph_clock;
+  s->graphs.push_back(g);
+  *out = &s->graphs.back();
+  return PBF_OK;
+}
+
+// Allocation of a solver for `capacity` particles whose initial grid has
+// `cells` cells.  The caller fills in the boxes.
+int create_impl(const pbf_params& params, int capacity, long long cells, long long max_cells,
+                int layout, int nscenes, pbf_solver** out) {
   if (layout != PBF_LAYOUT_FLOAT3 && layout != PBF_LAYOUT_FLOAT4)
     return fail(nullptr, PBF_ERR_INVALID_ARGUMENT, "layout must be 3 or 4, got %d", layout);
-  pbf::StepConsts c;
-  rc = make_consts(nullptr, *params, ulim, llim, c);
-  if (rc) return rc;
   // Simulator.h:13-14 sizes the grid at 4x the initial box's cells.
-  if (max_cells <= 0) max_cells = 4ll * c.cells;
+  if (max_cells <= 0) max_cells = 4ll * cells;
   if (max_cells > kMaxCellsLimit) max_cells = kMaxCellsLimit;
-  if (max_cells < c.cells)
-    return fail(nullptr, PBF_ERR_CAPACITY, "max_cells %lld < cells of the box %d", max_cells, c.cells);
+  if (max_cells < cells)
+    return fail(nullptr, PBF_ERR_CAPACITY, "max_cells %lld < cells of the box %lld", max_cells, cells);
 
   int ndev = 0;
   hipError_t e = hipGetDeviceCount(&ndev);
@@ -309,9 +474,7 @@ int pbf_create(const pbf_params* params, pbf_vec3 ulim, pbf_vec3 llim, int capac
   pbf_solver* s = new (std::nothrow) pbf_solver();
   if (!s) return fail(nullptr, PBF_ERR_INVALID_ARGUMENT, "out of host memory");
   s->device = dev;
-  s->prm = *params;
-  s->ulim = ulim;
-  s->llim = llim;
+  s->prm = params;
   s->capacity = capacity;
   s->layout = layout;
   s->max_cells = max_cells;
@@ -336,7 +499,9 @@ int pbf_create(const pbf_params* params, pbf_vec3 ulim, pbf_vec3 llim, int capac
                o_cnt = carve(4 * 8), o_de = carve(8 * N), o_dp = carve(16 * dblk),
                o_do = carve(16), o_cnt2 = carve(4 * (size_t)max_cells),
                o_cs = carve(4 * ((size_t)max_cells + 1)),
-               o_ts = carve(4 * ((size_t)pbf::scan_tiles(max_cells) + 1));
+               o_ts = carve(4 * ((size_t)pbf::scan_tiles(max_cells) + 1)),
+               o_geo = carve(sizeof(pbf::SceneGeo) * (size_t)(nscenes > 0 ? nscenes : 1)),
+               o_sid = carve(nscenes > 0 ? 2 * N : 2);
   void* arena = nullptr;
   if ((e = hipMalloc(&arena, off)) != hipSuccess) {
     delete s;
@@ -367,6 +532,8 @@ int pbf_create(const pbf_params* params, pbf_vec3 ulim, pbf_vec3 llim, int capac
   s->cnt = (uint32_t*)(b + o_cnt2);
   s->cs = (uint32_t*)(b + o_cs);
   s->tile_sum = (uint32_t*)(b + o_ts);
+  s->geo_d = (pbf::SceneGeo*)(b + o_geo);
+  s->sid_d = (uint16_t*)(b + o_sid);
   // the per-cell counts start (and stay, between steps) all zero
   pbf::launch_fill_u32(s->cnt, max_cells, 0u, nullptr);
   if ((e = hipDeviceSynchronize()) != hipSuccess) {
@@ -378,10 +545,125 @@ int pbf_create(const pbf_params* params, pbf_vec3 ulim, pbf_vec3 llim, int capac
   return PBF_OK;
 }
 
+}  // namespace
+
+extern "C" {
+
+
+int pbf_abi_version(void) { return PBF_ABI_VERSION; }
+
+const char* pbf_status_string(int code) {
+  switch (code) {
+    case PBF_OK: return "PBF_OK";
+    case PBF_ERR_INVALID_ARGUMENT: return "PBF_ERR_INVALID_ARGUMENT";
+    case PBF_ERR_CAPACITY: return "PBF_ERR_CAPACITY";
+    case PBF_ERR_HIP: return "PBF_ERR_HIP";
+    case PBF_ERR_NO_DEVICE: return "PBF_ERR_NO_DEVICE";
+    case PBF_ERR_STATE: return "PBF_ERR_STATE";
+    default: return "PBF_ERR_UNKNOWN";
+  }
+}
+
+const char* pbf_last_error(const pbf_solver* s) {
+  return s ? s->err.c_str() : g_create_error.c_str();
+}
+
+int pbf_create(const pbf_params* params, pbf_vec3 ulim, pbf_vec3 llim, int capacity,
+               long long max_cells, int layout, pbf_solver** out) {
+  if (!out) return fail(nullptr, PBF_ERR_INVALID_ARGUMENT, "out is NULL");
+  *out = nullptr;
+  int rc = validate_params(nullptr, params);
+  if (rc) return rc;
+  if (capacity < 1 || capacity > kMaxParticles)
+    return fail(nullptr, PBF_ERR_INVALID_ARGUMENT, "capacity must be in [1, 2^30], got %d", capacity);
+  pbf::StepConsts c;
+  rc = make_consts(nullptr, *params, ulim, llim, c);
+  if (rc) return rc;
+  pbf_solver* s = nullptr;
+  rc = create_impl(*params, capacity, c.cells, max_cells, layout, 0, &s);
+  if (rc) return rc;
+  s->ulim = ulim;
+  s->llim = llim;
+  *out = s;
+  return PBF_OK;
+}
+
+int pbf_create_batched(const pbf_params* params, int nscenes, const pbf_vec3* ulim,
+                       const pbf_vec3* llim, const int* counts, long long max_cells, int layout,
+                       pbf_solver** out) {
+  if (!out) return fail(nullptr, PBF_ERR_INVALID_ARGUMENT, "out is NULL");
+  *out = nullptr;
+  int rc = validate_params(nullptr, params);
+  if (rc) return rc;
+  if (nscenes < 1 || nscenes > kMaxScenes)
+    return fail(nullptr, PBF_ERR_INVALID_ARGUMENT, "nscenes must be in [1, %d], got %d", kMaxScenes,
+                nscenes);
+  if (!ulim || !llim || !counts)
+    return fail(nullptr, PBF_ERR_INVALID_ARGUMENT, "ulim, llim and counts must not be NULL");
+  long long total = 0;
+  for (int k = 0; k < nscenes; ++k) {
+    if (counts[k] < 0)
+      return fail(nullptr, PBF_ERR_INVALID_ARGUMENT, "counts[%d] = %d is negative", k, counts[k]);
+    total += counts[k];
+  }
+  if (total < 1 || total > kMaxParticles)
+    return fail(nullptr, PBF_ERR_INVALID_ARGUMENT, "total particle count must be in [1, 2^30], got %lld",
+                total);
+  pbf_solver tmp;
+  tmp.nscenes = nscenes;
+  tmp.sc_ulim.assign(ulim, ulim + nscenes);
+  tmp.sc_llim.assign(llim, llim + nscenes);
+  std::vector<pbf::SceneGeo> geo;
+  long long cells = 0;
+  rc = make_batch_geo(&tmp, *params, geo, cells);
+  if (rc) {
+    g_create_error = tmp.err;
+    return rc;
+  }
+  pbf_solver* s = nullptr;
+  rc = create_impl(*params, (int)total, cells, max_cells, layout, nscenes, &s);
+  if (rc) return rc;
+  s->nscenes = nscenes;
+  s->counts.assign(counts, counts + nscenes);
+  s->sc_ulim = tmp.sc_ulim;
+  s->sc_llim = tmp.sc_llim;
+  s->ulim = ulim[0];
+  s->llim = llim[0];
+  s->batch_cells = cells;
+  // scene table and per-index scene ids: set once (counts are fixed)
+  std::vector<uint16_t> sid((size_t)total);
+  size_t o = 0;
+  for (int k = 0; k < nscenes; ++k)
+    for (int q = 0; q < counts[k]; ++q) sid[o++] = (uint16_t)k;
+  hipError_t e = hipMemcpy(s->sid_d, sid.data(), sid.size() * sizeof(uint16_t), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(s->geo_d, geo.data(), geo.size() * sizeof(pbf::SceneGeo), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    g_create_error = std::string("uploading the scene table: ") + hipGetErrorString(e);
+    pbf_destroy(s);
+    return PBF_ERR_HIP;
+  }
+  *out = s;
+  return PBF_OK;
+}
+
+int pbf_get_scenes(const pbf_solver* s, int* nscenes, int* counts) {
+  if (!s || !nscenes) return fail(nullptr, PBF_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (s->nscenes == 0) {
+    *nscenes = 1;
+    if (counts) counts[0] = s->capacity;
+  } else {
+    *nscenes = s->nscenes;
+    if (counts) std::memcpy(counts, s->counts.data(), sizeof(int) * (size_t)s->nscenes);
+  }
+  return PBF_OK;
+}
+
 void pbf_destroy(pbf_solver* s) {
   if (!s) return;
   {
     DeviceGuard g(s->device);
+    drop_graphs(s);
     for (auto& r : s->timer.recs) {
       (void)hipEventDestroy(r.a);
       (void)hipEventDestroy(r.b);
@@ -401,9 +683,25 @@ int pbf_set_params(pbf_solver* s, const pbf_params* params) {
   pbf::StepConsts c;
   rc = make_consts(s, *params, s->ulim, s->llim, c);
   if (rc) return rc;
-  if (c.cells > s->max_cells)
+  if (s->nscenes) {
+    // the grid dims follow h: re-derive and upload the scene table
+    std::vector<pbf::SceneGeo> geo;
+    long long cells = 0;
+    if ((rc = make_batch_geo(s, *params, geo, cells))) return rc;
+    if (cells > s->max_cells)
+      return fail(s, PBF_ERR_CAPACITY, "h gives %lld cells > max_cells %lld", cells, s->max_cells);
+    DeviceGuard g(s->device);
+    hipError_t e = hipDeviceSynchronize();  // no step in flight reads the old table
+    if (e == hipSuccess)
+      e = hipMemcpy(s->geo_d, geo.data(), geo.size() * sizeof(pbf::SceneGeo),
+                    hipMemcpyHostToDevice);
+    if (e != hipSuccess) return fail_hip(s, e, "uploading the scene table");
+    s->batch_cells = cells;
+  } else if (c.cells > s->max_cells) {
     return fail(s, PBF_ERR_CAPACITY, "h gives %d cells > max_cells %lld", c.cells, s->max_cells);
+  }
   s->prm = *params;
+  drop_graphs(s);
   return PBF_OK;
 }
 
@@ -415,11 +713,15 @@ int pbf_get_params(const pbf_solver* s, pbf_params* out) {
 
 int pbf_set_lim(pbf_solver* s, pbf_vec3 ulim, pbf_vec3 llim) {
   if (!s) return fail(nullptr, PBF_ERR_INVALID_ARGUMENT, "solver is NULL");
+  if (s->nscenes) return fail(s, PBF_ERR_STATE, "the boxes of a batched solver are fixed");
   pbf::StepConsts c;
   int rc = make_consts(s, s->prm, ulim, llim, c);
   if (rc) return rc;
   if (c.cells > s->max_cells)
     return fail(s, PBF_ERR_CAPACITY, "box needs %d cells > max_cells %lld", c.cells, s->max_cells);
+  if (ulim.x != s->ulim.x || ulim.y != s->ulim.y || ulim.z != s->ulim.z || llim.x != s->llim.x ||
+      llim.y != s->llim.y || llim.z != s->llim.z)
+    drop_graphs(s);
   s->ulim = ulim;
   s->llim = llim;
   return PBF_OK;
@@ -427,6 +729,8 @@ int pbf_set_lim(pbf_solver* s, pbf_vec3 ulim, pbf_vec3 llim) {
 
 int pbf_get_lim(const pbf_solver* s, pbf_vec3* ulim, pbf_vec3* llim) {
   if (!s || !ulim || !llim) return fail(nullptr, PBF_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (s->nscenes)
+    return fail(const_cast<pbf_solver*>(s), PBF_ERR_STATE, "a batched solver has one box per scene");
   *ulim = s->ulim;
   *llim = s->llim;
   return PBF_OK;
@@ -438,6 +742,9 @@ int pbf_step(pbf_solver* s, float* pos, float* npos, float* vel, float* nvel, ui
   if (n < 0) return fail(s, PBF_ERR_INVALID_ARGUMENT, "n must be >= 0, got %d", n);
   if (n > s->capacity)
     return fail(s, PBF_ERR_CAPACITY, "n = %d exceeds capacity %d", n, s->capacity);
+  if (s->nscenes && n != s->capacity)
+    return fail(s, PBF_ERR_INVALID_ARGUMENT,
+                "a batched solver steps all its particles: n must be %d, got %d", s->capacity, n);
   if (n == 0) return PBF_OK;
   int rc;
   if ((rc = check_vec_ptr(s, pos, "pos")) || (rc = check_vec_ptr(s, npos, "npos")) ||
@@ -451,59 +758,180 @@ int pbf_step(pbf_solver* s, float* pos, float* npos, float* vel, float* nvel, ui
         if (p[a] == p[b]) return fail(s, PBF_ERR_INVALID_ARGUMENT, "particle arrays must be distinct");
   }
   pbf::StepConsts c;
-  if ((rc = make_consts(s, s->prm, s->ulim, s->llim, c))) return rc;
-  if (c.cells > s->max_cells)
-    return fail(s, PBF_ERR_CAPACITY, "box needs %d cells > max_cells %lld", c.cells, s->max_cells);
+  if ((rc = step_consts(s, c))) return rc;
   DeviceGuard g(s->device);
   if (!g.ok) return fail(s, PBF_ERR_HIP, "hipSetDevice(%d) failed", s->device);
   hipStream_t st = (hipStream_t)stream;
-  const int L = s->layout;
+  const StepArrays arr{pos, npos, vel, nvel, iid, n};
 
-  tbegin(s, PBF_PHASE_STEP, st);
-  // advect() + getGridId (Simulator.cpp:50, Simulator.cu:165-193)
-  tbegin(s, PBF_PHASE_PREDICT, st);
   if (s->cnt_dirty) pbf::launch_fill_u32(s->cnt, s->max_cells, 0u, st);
   s->cnt_dirty = true;
-  pbf::launch_predict_key(L, pos, vel, n, c, s->xp, s->key0, s->val0, s->cnt, s->counters, st);
-  tend(s, PBF_PHASE_PREDICT, st);
-  // thrust::sort_by_key (Simulator.cu:196-198)
-  tbegin(s, PBF_PHASE_SORT, st);
-  const int which = pbf::radix_sort_pairs(s->key0, s->val0, s->key1, s->val1, n,
-                                          key_bits_for(c.cells), s->hist, s->totals, st);
-  tend(s, PBF_PHASE_SORT, st);
-  const uint32_t* sval = which ? s->val1 : s->val0;
-  // memset + computeGridRange (Simulator.cu:200-204), fused with the payload gather
-  tbegin(s, PBF_PHASE_GRID, st);
-  pbf::launch_cell_starts(s->cnt, c.cells, s->tile_sum, s->cs, st);
-  pbf::launch_permute(L, sval, pos, s->xp, iid, n, s->x0s, s->xa, s->iid_s, st);
-  tend(s, PBF_PHASE_GRID, st);
-  // niter x correctDensity (Simulator.cpp:62-66, Simulator.cu:213-249);
-  // xl (x, lambda) aliases xp, which is dead after the permute.
-  float4* xl = s->xp;
-  int pass = 0;
-  for (int it = 0; it < s->prm.niter; ++it) {
-    tbegin(s, PBF_PHASE_LAMBDA, st);
-    pbf::launch_lambda(s->xa, s->cs, n, c, xl, it + 1 == s->prm.niter ? s->rho : nullptr,
-                       s->hl, pass++, st);
-    tend(s, PBF_PHASE_LAMBDA, st);
-    tbegin(s, PBF_PHASE_DELTA, st);
-    pbf::launch_delta(xl, s->cs, n, c, s->xa, s->hl, pass++, st);
-    tend(s, PBF_PHASE_DELTA, st);
+  if (s->graph_on && st != nullptr) {
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    hipError_t e = hipStreamIsCapturing(st, &cap);
+    if (e != hipSuccess) return fail_hip(s, e, "hipStreamIsCapturing");
+    if (cap == hipStreamCaptureStatusNone) {
+      GraphEntry* ge = nullptr;
+      if ((rc = find_or_record(s, arr, c, st, &ge))) return rc;
+      tbegin(s, PBF_PHASE_STEP, st);
+      e = hipGraphLaunch(ge->exec, st);
+      tend(s, PBF_PHASE_STEP, st);
+      if (e != hipSuccess) return fail_hip(s, e, "hipGraphLaunch");
+      s->cnt_dirty = false;
+      return PBF_OK;
+    }
   }
-  // updateVelocity (Simulator.cpp:72, Simulator.cu:267-274); vr aliases xp.
-  float4* vr = s->xp;
-  tbegin(s, PBF_PHASE_VEL, st);
-  pbf::launch_velocity(L, s->xa, s->x0s, s->iid_s, s->rho, n, c, vr, pos, vel, iid, st);
-  tend(s, PBF_PHASE_VEL, st);
-  // correctVelocity (Simulator.cpp:77, Simulator.cu:251-265)
-  tbegin(s, PBF_PHASE_XSPH, st);
-  pbf::launch_xsph(L, s->xa, vr, s->cs, n, c, npos, nvel, s->hl, pass, st);
-  tend(s, PBF_PHASE_XSPH, st);
-  tend(s, PBF_PHASE_STEP, st);
-
+  enqueue_step(s, arr, c, st, true);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail_hip(s, e, "kernel launch");
   s->cnt_dirty = false;
+  return PBF_OK;
+}
+
+int pbf_step_batched(pbf_solver* s, float* pos, float* npos, float* vel, float* nvel,
+                     uint32_t* iid, void* stream) {
+  if (!s) return fail(nullptr, PBF_ERR_INVALID_ARGUMENT, "solver is NULL");
+  if (!s->nscenes) return fail(s, PBF_ERR_STATE, "not a batched solver");
+  return pbf_step(s, pos, npos, vel, nvel, iid, s->capacity, stream);
+}
+
+int pbf_set_graph(pbf_solver* s, int enable) {
+  if (!s) return fail(nullptr, PBF_ERR_INVALID_ARGUMENT, "solver is NULL");
+  DeviceGuard g(s->device);
+  drop_graphs(s);
+  s->graph_on = enable != 0;
+  return PBF_OK;
+}
+
+int pbf_set_shapes(pbf_solver* s, const pbf_shapes* sh) {
+  if (!s) return fail(nullptr, PBF_ERR_INVALID_ARGUMENT, "solver is NULL");
+  if (!sh) return fail(s, PBF_ERR_INVALID_ARGUMENT, "shapes is NULL");
+  if (sh->light_block != 64 && sh->light_block != 128 && sh->light_block != 256)
+    return fail(s, PBF_ERR_INVALID_ARGUMENT, "light_block must be 64, 128 or 256, got %d",
+                sh->light_block);
+  if (sh->chunk_blocks < 64 || sh->chunk_blocks > 65536)
+    return fail(s, PBF_ERR_INVALID_ARGUMENT, "chunk_blocks must be in [64, 65536], got %d",
+                sh->chunk_blocks);
+  if (sh->light_block != s->shapes.light_block || sh->chunk_blocks != s->shapes.chunk_blocks) {
+    DeviceGuard g(s->device);
+    drop_graphs(s);
+  }
+  s->shapes.light_block = sh->light_block;
+  s->shapes.chunk_blocks = sh->chunk_blocks;
+  return PBF_OK;
+}
+
+int pbf_get_shapes(const pbf_solver* s, pbf_shapes* out) {
+  if (!s || !out) return fail(nullptr, PBF_ERR_INVALID_ARGUMENT, "NULL argument");
+  out->light_block = s->shapes.light_block;
+  out->chunk_blocks = s->shapes.chunk_blocks;
+  return PBF_OK;
+}
+
+int pbf_autotune(pbf_solver* s, const float* pos, const float* vel, const uint32_t* iid, int n,
+                 int reps, void* stream, pbf_shapes* best, double* best_ms) {
+  if (!s) return fail(nullptr, PBF_ERR_INVALID_ARGUMENT, "solver is NULL");
+  if (n < 1 || n > s->capacity)
+    return fail(s, PBF_ERR_INVALID_ARGUMENT, "n must be in [1, capacity = %d], got %d", s->capacity,
+                n);
+  if (s->nscenes && n != s->capacity)
+    return fail(s, PBF_ERR_INVALID_ARGUMENT, "a batched solver needs n = %d, got %d", s->capacity,
+                n);
+  if (reps < 1 || reps > 100) return fail(s, PBF_ERR_INVALID_ARGUMENT, "reps must be in [1, 100]");
+  int rc;
+  if ((rc = check_vec_ptr(s, pos, "pos")) || (rc = check_vec_ptr(s, vel, "vel"))) return rc;
+  if (!iid) return fail(s, PBF_ERR_INVALID_ARGUMENT, "iid is NULL");
+  DeviceGuard g(s->device);
+  if (!g.ok) return fail(s, PBF_ERR_HIP, "hipSetDevice(%d) failed", s->device);
+  hipStream_t st = (hipStream_t)stream;
+  const size_t vb = (size_t)n * (s->layout == PBF_LAYOUT_FLOAT4 ? 16 : 12);
+  const size_t ib = (size_t)n * 4;
+  const size_t vba = align_up(vb), iba = align_up(ib);
+  char* work = nullptr;
+  hipError_t e = hipMalloc((void**)&work, 4 * vba + iba);
+  if (e != hipSuccess) return fail_hip(s, e, "hipMalloc of autotune arrays");
+  float *wp = (float*)work, *wnp = (float*)(work + vba), *wv = (float*)(work + 2 * vba),
+        *wnv = (float*)(work + 3 * vba);
+  uint32_t* wi = (uint32_t*)(work + 4 * vba);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  const pbf::Shapes saved = s->shapes;
+  const bool graph_saved = s->graph_on, timer_saved = s->timer.on;
+  s->graph_on = false;
+  s->timer.on = false;
+  std::vector<char> ref, out;
+  pbf::Shapes win = saved;
+  double win_ms = 1e300;
+  rc = PBF_OK;
+  if ((e = hipEventCreate(&e0)) != hipSuccess || (e = hipEventCreate(&e1)) != hipSuccess) {
+    rc = fail_hip(s, e, "hipEventCreate");
+  }
+  const int blocks[3] = {64, 128, 256};
+  const int grids[4] = {512, 1024, 2048, 4096};
+  bool first = true;
+  for (int bi = 0; bi < 3 && rc == PBF_OK; ++bi) {
+    for (int gi = 0; gi < 4 && rc == PBF_OK; ++gi) {
+      s->shapes.light_block = blocks[bi];
+      s->shapes.chunk_blocks = grids[gi];
+      std::vector<double> t;
+      for (int r = 0; r < reps + (first ? 1 : 0) && rc == PBF_OK; ++r) {
+        if ((e = hipMemcpyAsync(wp, pos, vb, hipMemcpyDeviceToDevice, st)) != hipSuccess ||
+            (e = hipMemcpyAsync(wv, vel, vb, hipMemcpyDeviceToDevice, st)) != hipSuccess ||
+            (e = hipMemcpyAsync(wi, iid, ib, hipMemcpyDeviceToDevice, st)) != hipSuccess ||
+            (e = hipEventRecord(e0, st)) != hipSuccess) {
+          rc = fail_hip(s, e, "autotune copy");
+          break;
+        }
+        if ((rc = pbf_step(s, wp, wnp, wv, wnv, wi, n, stream))) break;
+        float ms = 0.0f;
+        if ((e = hipEventRecord(e1, st)) != hipSuccess || (e = hipEventSynchronize(e1)) != hipSuccess ||
+            (e = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess) {
+          rc = fail_hip(s, e, "autotune timing");
+          break;
+        }
+        if (!(first && r == 0)) t.push_back(ms);  // the very first run warms up
+      }
+      if (rc) break;
+      // every shape must give the same bits (the canonical summation order)
+      out.resize(2 * vb + ib);
+      if ((e = hipMemcpy(out.data(), wnp, vb, hipMemcpyDeviceToHost)) != hipSuccess ||
+          (e = hipMemcpy(out.data() + vb, wnv, vb, hipMemcpyDeviceToHost)) != hipSuccess ||
+          (e = hipMemcpy(out.data() + 2 * vb, wi, ib, hipMemcpyDeviceToHost)) != hipSuccess) {
+        rc = fail_hip(s, e, "autotune result copy");
+        break;
+      }
+      if (first) {
+        ref.swap(out);
+      } else if (std::memcmp(ref.data(), out.data(), ref.size()) != 0) {
+        rc = fail(s, PBF_ERR_STATE, "launch shape (%d, %d) changed the result", blocks[bi],
+                  grids[gi]);
+        break;
+      }
+      first = false;
+      std::sort(t.begin(), t.end());
+      const double med = t[t.size() / 2];
+      if (med < win_ms) {
+        win_ms = med;
+        win = s->shapes;
+      }
+    }
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipStreamSynchronize(st);
+  (void)hipFree(work);
+  s->graph_on = graph_saved;
+  s->timer.on = timer_saved;
+  drop_graphs(s);
+  if (rc) {
+    s->shapes = saved;
+    return rc;
+  }
+  s->shapes = win;
+  if (best) {
+    best->light_block = win.light_block;
+    best->chunk_blocks = win.chunk_blocks;
+  }
+  if (best_ms) *best_ms = win_ms;
   return PBF_OK;
 }
 
@@ -531,6 +959,7 @@ int pbf_run_frames(pbf_solver* s, float* pos_a, float* vel_a, float* pos_b, floa
   if (!which) return fail(s, PBF_ERR_INVALID_ARGUMENT, "which is NULL");
   *which = 0;
   if (nframes < 0) return fail(s, PBF_ERR_INVALID_ARGUMENT, "nframes must be >= 0");
+  if (sweep && s->nscenes) return fail(s, PBF_ERR_STATE, "the boxes of a batched solver are fixed");
   for (int k = 0; k < nframes; ++k) {
     const int frame = first_frame + k;
     if (sweep) {
@@ -556,25 +985,28 @@ int pbf_density_error(pbf_solver* s, const float* pos, int n, void* stream, doub
   if (n < 0) return fail(s, PBF_ERR_INVALID_ARGUMENT, "n must be >= 0, got %d", n);
   if (n > s->capacity)
     return fail(s, PBF_ERR_CAPACITY, "n = %d exceeds capacity %d", n, s->capacity);
+  if (s->nscenes && n != s->capacity)
+    return fail(s, PBF_ERR_INVALID_ARGUMENT, "a batched solver needs n = %d, got %d", s->capacity,
+                n);
   if (n == 0) return PBF_OK;
   int rc;
   if ((rc = check_vec_ptr(s, pos, "pos"))) return rc;
   pbf::StepConsts c;
-  if ((rc = make_consts(s, s->prm, s->ulim, s->llim, c))) return rc;
-  if (c.cells > s->max_cells)
-    return fail(s, PBF_ERR_CAPACITY, "box needs %d cells > max_cells %lld", c.cells, s->max_cells);
+  if ((rc = step_consts(s, c))) return rc;
   DeviceGuard g(s->device);
   if (!g.ok) return fail(s, PBF_ERR_HIP, "hipSetDevice(%d) failed", s->device);
   hipStream_t st = (hipStream_t)stream;
+  const pbf::Scenes scn = scenes_of(s);
   if (s->cnt_dirty) pbf::launch_fill_u32(s->cnt, s->max_cells, 0u, st);
   s->cnt_dirty = true;
-  pbf::launch_key_only(s->layout, pos, n, c, s->xp, s->key0, s->val0, s->cnt, s->counters, st);
+  pbf::launch_key_only(s->layout, pos, n, c, scn, s->xp, s->key0, s->val0, s->cnt, s->counters,
+                       st);
   const int which = pbf::radix_sort_pairs(s->key0, s->val0, s->key1, s->val1, n,
                                           key_bits_for(c.cells), s->hist, s->totals, st);
   pbf::launch_cell_starts(s->cnt, c.cells, s->tile_sum, s->cs, st);
   pbf::launch_permute_pos(which ? s->val1 : s->val0, s->xp, n, s->xa, st);
-  pbf::launch_density_error(s->xa, s->cs, n, c, s->dens_err, s->dens_partial, s->dens_out,
-                            s->hl, st);
+  pbf::launch_density_error(s->xa, s->cs, n, c, scn, s->dens_err, s->dens_partial, s->dens_out,
+                            s->hl, s->shapes, st);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail_hip(s, e, "kernel launch");
   s->cnt_dirty = false;

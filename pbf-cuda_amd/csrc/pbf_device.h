@@ -6,14 +6,25 @@
 
 namespace pbf {
 
+// Geometry of one scene's box and grid.  A batch of scenes shares one global
+// cell-id space: scene s owns ids [base, base + dx*dy*dz), so one stable sort
+// of all particles by cell id leaves every scene's particles in its own
+// contiguous range, in the order sorting that scene alone would give.
+struct alignas(16) SceneGeo {
+  float llx, lly, llz;  // box lower corner (llim)
+  float ulx, uly, ulz;  // box upper corner (ulim)
+  float lox, loy, loz;  // box clamp lower: llim + LIM_EPS, rounded to FP32
+  float hix, hiy, hiz;  // box clamp upper: ulim - LIM_EPS, rounded to FP32
+  int dx, dy, dz;       // grid dims, (int)ceilf((ulim-llim)/h)
+  uint32_t base;        // first global cell id of the scene
+};
+
 // Per-step constants, built on the host once per step and passed by value
 // (kernarg segment).  Field meanings follow the reference's functors
 // (fluids/Simulator.cu:22-161) and constants (fluids/helper.h:4-9).
 struct StepConsts {
-  float llx, lly, llz;     // box lower corner (llim)
-  float ulx, uly, ulz;     // box upper corner (ulim)
-  int dx, dy, dz;          // grid dims, (int)ceilf((ulim-llim)/h)
-  int cells;               // dx*dy*dz
+  SceneGeo g;              // the scene of a single-scene step
+  int cells;               // cells of all scenes
   float h, h2;             // kernel radius, h*h
   float h2_outer;          // prefilter bound: r2 >= h2_outer => both kernels are 0
   float keps;              // KERNAL_EPS
@@ -27,12 +38,25 @@ struct StepConsts {
   int n_corr_int;          // n_corr when it is an integer in [0,16], else -1
   float k_boundary;        // k_boundaryDensity
   float wall_c;            // 2 pi / 3 (DensityBoundary)
-  float lox, loy, loz;     // box clamp lower: llim + LIM_EPS, rounded to FP32
-  float hix, hiy, hiz;     // box clamp upper: ulim - LIM_EPS, rounded to FP32
   float max_dp;            // MAX_DP
   float dt, inv_dt;        // dt, 1/dt (FP32, as h_updateVelocity)
   float gz;                // -g
   float c_xsph;
+};
+
+// Scene sources: where a kernel finds the geometry of particle i's scene.
+// OneScene: the single scene in the kernarg constants.  SceneTable: a batch;
+// sid[i] is the scene of index i, valid in input and in sorted order alike
+// (both keep scene s in its own index range).
+struct OneScene {
+  __device__ __forceinline__ SceneGeo at(uint32_t, const StepConsts& c) const { return c.g; }
+};
+struct SceneTable {
+  const SceneGeo* __restrict__ geo;
+  const uint16_t* __restrict__ sid;
+  __device__ __forceinline__ SceneGeo at(uint32_t i, const StepConsts&) const {
+    return geo[sid[i]];
+  }
 };
 
 // Workgroup index remapped so that each of the 8 XCDs (which receive
@@ -60,14 +84,15 @@ struct Cell {
   int x, y, z;
 };
 
-__device__ __forceinline__ Cell cell_of(float x, float y, float z, const StepConsts& c) {
-  return Cell{cell_coord(x, c.llx, c.h, c.dx), cell_coord(y, c.lly, c.h, c.dy),
-              cell_coord(z, c.llz, c.h, c.dz)};
+__device__ __forceinline__ Cell cell_of(float x, float y, float z, const SceneGeo& g, float h) {
+  return Cell{cell_coord(x, g.llx, h, g.dx), cell_coord(y, g.lly, h, g.dy),
+              cell_coord(z, g.llz, h, g.dz)};
 }
 
-// Cell id, x-major and z fastest (xyzToId, fluids/Simulator.cu:52).
-__device__ __forceinline__ uint32_t cell_id(const Cell& q, const StepConsts& c) {
-  return (uint32_t)((q.x * c.dy + q.y) * c.dz + q.z);
+// Cell id, x-major and z fastest (xyzToId, fluids/Simulator.cu:52), offset
+// by the scene's base.
+__device__ __forceinline__ uint32_t cell_id(const Cell& q, const SceneGeo& g) {
+  return g.base + (uint32_t)((q.x * g.dy + q.y) * g.dz + q.z);
 }
 
 // Candidate neighbours of a particle whose query cell is q, in the
@@ -82,13 +107,13 @@ __device__ __forceinline__ uint32_t cell_id(const Cell& q, const StepConsts& c) 
 // empty, it is an empty range.
 __device__ __forceinline__ uint2 column_range(const Cell& q, int k,
                                               const uint32_t* __restrict__ cs,
-                                              const StepConsts& c) {
+                                              const SceneGeo& g) {
   const int X = q.x + k / 3 - 1;
   const int Y = q.y + k % 3 - 1;
-  if (X < 0 || X >= c.dx || Y < 0 || Y >= c.dy) return make_uint2(0u, 0u);
-  const int col = (X * c.dy + Y) * c.dz;
+  if (X < 0 || X >= g.dx || Y < 0 || Y >= g.dy) return make_uint2(0u, 0u);
+  const uint32_t col = g.base + (uint32_t)((X * g.dy + Y) * g.dz);
   const int z0 = q.z > 0 ? q.z - 1 : 0;
-  const int z1 = q.z + 1 < c.dz ? q.z + 1 : c.dz - 1;
+  const int z1 = q.z + 1 < g.dz ? q.z + 1 : g.dz - 1;
   return make_uint2(cs[col + z0], cs[col + z1 + 1]);
 }
 
@@ -102,84 +127,6 @@ __device__ __forceinline__ uint2 column_range(const Cell& q, int k,
 // lengths instead of one lane thousands of dependent gathers.
 constexpr uint32_t kLightMax = 256;
 constexpr uint32_t kChunk = 64;
-
-// Calls add(j, load(j)) for j in [j0, j1) in order, with the loads of four
-// consecutive candidates issued together so their latencies overlap.
-template <class Load, class Add>
-__device__ __forceinline__ void sweep_range(uint32_t j0, uint32_t j1, Load&& load, Add&& add) {
-#pragma unroll 1
-  for (uint32_t j = j0; j < j1; j += 4) {
-    const uint32_t last = j1 - 1;
-    const auto d0 = load(j);
-    const auto d1 = load(j + 1 < j1 ? j + 1 : last);
-    const auto d2 = load(j + 2 < j1 ? j + 2 : last);
-    const auto d3 = load(j + 3 < j1 ? j + 3 : last);
-    add(j, d0);
-    if (j + 1 < j1) add(j + 1, d1);
-    if (j + 2 < j1) add(j + 2, d2);
-    if (j + 3 < j1) add(j + 3, d3);
-  }
-}
-
-// Same visit as sweep_range restricted to the candidates that can
-// contribute: blocks of up to 32 candidates are first screened with the
-// cheap test hit(j, data) (loads four at a time) into a bit mask, then
-// add(j, data) runs only for the set bits, in ascending j, with those
-// candidates reloaded four at a time.  Candidates failing hit() contribute
-// exactly nothing in add(), so the sums are identical to the plain sweep;
-// the expensive part of add() no longer runs for every candidate of every
-// lane whenever any lane of the wavefront has a hit.
-template <class Load, class Hit, class Add>
-__device__ __forceinline__ void sweep_filtered(uint32_t j0, uint32_t j1, Load&& load, Hit&& hit,
-                                               Add&& add) {
-#pragma unroll 1
-  for (uint32_t b = j0; b < j1; b += 32) {
-    const uint32_t e = j1 - b < 32u ? j1 : b + 32u;
-    const uint32_t last = e - 1;
-    uint32_t mask = 0;
-#pragma unroll 1
-    for (uint32_t j = b; j < e; j += 4) {
-      const auto d0 = load(j);
-      const auto d1 = load(j + 1 < e ? j + 1 : last);
-      const auto d2 = load(j + 2 < e ? j + 2 : last);
-      const auto d3 = load(j + 3 < e ? j + 3 : last);
-      const uint32_t u = j - b;
-      if (hit(j, d0)) mask |= 1u << u;
-      if (j + 1 < e && hit(j + 1, d1)) mask |= 2u << u;
-      if (j + 2 < e && hit(j + 2, d2)) mask |= 4u << u;
-      if (j + 3 < e && hit(j + 3, d3)) mask |= 8u << u;
-    }
-#pragma unroll 1
-    while (mask) {
-      const uint32_t t0 = __builtin_ctz(mask);
-      mask &= mask - 1u;
-      uint32_t t1 = t0, t2 = t0, t3 = t0;
-      const bool h1 = mask != 0u;
-      if (h1) {
-        t1 = __builtin_ctz(mask);
-        mask &= mask - 1u;
-      }
-      const bool h2 = mask != 0u;
-      if (h2) {
-        t2 = __builtin_ctz(mask);
-        mask &= mask - 1u;
-      }
-      const bool h3 = mask != 0u;
-      if (h3) {
-        t3 = __builtin_ctz(mask);
-        mask &= mask - 1u;
-      }
-      const auto d0 = load(b + t0);
-      const auto d1 = load(b + t1);
-      const auto d2 = load(b + t2);
-      const auto d3 = load(b + t3);
-      add(b + t0, d0);
-      if (h1) add(b + t1, d1);
-      if (h2) add(b + t2, d2);
-      if (h3) add(b + t3, d3);
-    }
-  }
-}
 
 // A work-item's candidate sequence as a compact column table in LDS: entry
 // k < nk is (first slot of the k-th non-empty column, sequence position of
@@ -196,12 +143,12 @@ struct SeqTable {
 // returns the candidate count m.
 __device__ __forceinline__ uint32_t build_seq(const SeqTable& st, const Cell& q,
                                               const uint32_t* __restrict__ cs,
-                                              const StepConsts& c) {
+                                              const SceneGeo& g) {
   uint32_t m = 0;
   int nk = 0;
 #pragma unroll
   for (int k = 0; k < 9; ++k) {
-    const uint2 r = column_range(q, k, cs, c);
+    const uint2 r = column_range(q, k, cs, g);
     if (r.y > r.x) {
       st.tab[nk * st.stride] = make_uint2(r.x, m);
       ++nk;
@@ -367,13 +314,14 @@ __device__ __forceinline__ float wall_density_at(float d, const StepConsts& c) {
   return ((c.wall_c * (c.h - d)) * (c.h - d)) * (c.h + d);
 }
 
-__device__ __forceinline__ float boundary_density(float x, float y, float z, const StepConsts& c) {
-  float s = wall_density_at(c.ulx - x, c);
-  s = s + wall_density_at(x - c.llx, c);
-  s = s + wall_density_at(c.uly - y, c);
-  s = s + wall_density_at(y - c.lly, c);
-  s = s + wall_density_at(c.ulz - z, c);
-  s = s + wall_density_at(z - c.llz, c);
+__device__ __forceinline__ float boundary_density(float x, float y, float z, const SceneGeo& g,
+                                                  const StepConsts& c) {
+  float s = wall_density_at(g.ulx - x, c);
+  s = s + wall_density_at(x - g.llx, c);
+  s = s + wall_density_at(g.uly - y, c);
+  s = s + wall_density_at(y - g.lly, c);
+  s = s + wall_density_at(g.ulz - z, c);
+  s = s + wall_density_at(z - g.llz, c);
   return s;
 }
 

@@ -10,6 +10,21 @@
 
 namespace pbf {
 
+// Scene geometry of a launch: geo == nullptr is the single scene of
+// StepConsts::g; otherwise a batch, with the device table geo and the
+// per-index scene ids sid.
+struct Scenes {
+  const SceneGeo* geo = nullptr;
+  const uint16_t* sid = nullptr;
+};
+
+// Launch shapes of the neighbour passes.  Results do not depend on them
+// (every shape sums in the canonical order); only the speed does.
+struct Shapes {
+  int light_block = 256;   // workgroup size of the light kernels: 64, 128 or 256
+  int chunk_blocks = 2048;  // grid cap of the chunk kernels (grid-stride)
+};
+
 // sort.hip
 int radix_tiles(int n);
 // Sorts (k0,v0) by the low key_bits bits of k0, stably; (k1,v1) is scratch.
@@ -22,10 +37,10 @@ int radix_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, int
 // numbers the neighbour passes of one step from 0 (predict / key_only clear
 // the parity-0 counters, each light kernel clears the other parity's).
 void launch_predict_key(int layout, const float* pos, const float* vel, int n, const StepConsts& c,
-                        float4* xp, uint32_t* key, uint32_t* val, uint32_t* cnt,
-                        uint32_t* counters, hipStream_t st);
-void launch_key_only(int layout, const float* pos, int n, const StepConsts& c, float4* xp,
-                     uint32_t* key, uint32_t* val, uint32_t* cnt, uint32_t* counters,
+                        const Scenes& scn, float4* xp, uint32_t* key, uint32_t* val,
+                        uint32_t* cnt, uint32_t* counters, hipStream_t st);
+void launch_key_only(int layout, const float* pos, int n, const StepConsts& c, const Scenes& scn,
+                     float4* xp, uint32_t* key, uint32_t* val, uint32_t* cnt, uint32_t* counters,
                      hipStream_t st);
 void launch_fill_u32(uint32_t* a, long long n, uint32_t v, hipStream_t st);
 // Cell-start table from the per-cell counts (which it clears): cs holds
@@ -38,20 +53,22 @@ void launch_permute(int layout, const uint32_t* sval, const float* pos, const fl
                     hipStream_t st);
 void launch_permute_pos(const uint32_t* sval, const float4* xp, int n, float4* xa,
                         hipStream_t st);
-void launch_lambda(const float4* xa, const uint32_t* cs, int n, const StepConsts& c, float4* xl,
-                   float* rho, const HeavyLists& hl, int pass, hipStream_t st);
-void launch_delta(const float4* xl, const uint32_t* cs, int n, const StepConsts& c, float4* xa,
-                  const HeavyLists& hl, int pass, hipStream_t st);
+void launch_lambda(const float4* xa, const uint32_t* cs, int n, const StepConsts& c,
+                   const Scenes& scn, float4* xl, float* rho, const HeavyLists& hl, int pass,
+                   const Shapes& sh, hipStream_t st);
+void launch_delta(const float4* xl, const uint32_t* cs, int n, const StepConsts& c,
+                  const Scenes& scn, float4* xa, const HeavyLists& hl, int pass, const Shapes& sh,
+                  hipStream_t st);
 void launch_velocity(int layout, const float4* xa, const float4* x0s, const uint32_t* iid_s,
                      const float* rho, int n, const StepConsts& c, float4* vr, float* pos,
                      float* vel, uint32_t* iid, hipStream_t st);
 void launch_xsph(int layout, const float4* xa, const float4* vr, const uint32_t* cs, int n,
-                 const StepConsts& c, float* npos, float* nvel, const HeavyLists& hl, int pass,
-                 hipStream_t st);
+                 const StepConsts& c, const Scenes& scn, float* npos, float* nvel,
+                 const HeavyLists& hl, int pass, const Shapes& sh, hipStream_t st);
 // err holds n doubles, partial 2*density_partials(n), out (device) 2.
 int density_partials(int n);
 void launch_density_error(const float4* xa, const uint32_t* cs, int n, const StepConsts& c,
-                          double* err, double* partial, double* out, const HeavyLists& hl,
-                          hipStream_t st);
+                          const Scenes& scn, double* err, double* partial, double* out,
+                          const HeavyLists& hl, const Shapes& sh, hipStream_t st);
 
 }  // namespace pbf

@@ -29,7 +29,6 @@ namespace pbf {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kWavesPerBlock = kBlock / 64;
 
 inline int blocks_for(long long n) { return (int)((n + kBlock - 1) / kBlock); }
 
@@ -95,9 +94,10 @@ struct LambdaPass {
     tot.gl2 = tot.gl2 + p.gl2;
   }
   static __device__ __forceinline__ void finish(const Args& a, const Target& t, const Acc& s,
-                                                const StepConsts& c) {
+                                                const StepConsts& c, const SceneGeo& g) {
     float rho = s.rho;
-    if (c.k_boundary != 0.0f) rho = rho + c.k_boundary * boundary_density(t.x.x, t.x.y, t.x.z, c);
+    if (c.k_boundary != 0.0f)
+      rho = rho + c.k_boundary * boundary_density(t.x.x, t.x.y, t.x.z, g, c);
     float den = fmaf(s.gx, s.gx, s.gl2);
     den = fmaf(s.gy, s.gy, den);
     den = fmaf(s.gz, s.gz, den);
@@ -160,13 +160,13 @@ struct DeltaPass {
     tot.pz = tot.pz + p.pz;
   }
   static __device__ __forceinline__ void finish(const Args& a, const Target& t, const Acc& s,
-                                                const StepConsts& c) {
+                                                const StepConsts& c, const SceneGeo& g) {
     const float px = fminf(fmaxf(s.px / c.pho0, -c.max_dp), c.max_dp);
     const float py = fminf(fmaxf(s.py / c.pho0, -c.max_dp), c.max_dp);
     const float pz = fminf(fmaxf(s.pz / c.pho0, -c.max_dp), c.max_dp);
-    const float x = fmaxf(fminf(t.x.x + px, c.hix), c.lox);
-    const float y = fmaxf(fminf(t.x.y + py, c.hiy), c.loy);
-    const float z = fmaxf(fminf(t.x.z + pz, c.hiz), c.loz);
+    const float x = fmaxf(fminf(t.x.x + px, g.hix), g.lox);
+    const float y = fmaxf(fminf(t.x.y + py, g.hiy), g.loy);
+    const float z = fmaxf(fminf(t.x.z + pz, g.hiz), g.loz);
     a.xa[t.i] = make_float4(x, y, z, 0.0f);
   }
 };
@@ -227,7 +227,7 @@ struct XsphPass {
     tot.az = tot.az + p.az;
   }
   static __device__ __forceinline__ void finish(const Args& a, const Target& t, const Acc& s,
-                                                const StepConsts& c) {
+                                                const StepConsts& c, const SceneGeo&) {
     store_vec<L>(a.nvel, t.i, fmaf(c.c_xsph, s.ax, t.v.x), fmaf(c.c_xsph, s.ay, t.v.y),
                  fmaf(c.c_xsph, s.az, t.v.z));
     store_vec<L>(a.npos, t.i, t.x.x, t.x.y, t.x.z);
@@ -268,9 +268,10 @@ struct DensityPass {
     tot.rho = tot.rho + p.rho;
   }
   static __device__ __forceinline__ void finish(const Args& a, const Target& t, const Acc& s,
-                                                const StepConsts& c) {
+                                                const StepConsts& c, const SceneGeo& g) {
     float rho = s.rho;
-    if (c.k_boundary != 0.0f) rho = rho + c.k_boundary * boundary_density(t.x.x, t.x.y, t.x.z, c);
+    if (c.k_boundary != 0.0f)
+      rho = rho + c.k_boundary * boundary_density(t.x.x, t.x.y, t.x.z, g, c);
     a.err[t.i] = fabs((double)rho / (double)c.pho0 - 1.0);
   }
 };
@@ -279,29 +280,32 @@ struct DensityPass {
 // Generic neighbour-pass kernels over a pass policy.
 // ---------------------------------------------------------------------------
 
-// Light: one work-item per target.  A target with more than kLightMax
-// candidates is heavy: it gets one chunk item per kChunk candidates and a
-// record, appended with one atomic per wavefront each.  counters[pass & 1]
-// belong to this pass; the kernel clears the other parity's counters for the
-// next pass (their previous users finished earlier on the stream).  If the
-// chunk list is full the lane sums its target itself, still chunk by chunk,
-// so the canonical order holds either way.
-template <class P>
-__global__ __launch_bounds__(kBlock) void k_light(typename P::Args a, int n, StepConsts c,
+// Light: one work-item per target; the workgroup size is a launch shape
+// (64, 128 or 256), the column tables live in dynamic LDS.  A target with
+// more than kLightMax candidates is heavy: it gets one chunk item per kChunk
+// candidates and a record, appended with one atomic per wavefront each.
+// counters[pass & 1] belong to this pass; the kernel clears the other
+// parity's counters for the next pass (their previous users finished earlier
+// on the stream).  If the chunk list is full the lane sums its target itself,
+// still chunk by chunk, so the canonical order holds either way.
+template <class P, class S>
+__global__ __launch_bounds__(kBlock) void k_light(typename P::Args a, int n, StepConsts c, S sc,
                                                   const uint32_t* __restrict__ cs, HeavyLists hl,
                                                   int pass) {
-  __shared__ uint2 seq[10][kBlock];
+  extern __shared__ uint2 seq_dyn[];
   uint32_t* cnt = hl.counters + 4 * (pass & 1);
   if (blockIdx.x == 0 && threadIdx.x < 4) hl.counters[4 * ((pass + 1) & 1) + threadIdx.x] = 0u;
   const int tid = threadIdx.x;
-  const int i = xcd_block_index() * kBlock + tid;
+  const int nb = (int)blockDim.x;
+  const int i = xcd_block_index() * nb + tid;
   const bool active = i < n;
-  const SeqTable st{&seq[0][tid], kBlock};
+  const SeqTable st{&seq_dyn[tid], nb};
   typename P::Target t{};
   uint32_t m = 0;
   if (active) {
     t = P::target(a, (uint32_t)i, c);
-    m = build_seq(st, cell_of(t.x.x, t.x.y, t.x.z, c), cs, c);
+    const SceneGeo g = sc.at((uint32_t)i, c);
+    m = build_seq(st, cell_of(t.x.x, t.x.y, t.x.z, g, c.h), cs, g);
   }
   const bool is_heavy = active && m > kLightMax;
   const uint32_t nch = is_heavy ? (m + kChunk - 1) / kChunk : 0u;
@@ -347,13 +351,14 @@ __global__ __launch_bounds__(kBlock) void k_light(typename P::Args a, int n, Ste
       P::accumulate(s, part);
     }
   }
-  P::finish(a, t, s, c);
+  P::finish(a, t, s, c, sc.at(t.i, c));
 }
 
 // One work-item per chunk item: the partial sum of kChunk consecutive
-// candidates of a heavy target.
-template <class P>
-__global__ __launch_bounds__(kBlock) void k_chunks(typename P::Args a, StepConsts c,
+// candidates of a heavy target.  Grid-stride over the items; the grid size
+// is a launch shape.
+template <class P, class S>
+__global__ __launch_bounds__(kBlock) void k_chunks(typename P::Args a, StepConsts c, S sc,
                                                    const uint32_t* __restrict__ cs, HeavyLists hl,
                                                    int pass) {
   __shared__ uint2 seq[10][kBlock];
@@ -365,7 +370,8 @@ __global__ __launch_bounds__(kBlock) void k_chunks(typename P::Args a, StepConst
     const uint2 it = hl.items[u];
     if (it.x == ~0u) continue;
     const typename P::Target t = P::target(a, it.x, c);
-    const uint32_t m = build_seq(st, cell_of(t.x.x, t.x.y, t.x.z, c), cs, c);
+    const SceneGeo g = sc.at(it.x, c);
+    const uint32_t m = build_seq(st, cell_of(t.x.x, t.x.y, t.x.z, g, c.h), cs, g);
     typename P::Acc part;
     const uint32_t s0 = it.y * kChunk;
     const uint32_t s1 = s0 + kChunk < m ? s0 + kChunk : m;
@@ -380,8 +386,8 @@ __global__ __launch_bounds__(kBlock) void k_chunks(typename P::Args a, StepConst
 }
 
 // One work-item per heavy target: chunk partials added in chunk order.
-template <class P>
-__global__ __launch_bounds__(kBlock) void k_combine(typename P::Args a, StepConsts c,
+template <class P, class S>
+__global__ __launch_bounds__(kBlock) void k_combine(typename P::Args a, StepConsts c, S sc,
                                                     HeavyLists hl, int pass) {
   const uint32_t* cnt = hl.counters + 4 * (pass & 1);
   const uint32_t nrec = cnt[0];
@@ -394,7 +400,7 @@ __global__ __launch_bounds__(kBlock) void k_combine(typename P::Args a, StepCons
       P::accumulate(
           s, *reinterpret_cast<const typename P::Acc*>(hl.partials +
                                                        (size_t)(rec.y + k) * kPartialWords));
-    P::finish(a, t, s, c);
+    P::finish(a, t, s, c, sc.at(rec.x, c));
   }
 }
 
@@ -403,12 +409,25 @@ int grid_for_items(long long items, int max_blocks) {
   return (int)(b < 1 ? 1 : (b > max_blocks ? max_blocks : b));
 }
 
+template <class P, class S>
+void launch_pass_s(const typename P::Args& a, int n, const StepConsts& c, const S& sc,
+                   const uint32_t* cs, const HeavyLists& hl, int pass, const Shapes& sh,
+                   hipStream_t st) {
+  const int lb = sh.light_block;
+  k_light<P, S><<<(n + lb - 1) / lb, lb, 10 * lb * sizeof(uint2), st>>>(a, n, c, sc, cs, hl, pass);
+  k_chunks<P, S><<<grid_for_items(hl.item_cap, sh.chunk_blocks), kBlock, 0, st>>>(a, c, sc, cs,
+                                                                                    hl, pass);
+  k_combine<P, S><<<grid_for_items(n, 512), kBlock, 0, st>>>(a, c, sc, hl, pass);
+}
+
 template <class P>
-void launch_pass(const typename P::Args& a, int n, const StepConsts& c, const uint32_t* cs,
-                 const HeavyLists& hl, int pass, hipStream_t st) {
-  k_light<P><<<blocks_for(n), kBlock, 0, st>>>(a, n, c, cs, hl, pass);
-  k_chunks<P><<<grid_for_items(hl.item_cap, 2048), kBlock, 0, st>>>(a, c, cs, hl, pass);
-  k_combine<P><<<grid_for_items(n, 512), kBlock, 0, st>>>(a, c, hl, pass);
+void launch_pass(const typename P::Args& a, int n, const StepConsts& c, const Scenes& scn,
+                 const uint32_t* cs, const HeavyLists& hl, int pass, const Shapes& sh,
+                 hipStream_t st) {
+  if (scn.geo)
+    launch_pass_s<P>(a, n, c, SceneTable{scn.geo, scn.sid}, cs, hl, pass, sh, st);
+  else
+    launch_pass_s<P>(a, n, c, OneScene{}, cs, hl, pass, sh, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -439,10 +458,11 @@ __device__ __forceinline__ void count_cell(uint32_t k, bool valid, uint32_t* __r
 // reference overwrites it before any read (fluids/Simulator.cu:267-274).
 // Also counts particles per cell (the input of the cell-start scan) and
 // clears the heavy counters of the first neighbour pass.
-template <int L>
+template <int L, class S>
 __global__ __launch_bounds__(kBlock) void k_predict_key(const float* __restrict__ pos,
                                                         const float* __restrict__ vel, int n,
-                                                        StepConsts c, float4* __restrict__ xp,
+                                                        StepConsts c, S sc,
+                                                        float4* __restrict__ xp,
                                                         uint32_t* __restrict__ key,
                                                         uint32_t* __restrict__ val,
                                                         uint32_t* __restrict__ cnt,
@@ -459,7 +479,8 @@ __global__ __launch_bounds__(kBlock) void k_predict_key(const float* __restrict_
   const float x = fmaf(c.dt, vx, p.x);
   const float y = fmaf(c.dt, vy, p.y);
   const float z = fmaf(c.dt, vz, p.z);
-  const uint32_t k = cell_id(cell_of(x, y, z, c), c);
+  const SceneGeo g = sc.at((uint32_t)ii, c);
+  const uint32_t k = cell_id(cell_of(x, y, z, g, c.h), g);
   count_cell(k, i < n, cnt);
   if (i >= n) return;
   xp[i] = make_float4(x, y, z, 0.0f);
@@ -468,9 +489,9 @@ __global__ __launch_bounds__(kBlock) void k_predict_key(const float* __restrict_
 }
 
 // Cell key of positions as they are (the fresh grid of the density error).
-template <int L>
+template <int L, class S>
 __global__ __launch_bounds__(kBlock) void k_key_only(const float* __restrict__ pos, int n,
-                                                     StepConsts c, float4* __restrict__ xp,
+                                                     StepConsts c, S sc, float4* __restrict__ xp,
                                                      uint32_t* __restrict__ key,
                                                      uint32_t* __restrict__ val,
                                                      uint32_t* __restrict__ cnt,
@@ -480,7 +501,8 @@ __global__ __launch_bounds__(kBlock) void k_key_only(const float* __restrict__ p
   if (blockIdx.x * kBlock >= n) return;
   const int ii = i < n ? i : n - 1;
   const float4 p = load_vec<L>(pos, ii);
-  const uint32_t k = cell_id(cell_of(p.x, p.y, p.z, c), c);
+  const SceneGeo g = sc.at((uint32_t)ii, c);
+  const uint32_t k = cell_id(cell_of(p.x, p.y, p.z, g, c.h), g);
   count_cell(k, i < n, cnt);
   if (i >= n) return;
   xp[i] = make_float4(p.x, p.y, p.z, 0.0f);
@@ -701,22 +723,52 @@ __global__ __launch_bounds__(kRedBlock) void k_reduce_partials(const double* __r
 
 }  // namespace
 
-void launch_predict_key(int layout, const float* pos, const float* vel, int n, const StepConsts& c,
-                        float4* xp, uint32_t* key, uint32_t* val, uint32_t* cnt,
-                        uint32_t* counters, hipStream_t st) {
-  if (layout == 3)
-    k_predict_key<3><<<blocks_for(n), kBlock, 0, st>>>(pos, vel, n, c, xp, key, val, cnt, counters);
-  else
-    k_predict_key<4><<<blocks_for(n), kBlock, 0, st>>>(pos, vel, n, c, xp, key, val, cnt, counters);
+template <int L, class S>
+void predict_key_s(const float* pos, const float* vel, int n, const StepConsts& c, const S& sc,
+                   float4* xp, uint32_t* key, uint32_t* val, uint32_t* cnt, uint32_t* counters,
+                   hipStream_t st) {
+  k_predict_key<L, S><<<blocks_for(n), kBlock, 0, st>>>(pos, vel, n, c, sc, xp, key, val, cnt,
+                                                        counters);
 }
 
-void launch_key_only(int layout, const float* pos, int n, const StepConsts& c, float4* xp,
-                     uint32_t* key, uint32_t* val, uint32_t* cnt, uint32_t* counters,
+template <int L, class S>
+void key_only_s(const float* pos, int n, const StepConsts& c, const S& sc, float4* xp,
+                uint32_t* key, uint32_t* val, uint32_t* cnt, uint32_t* counters, hipStream_t st) {
+  k_key_only<L, S><<<blocks_for(n), kBlock, 0, st>>>(pos, n, c, sc, xp, key, val, cnt, counters);
+}
+
+void launch_predict_key(int layout, const float* pos, const float* vel, int n, const StepConsts& c,
+                        const Scenes& scn, float4* xp, uint32_t* key, uint32_t* val,
+                        uint32_t* cnt, uint32_t* counters, hipStream_t st) {
+  const SceneTable tab{scn.geo, scn.sid};
+  if (layout == 3) {
+    if (scn.geo)
+      predict_key_s<3>(pos, vel, n, c, tab, xp, key, val, cnt, counters, st);
+    else
+      predict_key_s<3>(pos, vel, n, c, OneScene{}, xp, key, val, cnt, counters, st);
+  } else {
+    if (scn.geo)
+      predict_key_s<4>(pos, vel, n, c, tab, xp, key, val, cnt, counters, st);
+    else
+      predict_key_s<4>(pos, vel, n, c, OneScene{}, xp, key, val, cnt, counters, st);
+  }
+}
+
+void launch_key_only(int layout, const float* pos, int n, const StepConsts& c, const Scenes& scn,
+                     float4* xp, uint32_t* key, uint32_t* val, uint32_t* cnt, uint32_t* counters,
                      hipStream_t st) {
-  if (layout == 3)
-    k_key_only<3><<<blocks_for(n), kBlock, 0, st>>>(pos, n, c, xp, key, val, cnt, counters);
-  else
-    k_key_only<4><<<blocks_for(n), kBlock, 0, st>>>(pos, n, c, xp, key, val, cnt, counters);
+  const SceneTable tab{scn.geo, scn.sid};
+  if (layout == 3) {
+    if (scn.geo)
+      key_only_s<3>(pos, n, c, tab, xp, key, val, cnt, counters, st);
+    else
+      key_only_s<3>(pos, n, c, OneScene{}, xp, key, val, cnt, counters, st);
+  } else {
+    if (scn.geo)
+      key_only_s<4>(pos, n, c, tab, xp, key, val, cnt, counters, st);
+    else
+      key_only_s<4>(pos, n, c, OneScene{}, xp, key, val, cnt, counters, st);
+  }
 }
 
 void launch_fill_u32(uint32_t* a, long long n, uint32_t v, hipStream_t st) {
@@ -750,14 +802,16 @@ void launch_permute_pos(const uint32_t* sval, const float4* xp, int n, float4* x
   k_permute_pos<<<blocks_for(n), kBlock, 0, st>>>(sval, xp, n, xa);
 }
 
-void launch_lambda(const float4* xa, const uint32_t* cs, int n, const StepConsts& c, float4* xl,
-                   float* rho, const HeavyLists& hl, int pass, hipStream_t st) {
-  launch_pass<LambdaPass>(LambdaPass::Args{xa, xl, rho}, n, c, cs, hl, pass, st);
+void launch_lambda(const float4* xa, const uint32_t* cs, int n, const StepConsts& c,
+                   const Scenes& scn, float4* xl, float* rho, const HeavyLists& hl, int pass,
+                   const Shapes& sh, hipStream_t st) {
+  launch_pass<LambdaPass>(LambdaPass::Args{xa, xl, rho}, n, c, scn, cs, hl, pass, sh, st);
 }
 
-void launch_delta(const float4* xl, const uint32_t* cs, int n, const StepConsts& c, float4* xa,
-                  const HeavyLists& hl, int pass, hipStream_t st) {
-  launch_pass<DeltaPass>(DeltaPass::Args{xl, xa}, n, c, cs, hl, pass, st);
+void launch_delta(const float4* xl, const uint32_t* cs, int n, const StepConsts& c,
+                  const Scenes& scn, float4* xa, const HeavyLists& hl, int pass, const Shapes& sh,
+                  hipStream_t st) {
+  launch_pass<DeltaPass>(DeltaPass::Args{xl, xa}, n, c, scn, cs, hl, pass, sh, st);
 }
 
 void launch_velocity(int layout, const float4* xa, const float4* x0s, const uint32_t* iid_s,
@@ -770,20 +824,22 @@ void launch_velocity(int layout, const float4* xa, const float4* x0s, const uint
 }
 
 void launch_xsph(int layout, const float4* xa, const float4* vr, const uint32_t* cs, int n,
-                 const StepConsts& c, float* npos, float* nvel, const HeavyLists& hl, int pass,
-                 hipStream_t st) {
+                 const StepConsts& c, const Scenes& scn, float* npos, float* nvel,
+                 const HeavyLists& hl, int pass, const Shapes& sh, hipStream_t st) {
   if (layout == 3)
-    launch_pass<XsphPass<3>>(XsphPass<3>::Args{xa, vr, npos, nvel}, n, c, cs, hl, pass, st);
+    launch_pass<XsphPass<3>>(XsphPass<3>::Args{xa, vr, npos, nvel}, n, c, scn, cs, hl, pass, sh,
+                             st);
   else
-    launch_pass<XsphPass<4>>(XsphPass<4>::Args{xa, vr, npos, nvel}, n, c, cs, hl, pass, st);
+    launch_pass<XsphPass<4>>(XsphPass<4>::Args{xa, vr, npos, nvel}, n, c, scn, cs, hl, pass, sh,
+                             st);
 }
 
 int density_partials(int n) { return (n + kRedBlock * kRedItems - 1) / (kRedBlock * kRedItems); }
 
 void launch_density_error(const float4* xa, const uint32_t* cs, int n, const StepConsts& c,
-                          double* err, double* partial, double* out, const HeavyLists& hl,
-                          hipStream_t st) {
-  launch_pass<DensityPass>(DensityPass::Args{xa, err}, n, c, cs, hl, 0, st);
+                          const Scenes& scn, double* err, double* partial, double* out,
+                          const HeavyLists& hl, const Shapes& sh, hipStream_t st) {
+  launch_pass<DensityPass>(DensityPass::Args{xa, err}, n, c, scn, cs, hl, 0, sh, st);
   const int nb = density_partials(n);
   k_reduce_err<<<nb, kRedBlock, 0, st>>>(err, n, partial);
   k_reduce_partials<<<1, kRedBlock, 0, st>>>(partial, nb, n, out);

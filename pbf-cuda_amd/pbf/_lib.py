@@ -56,6 +56,12 @@ class PbfSweep(ctypes.Structure):
                 ("w", ctypes.c_float), ("start_frame", ctypes.c_int)]
 
 
+class PbfShapes(ctypes.Structure):
+    """struct pbf_shapes: launch shapes of the neighbour passes."""
+
+    _fields_ = [("light_block", ctypes.c_int), ("chunk_blocks", ctypes.c_int)]
+
+
 class PBFError(RuntimeError):
     """A non-zero status from libpbf.so, with the library's message."""
 
@@ -86,6 +92,17 @@ _SIGNATURES = [
     ("pbf_set_timing", ctypes.c_int, [_P, ctypes.c_int]),
     ("pbf_get_timing", ctypes.c_int,
      [_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_longlong)]),
+    ("pbf_create_batched", ctypes.c_int,
+     [ctypes.POINTER(PbfParams), ctypes.c_int, ctypes.POINTER(PbfVec3), ctypes.POINTER(PbfVec3),
+      ctypes.POINTER(ctypes.c_int), ctypes.c_longlong, ctypes.c_int, ctypes.POINTER(_P)]),
+    ("pbf_get_scenes", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
+    ("pbf_step_batched", ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P]),
+    ("pbf_set_graph", ctypes.c_int, [_P, ctypes.c_int]),
+    ("pbf_set_shapes", ctypes.c_int, [_P, ctypes.POINTER(PbfShapes)]),
+    ("pbf_get_shapes", ctypes.c_int, [_P, ctypes.POINTER(PbfShapes)]),
+    ("pbf_autotune", ctypes.c_int,
+     [_P, _P, _P, _P, ctypes.c_int, ctypes.c_int, _P, ctypes.POINTER(PbfShapes),
+      ctypes.POINTER(ctypes.c_double)]),
     ("pbf_last_error", ctypes.c_char_p, [_P]),
     ("pbf_status_string", ctypes.c_char_p, [ctypes.c_int]),
     ("pbf_abi_version", ctypes.c_int, []),

@@ -122,6 +122,45 @@ def block(counts: Sequence[int], box: Sequence[float], seed: int, h: float = 0.1
                  (0.0, 0.0, 0.0), f"block_{n}")
 
 
+def batch_scenes(nscenes: int = 64, seed: int = SEED_BASE + 3, h: float = 0.1,
+                 nmin: int = 4096, nmax: int = 262144, fill=(0.2, 0.5)) -> List[Scene]:
+    """Independent box scenes of uneven size.  Scene k: N_k =
+    round(exp(U(ln nmin, ln nmax))) particles and fill fraction f_k = U(fill),
+    both from SplitMix64(seed) outputs 2k and 2k+1; a cubic box [0, L]^3 with
+    L = h * ceil(0.5 * cbrt(N_k / f_k)) (so N_k lattice sites at spacing 0.5h
+    fill about f_k of it); the particles are the first N_k sites (x outer, z
+    inner) of an m x m x ceil(N_k / m^2) lattice block, m = ceil(cbrt(N_k)),
+    in the floor corner, spacing 0.5h, jitter +-0.05h, zero velocity."""
+    u = uniform01(seed, 0, 2 * nscenes).reshape(nscenes, 2)
+    out: List[Scene] = []
+    for k in range(nscenes):
+        n = int(round(np.exp(np.log(nmin) + u[k, 0] * (np.log(nmax) - np.log(nmin)))))
+        f = fill[0] + u[k, 1] * (fill[1] - fill[0])
+        cells = int(np.ceil(0.5 * np.cbrt(n / f)))
+        m = int(np.ceil(np.cbrt(n)))
+        nz = -(-n // (m * m))
+        side = float(np.float32(cells * h))
+        if m * 0.5 * h > side or nz * 0.5 * h > side:
+            raise ValueError(f"scene {k}: block does not fit its box")
+        pos = lattice((m, m, nz), (0.0, 0.0, 0.0), 0.5 * h, 0.05 * h,
+                      seed ^ ((k + 1) << 32), 0)[:n]
+        out.append(Scene(pos, np.zeros_like(pos), np.arange(n, dtype=np.uint32),
+                         (side, side, side), (0.0, 0.0, 0.0), f"batch{k}_{n}"))
+    return out
+
+
+def concat(scenes: Sequence[Scene], layout: int):
+    """The batched arrays of `scenes`: pos, vel (N_total, layout), iid, and
+    the per-scene counts, boxes and offsets."""
+    pv = [s.as_layout(layout) for s in scenes]
+    pos = np.concatenate([p for p, _ in pv])
+    vel = np.concatenate([v for _, v in pv])
+    iid = np.concatenate([s.iid for s in scenes])
+    counts = [s.n for s in scenes]
+    offsets = np.concatenate([[0], np.cumsum(counts)[:-1]]).astype(np.int64)
+    return pos, vel, iid, counts, [s.ulim for s in scenes], [s.llim for s in scenes], offsets
+
+
 # ---- binary snapshot -----------------------------------------------------
 # Little-endian: magic b"PBFS", u32 version=1, u32 n, u32 stride (3|4),
 # f32 ulim[3], f32 llim[3], then pos (n*stride f32), vel (n*stride f32),

@@ -12,7 +12,7 @@ import ctypes
 from typing import Optional, Sequence
 
 from . import _lib
-from ._lib import PBFError, PbfParams, PbfSweep, PbfVec3
+from ._lib import PBFError, PbfParams, PbfShapes, PbfSweep, PbfVec3
 from .params import FluidParams
 
 
@@ -197,3 +197,91 @@ class Simulator:
         h = self._handle()
         _lib.check(self._lib.pbf_get_timing(h, ms, cnt), h)
         return {name: (float(ms[k]), int(cnt[k])) for k, name in enumerate(_lib.PHASES)}
+
+    def set_graph(self, enable: bool) -> None:
+        """Graph mode: record each (arrays, n, stream) step once as a HIP graph
+        and replay it (one graph launch per step)."""
+        h = self._handle()
+        _lib.check(self._lib.pbf_set_graph(h, 1 if enable else 0), h)
+
+    def set_shapes(self, light_block: int, chunk_blocks: int) -> None:
+        """Launch shapes of the neighbour passes (results never depend on them)."""
+        sh = PbfShapes(int(light_block), int(chunk_blocks))
+        h = self._handle()
+        _lib.check(self._lib.pbf_set_shapes(h, ctypes.byref(sh)), h)
+
+    def get_shapes(self):
+        sh = PbfShapes()
+        h = self._handle()
+        _lib.check(self._lib.pbf_get_shapes(h, ctypes.byref(sh)), h)
+        return int(sh.light_block), int(sh.chunk_blocks)
+
+    def autotune(self, pos, vel, iid, nparticle: int, reps: int = 3, stream=None):
+        """Times one step from (pos, vel, iid) per candidate launch shape on
+        private copies, checks every shape gives the same bits, installs the
+        fastest.  Returns ((light_block, chunk_blocks), median ms).
+        Synchronises; the arrays are not modified."""
+        n = int(nparticle)
+        L = self.layout
+        best = PbfShapes()
+        ms = ctypes.c_double(0.0)
+        h = self._handle()
+        _lib.check(self._lib.pbf_autotune(h, _ptr(pos, L, "pos", n, True),
+                                          _ptr(vel, L, "vel", n, True),
+                                          _ptr(iid, L, "iid", n, False), n, int(reps),
+                                          _stream_handle(stream), ctypes.byref(best),
+                                          ctypes.byref(ms)), h)
+        return (int(best.light_block), int(best.chunk_blocks)), float(ms.value)
+
+
+class BatchedSimulator(Simulator):
+    """Many independent box scenes stepped together (pbf_create_batched).
+
+    Scene k owns the index range [offsets[k], offsets[k] + counts[k]) of
+    every particle array and its own box (ulims[k], llims[k]); all scenes
+    share `params`.  Each scene's result equals stepping it alone in a
+    Simulator, bit for bit."""
+
+    def __init__(self, params: FluidParams, ulims, llims, counts, max_cells: int = 0,
+                 layout: int = _lib.PBF_LAYOUT_FLOAT4):
+        self._lib = _lib.load()
+        self._h = ctypes.c_void_p()
+        self.layout = int(layout)
+        k = len(counts)
+        if len(ulims) != k or len(llims) != k:
+            raise ValueError("ulims, llims and counts must have one entry per scene")
+        self.counts = [int(c) for c in counts]
+        self.offsets = [0] * k
+        for q in range(1, k):
+            self.offsets[q] = self.offsets[q - 1] + self.counts[q - 1]
+        self.capacity = sum(self.counts)
+        u = (PbfVec3 * k)(*[_vec3(v) for v in ulims])
+        l = (PbfVec3 * k)(*[_vec3(v) for v in llims])
+        c = (ctypes.c_int * k)(*self.counts)
+        p = params.to_c()
+        rc = self._lib.pbf_create_batched(ctypes.byref(p), k, u, l, c, int(max_cells), self.layout,
+                                          ctypes.byref(self._h))
+        _lib.check(rc, None)
+
+    @property
+    def nscenes(self) -> int:
+        return len(self.counts)
+
+    def step_batched(self, pos, npos, vel, nvel, iid, stream=None) -> None:
+        """One frame of every scene (pbf_step_batched)."""
+        n = self.capacity
+        L = self.layout
+        args = (_ptr(pos, L, "pos", n, True), _ptr(npos, L, "npos", n, True),
+                _ptr(vel, L, "vel", n, True), _ptr(nvel, L, "nvel", n, True),
+                _ptr(iid, L, "iid", n, False))
+        h = self._handle()
+        _lib.check(self._lib.pbf_step_batched(h, *args, _stream_handle(stream)), h)
+
+    def scenes(self):
+        """Per-scene counts as the library reports them."""
+        k = ctypes.c_int(0)
+        h = self._handle()
+        _lib.check(self._lib.pbf_get_scenes(h, ctypes.byref(k), None), h)
+        counts = (ctypes.c_int * k.value)()
+        _lib.check(self._lib.pbf_get_scenes(h, ctypes.byref(k), counts), h)
+        return list(counts)

@@ -25,7 +25,7 @@ def test_library_exports_every_declared_symbol(pbf_mod):
     for name in declared:
         assert hasattr(lib, name), name
     assert set(declared) == set(_lib.SYMBOLS), "ctypes prototypes and header disagree"
-    assert lib.pbf_abi_version() == 1
+    assert lib.pbf_abi_version() == 2
 
 
 def test_status_strings(pbf_mod):
@@ -129,3 +129,86 @@ def test_sweep_box_matches_fluidsystem_formula(pbf_mod):
         assert u[0] == pytest.approx(16.0 + 2.0 * phi, abs=2e-6)
         assert u[1] == pytest.approx(6.4, abs=1e-6) and l[1] == pytest.approx(-0.5 * phi, abs=1e-6)
     assert sw.box(35)[0][0] == pytest.approx(18.0, abs=1e-5)  # quarter period: peak
+
+
+def test_batched_create_rejects_bad_arguments(pbf_mod):
+    from pbf import _lib
+
+    lib = _lib.load()
+    p = pbf_mod.dam_params().to_c()
+    V = _lib.PbfVec3
+    u = (V * 2)(V(1, 1, 1), V(2, 1, 1))
+    l = (V * 2)(V(0, 0, 0), V(0, 0, 0))
+    h = ctypes.c_void_p()
+
+    def create(k, uu, ll, counts, layout=4, max_cells=0):
+        c = (ctypes.c_int * len(counts))(*counts) if counts is not None else None
+        return lib.pbf_create_batched(ctypes.byref(p), k, uu, ll, c, max_cells, layout,
+                                      ctypes.byref(h))
+
+    bad = _lib.PBF_ERR_INVALID_ARGUMENT
+    assert create(0, u, l, [10, 20]) == bad
+    assert create(70000, u, l, [10, 20]) == bad
+    assert create(2, None, l, [10, 20]) == bad
+    assert create(2, u, l, None) == bad
+    assert create(2, u, l, [10, -1]) == bad and "negative" in lib.pbf_last_error(None).decode()
+    assert create(2, u, l, [0, 0]) == bad
+    assert create(2, u, l, [10, 20], layout=2) == bad
+    ub = (V * 2)(V(1, 1, 1), V(0, 1, 1))
+    assert create(2, ub, l, [10, 20]) == bad
+    assert "ulim must exceed llim" in lib.pbf_last_error(None).decode()
+    assert create(2, u, l, [10, 20], max_cells=5) == _lib.PBF_ERR_CAPACITY
+    assert not h
+
+
+def test_batched_create_without_device_reports_no_device(pbf_mod):
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a HIP device is visible")
+    with pytest.raises(pbf_mod.PBFError) as e:
+        pbf_mod.BatchedSimulator(pbf_mod.dam_params(), [(1, 1, 1), (2, 1, 1)],
+                                 [(0, 0, 0), (0, 0, 0)], [10, 20])
+    assert e.value.code == 4
+    with pytest.raises(ValueError):
+        pbf_mod.BatchedSimulator(pbf_mod.dam_params(), [(1, 1, 1)], [(0, 0, 0), (0, 0, 0)],
+                                 [10, 20])
+
+
+def test_new_entry_points_reject_null_solver(pbf_mod):
+    from pbf import _lib
+
+    lib = _lib.load()
+    bad = _lib.PBF_ERR_INVALID_ARGUMENT
+    assert lib.pbf_step_batched(None, None, None, None, None, None, None) == bad
+    assert lib.pbf_set_graph(None, 1) == bad
+    assert lib.pbf_set_shapes(None, None) == bad
+    assert lib.pbf_get_shapes(None, None) == bad
+    k = ctypes.c_int(0)
+    assert lib.pbf_get_scenes(None, ctypes.byref(k), None) == bad
+    assert lib.pbf_autotune(None, None, None, None, 10, 3, None, None, None) == bad
+
+
+def test_batch_scene_generator(pbf_mod):
+    """The 64-scene workload: counts log-uniform in [4096, 262144], cubic
+    boxes filled to 0.2-0.5, every particle inside its box, reproducible."""
+    import numpy as np
+
+    from pbf import scenes
+
+    scs = scenes.batch_scenes()
+    assert len(scs) == 64
+    ns = np.array([s.n for s in scs])
+    assert ns.min() >= 4096 and ns.max() <= 262144
+    assert 3.5e6 < ns.sum() < 4.5e6  # E = 64 * 62,047
+    for s in scs:
+        side = s.ulim[0]
+        assert s.ulim == (side, side, side) and s.llim == (0.0, 0.0, 0.0)
+        cells = round(side / 0.1) ** 3
+        assert 0.15 < s.n / (8 * cells) <= 0.5  # lattice sites per cell = 8 at spacing h/2
+        assert (s.pos >= 0).all() and (s.pos <= side).all()
+        np.testing.assert_array_equal(s.iid, np.arange(s.n))
+    again = scenes.batch_scenes()
+    assert all(np.array_equal(a.pos, b.pos) for a, b in zip(scs, again))
+    pos, vel, iid, counts, ul, ll, offs = scenes.concat(scs, 4)
+    assert pos.shape == (ns.sum(), 4) and list(offs[1:]) == list(np.cumsum(ns)[:-1])

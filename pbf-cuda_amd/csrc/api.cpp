@@ -495,7 +495,7 @@ int create_impl(const pbf_params& params, int capacity, long long cells, long lo
                o_val1 = carve(4 * N), o_iid = carve(4 * N), o_hist = carve(4 * 256 * tiles),
                o_tot = carve(4 * 256), o_xp = carve(16 * N), o_x0s = carve(16 * N),
                o_xa = carve(16 * N), o_rho = carve(4 * N), o_recs = carve(16 * N),
-               o_items = carve(8 * ICAP), o_part = carve(4 * pbf::kPartialWords * ICAP),
+               o_items = carve(16 * ICAP), o_part = carve(4 * pbf::kPartialWords * ICAP),
                o_cnt = carve(4 * 8), o_de = carve(8 * N), o_dp = carve(16 * dblk),
                o_do = carve(16), o_cnt2 = carve(4 * (size_t)max_cells),
                o_cs = carve(4 * ((size_t)max_cells + 1)),
@@ -523,7 +523,7 @@ int create_impl(const pbf_params& params, int capacity, long long cells, long lo
   s->counters = (uint32_t*)(b + o_cnt);
   s->hl.counters = s->counters;
   s->hl.recs = (uint4*)(b + o_recs);
-  s->hl.items = (uint2*)(b + o_items);
+  s->hl.items = (uint4*)(b + o_items);
   s->hl.partials = (float*)(b + o_part);
   s->hl.item_cap = (uint32_t)ICAP;
   s->dens_err = (double*)(b + o_de);

@@ -258,9 +258,9 @@ __device__ __forceinline__ uint32_t wave_exclusive_sum(uint32_t v, uint32_t* tot
 // Work lists of the heavy targets of one neighbour pass.
 struct HeavyLists {
   uint32_t* counters;  // [2][4]: per pass parity {records, chunk items, -, -}
-  uint4* recs;         // per heavy target: (i, first item, chunks, 0)
-  uint2* items;        // per chunk: (i, chunk index); i = ~0u marks an unused slot
-  float* partials;     // kPartialWords floats per chunk item
+  uint4* recs;         // per heavy target: (i, first partial slot, chunks, 0)
+  uint4* items;        // per chunk: (i, chunk index, partial slot, 0); i = ~0u: unused
+  float* partials;     // kPartialWords floats per partial slot
   uint32_t item_cap;   // capacity of items / partials
 };
 constexpr int kPartialWords = 8;

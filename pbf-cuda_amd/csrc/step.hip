@@ -13,24 +13,15 @@
 //   cs   uint32 per cell + 1: first sorted slot of each cell (exclusive scan
 //        of cnt); a cell's particles are [cs[c], cs[c+1])
 //
-// Each neighbour pass (lambda, delta-p, XSPH, density) is three kernels:
-//   light  one work-item per target over the cell-sorted order; workgroups
-//          remapped so each XCD's L2 serves one contiguous particle range; a
-//          target with more than kLightMax candidates is appended to the
-//          heavy list instead;
-//   chunks one work-item per kChunk-candidate chunk of the heavy targets;
-//   combine one work-item per heavy target, adding its chunk sums in order.
-// Both follow the canonical summation order of pbf_device.h.
+// The neighbour passes (lambda, delta-p, XSPH, density) are pass policies
+// run by the generic kernels of pbf_pass.h.
 #include "pbf_device.h"
 #include "pbf_launch.h"
+#include "pbf_pass.h"
 
 namespace pbf {
 
 namespace {
-
-constexpr int kBlock = 256;
-
-inline int blocks_for(long long n) { return (int)((n + kBlock - 1) / kBlock); }
 
 // ---------------------------------------------------------------------------
 // Pass policies: what a target loads, what a candidate contributes, how the
@@ -45,7 +36,8 @@ struct LambdaPass {
   struct Args {
     const float4* __restrict__ x;
     float4* __restrict__ xl;
-    float* __restrict__ rho;  // written only when non-null (last iteration)
+    float* __restrict__ rho;  // written only when non-null (last iteration, or a tape)
+    float4* __restrict__ sd;  // tape only (else null): (sum of gradients, denominator)
   };
   struct Target {
     float4 x;
@@ -105,6 +97,7 @@ struct LambdaPass {
     const float lam = -(rho / c.pho0 - 1.0f) / den;
     a.xl[t.i] = make_float4(t.x.x, t.x.y, t.x.z, lam);
     if (a.rho) a.rho[t.i] = rho;
+    if (a.sd) a.sd[t.i] = make_float4(s.gx, s.gy, s.gz, den);
   }
 };
 
@@ -117,6 +110,7 @@ struct DeltaPass {
   struct Args {
     const float4* __restrict__ xl;
     float4* __restrict__ xa;
+    float4* __restrict__ dp;  // tape only (else null): the correction / pho0 before clamps
   };
   struct Target {
     float4 x;  // w = lambda_i
@@ -161,9 +155,11 @@ struct DeltaPass {
   }
   static __device__ __forceinline__ void finish(const Args& a, const Target& t, const Acc& s,
                                                 const StepConsts& c, const SceneGeo& g) {
-    const float px = fminf(fmaxf(s.px / c.pho0, -c.max_dp), c.max_dp);
-    const float py = fminf(fmaxf(s.py / c.pho0, -c.max_dp), c.max_dp);
-    const float pz = fminf(fmaxf(s.pz / c.pho0, -c.max_dp), c.max_dp);
+    const float ux = s.px / c.pho0, uy = s.py / c.pho0, uz = s.pz / c.pho0;
+    if (a.dp) a.dp[t.i] = make_float4(ux, uy, uz, 0.0f);
+    const float px = fminf(fmaxf(ux, -c.max_dp), c.max_dp);
+    const float py = fminf(fmaxf(uy, -c.max_dp), c.max_dp);
+    const float pz = fminf(fmaxf(uz, -c.max_dp), c.max_dp);
     const float x = fmaxf(fminf(t.x.x + px, g.hix), g.lox);
     const float y = fmaxf(fminf(t.x.y + py, g.hiy), g.loy);
     const float z = fmaxf(fminf(t.x.z + pz, g.hiz), g.loz);
@@ -275,160 +271,6 @@ struct DensityPass {
     a.err[t.i] = fabs((double)rho / (double)c.pho0 - 1.0);
   }
 };
-
-// ---------------------------------------------------------------------------
-// Generic neighbour-pass kernels over a pass policy.
-// ---------------------------------------------------------------------------
-
-// Light: one work-item per target; the workgroup size is a launch shape
-// (64, 128 or 256), the column tables live in dynamic LDS.  A target with
-// more than kLightMax candidates is heavy: it gets one chunk item per kChunk
-// candidates and a record, appended with one atomic per wavefront each.
-// counters[pass & 1] belong to this pass; the kernel clears the other
-// parity's counters for the next pass (their previous users finished earlier
-// on the stream).  If the chunk list is full the lane sums its target itself,
-// still chunk by chunk, so the canonical order holds either way.
-template <class P, class S>
-__global__ __launch_bounds__(kBlock) void k_light(typename P::Args a, int n, StepConsts c, S sc,
-                                                  const uint32_t* __restrict__ cs, HeavyLists hl,
-                                                  int pass) {
-  extern __shared__ uint2 seq_dyn[];
-  uint32_t* cnt = hl.counters + 4 * (pass & 1);
-  if (blockIdx.x == 0 && threadIdx.x < 4) hl.counters[4 * ((pass + 1) & 1) + threadIdx.x] = 0u;
-  const int tid = threadIdx.x;
-  const int nb = (int)blockDim.x;
-  const int i = xcd_block_index() * nb + tid;
-  const bool active = i < n;
-  const SeqTable st{&seq_dyn[tid], nb};
-  typename P::Target t{};
-  uint32_t m = 0;
-  if (active) {
-    t = P::target(a, (uint32_t)i, c);
-    const SceneGeo g = sc.at((uint32_t)i, c);
-    m = build_seq(st, cell_of(t.x.x, t.x.y, t.x.z, g, c.h), cs, g);
-  }
-  const bool is_heavy = active && m > kLightMax;
-  const uint32_t nch = is_heavy ? (m + kChunk - 1) / kChunk : 0u;
-  if (__ballot(is_heavy) != 0ull) {
-    // wave-aggregated allocation of chunk items and records
-    uint32_t item_total;
-    const uint32_t item_off = wave_exclusive_sum(nch, &item_total);
-    uint32_t item_base = 0, rec_base = 0;
-    if ((threadIdx.x & 63) == 0) item_base = atomicAdd(&cnt[1], item_total);
-    item_base = __shfl(item_base, 0);
-    const uint32_t first = item_base + item_off;
-    const bool fits = is_heavy && first + nch <= hl.item_cap;
-    if (is_heavy) {
-      const uint32_t stop = first + nch < hl.item_cap ? first + nch : hl.item_cap;
-      for (uint32_t u = first; u < stop; ++u)
-        hl.items[u] = fits ? make_uint2((uint32_t)i, u - first) : make_uint2(~0u, 0u);
-    }
-    const uint64_t fm = __ballot(fits);
-    if (fm != 0ull) {
-      if ((threadIdx.x & 63) == 0) rec_base = atomicAdd(&cnt[0], (uint32_t)__popcll(fm));
-      rec_base = __shfl(rec_base, 0);
-      if (fits)
-        hl.recs[rec_base + (uint32_t)__popcll(fm & ((1ull << (threadIdx.x & 63)) - 1ull))] =
-            make_uint4((uint32_t)i, first, nch, 0u);
-    }
-    if (fits) return;
-  }
-  if (!active) return;
-  auto ls = [&](uint32_t j) { return P::load_s(a, j); };
-  auto lf = [&](uint32_t j) { return P::load_f(a, j); };
-  auto hit = [&](uint32_t j, const auto& d) { return P::hit(t, j, d, c); };
-  typename P::Acc s;
-  if (!is_heavy) {
-    sweep_seq(st, 0u, m, ls, lf, hit, [&](uint32_t j, const auto& d) { P::add(s, t, j, d, c); });
-  } else {
-    // chunk list full: this lane sums its chunks in order
-    for (uint32_t ch = 0; ch < nch; ++ch) {
-      typename P::Acc part;
-      const uint32_t s0 = ch * kChunk;
-      const uint32_t s1 = s0 + kChunk < m ? s0 + kChunk : m;
-      sweep_seq(st, s0, s1, ls, lf, hit,
-                [&](uint32_t j, const auto& d) { P::add(part, t, j, d, c); });
-      P::accumulate(s, part);
-    }
-  }
-  P::finish(a, t, s, c, sc.at(t.i, c));
-}
-
-// One work-item per chunk item: the partial sum of kChunk consecutive
-// candidates of a heavy target.  Grid-stride over the items; the grid size
-// is a launch shape.
-template <class P, class S>
-__global__ __launch_bounds__(kBlock) void k_chunks(typename P::Args a, StepConsts c, S sc,
-                                                   const uint32_t* __restrict__ cs, HeavyLists hl,
-                                                   int pass) {
-  __shared__ uint2 seq[10][kBlock];
-  const uint32_t* cnt = hl.counters + 4 * (pass & 1);
-  const uint32_t total = cnt[1] < hl.item_cap ? cnt[1] : hl.item_cap;
-  const uint32_t stride = gridDim.x * kBlock;
-  const SeqTable st{&seq[0][threadIdx.x], kBlock};
-  for (uint32_t u = (uint32_t)xcd_block_index() * kBlock + threadIdx.x; u < total; u += stride) {
-    const uint2 it = hl.items[u];
-    if (it.x == ~0u) continue;
-    const typename P::Target t = P::target(a, it.x, c);
-    const SceneGeo g = sc.at(it.x, c);
-    const uint32_t m = build_seq(st, cell_of(t.x.x, t.x.y, t.x.z, g, c.h), cs, g);
-    typename P::Acc part;
-    const uint32_t s0 = it.y * kChunk;
-    const uint32_t s1 = s0 + kChunk < m ? s0 + kChunk : m;
-    sweep_seq(
-        st, s0, s1, [&](uint32_t j) { return P::load_s(a, j); },
-        [&](uint32_t j) { return P::load_f(a, j); },
-        [&](uint32_t j, const auto& d) { return P::hit(t, j, d, c); },
-        [&](uint32_t j, const auto& d) { P::add(part, t, j, d, c); });
-    static_assert(sizeof(part) <= kPartialWords * sizeof(float), "partial too large");
-    *reinterpret_cast<typename P::Acc*>(hl.partials + (size_t)u * kPartialWords) = part;
-  }
-}
-
-// One work-item per heavy target: chunk partials added in chunk order.
-template <class P, class S>
-__global__ __launch_bounds__(kBlock) void k_combine(typename P::Args a, StepConsts c, S sc,
-                                                    HeavyLists hl, int pass) {
-  const uint32_t* cnt = hl.counters + 4 * (pass & 1);
-  const uint32_t nrec = cnt[0];
-  const uint32_t stride = gridDim.x * kBlock;
-  for (uint32_t r = blockIdx.x * kBlock + threadIdx.x; r < nrec; r += stride) {
-    const uint4 rec = hl.recs[r];
-    const typename P::Target t = P::target(a, rec.x, c);
-    typename P::Acc s;
-    for (uint32_t k = 0; k < rec.z; ++k)
-      P::accumulate(
-          s, *reinterpret_cast<const typename P::Acc*>(hl.partials +
-                                                       (size_t)(rec.y + k) * kPartialWords));
-    P::finish(a, t, s, c, sc.at(rec.x, c));
-  }
-}
-
-int grid_for_items(long long items, int max_blocks) {
-  const long long b = (items + kBlock - 1) / kBlock;
-  return (int)(b < 1 ? 1 : (b > max_blocks ? max_blocks : b));
-}
-
-template <class P, class S>
-void launch_pass_s(const typename P::Args& a, int n, const StepConsts& c, const S& sc,
-                   const uint32_t* cs, const HeavyLists& hl, int pass, const Shapes& sh,
-                   hipStream_t st) {
-  const int lb = sh.light_block;
-  k_light<P, S><<<(n + lb - 1) / lb, lb, 10 * lb * sizeof(uint2), st>>>(a, n, c, sc, cs, hl, pass);
-  k_chunks<P, S><<<grid_for_items(hl.item_cap, sh.chunk_blocks), kBlock, 0, st>>>(a, c, sc, cs,
-                                                                                    hl, pass);
-  k_combine<P, S><<<grid_for_items(n, 512), kBlock, 0, st>>>(a, c, sc, hl, pass);
-}
-
-template <class P>
-void launch_pass(const typename P::Args& a, int n, const StepConsts& c, const Scenes& scn,
-                 const uint32_t* cs, const HeavyLists& hl, int pass, const Shapes& sh,
-                 hipStream_t st) {
-  if (scn.geo)
-    launch_pass_s<P>(a, n, c, SceneTable{scn.geo, scn.sid}, cs, hl, pass, sh, st);
-  else
-    launch_pass_s<P>(a, n, c, OneScene{}, cs, hl, pass, sh, st);
-}
 
 // ---------------------------------------------------------------------------
 // Streaming kernels.
@@ -803,15 +645,15 @@ void launch_permute_pos(const uint32_t* sval, const float4* xp, int n, float4* x
 }
 
 void launch_lambda(const float4* xa, const uint32_t* cs, int n, const StepConsts& c,
-                   const Scenes& scn, float4* xl, float* rho, const HeavyLists& hl, int pass,
-                   const Shapes& sh, hipStream_t st) {
-  launch_pass<LambdaPass>(LambdaPass::Args{xa, xl, rho}, n, c, scn, cs, hl, pass, sh, st);
+                   const Scenes& scn, float4* xl, float* rho, float4* sd, const HeavyLists& hl,
+                   int pass, const Shapes& sh, hipStream_t st) {
+  launch_pass<LambdaPass>(LambdaPass::Args{xa, xl, rho, sd}, n, c, scn, cs, hl, pass, sh, st);
 }
 
 void launch_delta(const float4* xl, const uint32_t* cs, int n, const StepConsts& c,
-                  const Scenes& scn, float4* xa, const HeavyLists& hl, int pass, const Shapes& sh,
-                  hipStream_t st) {
-  launch_pass<DeltaPass>(DeltaPass::Args{xl, xa}, n, c, scn, cs, hl, pass, sh, st);
+                  const Scenes& scn, float4* xa, float4* dp, const HeavyLists& hl, int pass,
+                  const Shapes& sh, hipStream_t st) {
+  launch_pass<DeltaPass>(DeltaPass::Args{xl, xa, dp}, n, c, scn, cs, hl, pass, sh, st);
 }
 
 void launch_velocity(int layout, const float4* xa, const float4* x0s, const uint32_t* iid_s,

@@ -43,6 +43,7 @@ def parse_args():
     ap.add_argument("--no-graph", action="store_true", help="time eager steps only")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batched rate")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the backward/forward ratio")
     return ap.parse_args()
 
 
@@ -147,6 +148,79 @@ def timed(run, steps, stream):
     return e0.elapsed_time(e1) / steps
 
 
+def adjoint_ratio(pbf, p, dev, stream, reps=3):
+    """Backward / forward time of the 30-frame differentiable rollout of the
+    16,384-particle scene (dam constants, 4 iterations): forward = the tape's
+    checkpointed 30 frames, backward = dL/d(pos0, vel0, pho0) through the
+    30 frames with each frame's iterations recomputed from its checkpoint."""
+    import numpy as np
+    import torch
+
+    from pbf import scenes
+
+    sc = scenes.differentiable()
+    n, nf = sc.n, 30
+    tgt = torch.from_numpy(scenes.target_cloud(sc)).to(dev)
+    sim = pbf.Simulator(p, sc.ulim, sc.llim, capacity=n)
+    tape = pbf.Tape(sim, n, nf)
+    pos, vel = sc.as_layout(4)
+    pos0 = torch.from_numpy(pos.ravel()).to(dev)
+    vel0 = torch.from_numpy(vel.ravel()).to(dev)
+    iid0 = torch.from_numpy(sc.iid.astype(np.int32)).to(dev)
+    xf = torch.empty(n * 4, device=dev)
+    iidf = torch.empty(n, dtype=torch.int32, device=dev)
+    gpos = torch.zeros(n, 4, device=dev)
+    gp0 = torch.empty(n * 4, device=dev)
+    gv0 = torch.empty(n * 4, device=dev)
+
+    def forward():
+        tape.forward(pos0, vel0, iid0, stream)
+
+    def backward():
+        tape.read(nf, pos=xf, iid=iidf, stream=stream)
+        x = xf.view(n, 4)[:, :3].double()
+        gpos[:, :3] = (2.0 * (x - tgt[iidf.long()]) / n).float()
+        return tape.backward(gpos.view(-1), None, gp0, gv0, stream)
+
+    def plain():
+        a = [pos0.clone(), vel0.clone()]
+        b = [torch.empty_like(pos0), torch.empty_like(vel0)]
+        ids = iid0.clone()
+        for _ in range(nf):
+            sim.step(a[0], b[0], a[1], b[1], ids, n, stream)
+            a, b = b, a
+
+    def time_it(fn):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        ts = []
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            e0.record(stream)
+            fn()
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1))
+        return sorted(ts)[reps // 2]
+
+    with torch.cuda.stream(stream):
+        forward()
+        grho = backward()
+        t_f = time_it(forward)
+        t_b = time_it(backward)
+        t_p = time_it(plain)
+    gv = gv0.view(n, 4)[:, :3]
+    return {"value": t_b / t_f, "unit": "backward time / forward time",
+            "forward_ms": t_f, "backward_ms": t_b, "plain_forward_ms": t_p,
+            "backward_over_plain_forward": t_b / t_p,
+            "gradients_finite": bool(torch.isfinite(gv).all().item()) and bool(np.isfinite(grho)),
+            "workload": "differentiable_16k: 16,384 particles (32x16x32 jittered lattice, "
+                        "v0 ~ N(0, 0.5^2)), box 40h x 20h x 40h, 30 frames, 4 iterations, dam "
+                        "constants; loss = mean squared distance to a seeded target cloud; "
+                        "gradients w.r.t. pos0, vel0, pho0; per-frame checkpoints, iterations "
+                        "recomputed"}
+
+
 def main():
     args = parse_args()
     if args.gpus != 1:
@@ -231,6 +305,12 @@ def main():
         print(f"batched: {ms_b:.4f} ms/step, {rate:.4g} /s", file=sys.stderr, flush=True)
         del br
 
+    adjoint = None
+    if not args.no_adjoint:
+        adjoint = adjoint_ratio(pbf, p, dev, stream)
+        print(f"adjoint: forward {adjoint['forward_ms']:.2f} ms, backward "
+              f"{adjoint['backward_ms']:.2f} ms", file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
         cur = eager.current()
@@ -266,7 +346,7 @@ def main():
         "cpu_baseline": cpu,
         "phase_ms": phase_ms,
         "batched_rate": batched,
-        "backward_forward_ratio": None,
+        "backward_forward_ratio": adjoint,
     }
     print(json.dumps(out), flush=True)
 

@@ -250,6 +250,45 @@ int pbf_get_shapes(const pbf_solver* s, pbf_shapes* out);
 int pbf_autotune(pbf_solver* s, const float* pos, const float* vel, const uint32_t* iid, int n,
                  int reps, void* stream, pbf_shapes* best, double* best_ms);
 
+/*
+ * Differentiation (no counterpart in the reference).  A tape runs `nframes`
+ * frames of a single-scene solver, keeping each frame's input state (x, v,
+ * iid) as a checkpoint; the backward pass walks the frames last to first,
+ * recomputing each frame's solver iterations from its checkpoint and
+ * running the adjoint kernels of that frame.  Gradients are of the frame
+ * map with its discrete choices held fixed (sort order, grid, candidate
+ * lists, kernel cutoffs, active clamps).
+ *
+ * pbf_tape_create: tape for n particles and nframes frames under the
+ *   solver's current niter (device memory ~ (36 nframes + 52 niter + 200) B
+ *   per particle).  Allocates and synchronises.
+ * pbf_tape_forward: runs the frames from (pos, vel, iid) (caller layout,
+ *   not modified); the final state is pbf_tape_state(t, nframes).  With a
+ *   sweep, frame k runs in the box pbf_sweep_box(sweep, first_frame + k)
+ *   (as pbf_run_frames; the solver keeps the last box); the backward pass
+ *   differentiates each frame in its own box.
+ * pbf_tape_state: device pointers of checkpoint `frame` (0 = the inputs,
+ *   nframes = the final state; caller layout, sorted as the frames left it).
+ * pbf_tape_read: copies checkpoint `frame` into caller arrays (any may be
+ *   NULL), stream-ordered.
+ * pbf_tape_backward: gpos, gvel = dL/d(final positions, final velocities)
+ *   in the final state's order (either may be NULL = zero); writes gpos0,
+ *   gvel0 = dL/d(initial positions, velocities) in the order of pos/vel as
+ *   given to pbf_tape_forward, and *grho0 = dL/dpho0 (host; may be NULL).
+ *   Synchronises.  The parameters must be those of the forward run.
+ */
+typedef struct pbf_tape pbf_tape;
+int pbf_tape_create(pbf_solver* s, int n, int nframes, pbf_tape** out);
+void pbf_tape_destroy(pbf_tape* t);
+int pbf_tape_forward(pbf_solver* s, pbf_tape* t, const float* pos, const float* vel,
+                     const uint32_t* iid, int first_frame, const pbf_sweep* sweep,
+                     void* stream);
+int pbf_tape_state(const pbf_tape* t, int frame, float** pos, float** vel, uint32_t** iid);
+int pbf_tape_read(const pbf_tape* t, int frame, float* pos, float* vel, uint32_t* iid,
+                  void* stream);
+int pbf_tape_backward(pbf_solver* s, pbf_tape* t, const float* gpos, const float* gvel,
+                      float* gpos0, float* gvel0, double* grho0, void* stream);
+
 /* Message of the last failure on `s` (or of the last pbf_create failure on
  * this thread when s is NULL).  Never NULL. */
 const char* pbf_last_error(const pbf_solver* s);

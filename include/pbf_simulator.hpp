@@ -62,10 +62,70 @@ class Simulator {
     check(pbf_set_lim(s_, ulim, llim), s_);
   }
 
+  // Additions without a reference counterpart: HIP graph replay, launch
+  // shapes and the autotuner (pbf.h).
+  void setGraph(bool enable) { check(pbf_set_graph(s_, enable ? 1 : 0), s_); }
+  void setShapes(int light_block, int chunk_blocks) {
+    const pbf_shapes sh{light_block, chunk_blocks};
+    check(pbf_set_shapes(s_, &sh), s_);
+  }
+  pbf_shapes autotune(const float* pos, const float* vel, const uint32_t* iid, int n, int reps = 3,
+                      void* stream = nullptr, double* best_ms = nullptr) {
+    pbf_shapes best;
+    check(pbf_autotune(s_, pos, vel, iid, n, reps, stream, &best, best_ms), s_);
+    return best;
+  }
+
   pbf_solver* handle() { return s_; }
 
- private:
+ protected:
+  Simulator() = default;
   pbf_solver* s_ = nullptr;
+};
+
+// Many boxes stepped together (pbf_create_batched): scene k owns indices
+// [off_k, off_k + counts[k]) of every array.
+class BatchedSimulator : public Simulator {
+ public:
+  BatchedSimulator(const pbf_params& params, int nscenes, const pbf_vec3* ulim,
+                   const pbf_vec3* llim, const int* counts, long long max_cells = 0,
+                   int layout = PBF_LAYOUT_FLOAT4) {
+    check(pbf_create_batched(&params, nscenes, ulim, llim, counts, max_cells, layout, &s_),
+          nullptr);
+  }
+  void stepBatched(float* pos, float* npos, float* vel, float* nvel, uint32_t* iid,
+                   void* stream = nullptr) {
+    check(pbf_step_batched(s_, pos, npos, vel, nvel, iid, stream), s_);
+  }
+};
+
+// Checkpointed multi-frame run of a single-scene Simulator and its adjoint.
+class Tape {
+ public:
+  Tape(Simulator& sim, int n, int nframes) : sim_(sim) {
+    check(pbf_tape_create(sim.handle(), n, nframes, &t_), sim.handle());
+  }
+  ~Tape() { pbf_tape_destroy(t_); }
+  Tape(const Tape&) = delete;
+  Tape& operator=(const Tape&) = delete;
+  void forward(const float* pos, const float* vel, const uint32_t* iid, void* stream = nullptr,
+               int first_frame = 0, const pbf_sweep* sweep = nullptr) {
+    check(pbf_tape_forward(sim_.handle(), t_, pos, vel, iid, first_frame, sweep, stream),
+          sim_.handle());
+  }
+  // Returns dL/dpho0; gpos0/gvel0 receive dL/d(initial positions, velocities).
+  double backward(const float* gpos, const float* gvel, float* gpos0, float* gvel0,
+                  void* stream = nullptr) {
+    double g = 0.0;
+    check(pbf_tape_backward(sim_.handle(), t_, gpos, gvel, gpos0, gvel0, &g, stream),
+          sim_.handle());
+    return g;
+  }
+  pbf_tape* handle() { return t_; }
+
+ private:
+  Simulator& sim_;
+  pbf_tape* t_ = nullptr;
 };
 
 }  // namespace pbf

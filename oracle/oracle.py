@@ -81,6 +81,11 @@ def load():
                                           ctypes.c_int, _F, ctypes.c_int, ctypes.c_int,
                                           ctypes.c_double * 2, _D]
     lib.orc_density_error_f32.restype = ctypes.c_int
+    lib.orc_rollout.argtypes = [ctypes.POINTER(OrcParams), ctypes.POINTER(OrcBox),
+                                ctypes.POINTER(OrcBox), ctypes.c_int, _D, _D, _U, ctypes.c_int,
+                                ctypes.c_double, ctypes.c_int, ctypes.c_int, _D, _D, _D, _D, _D,
+                                _D]
+    lib.orc_rollout.restype = ctypes.c_int
     lib.orc_max_threads.argtypes = []
     lib.orc_max_threads.restype = ctypes.c_int
     _lib = lib
@@ -218,3 +223,43 @@ def density_error_f32(params, ulim, llim, x: np.ndarray, threads: int = 0):
 
 def max_threads() -> int:
     return load().orc_max_threads()
+
+
+def rollout(params, ulim, llim, pos0: np.ndarray, vel0: np.ndarray, iid0: np.ndarray,
+            nframes: int, target: Optional[np.ndarray] = None, *, pho0: float = 0.0,
+            fp32: bool = False, threads: int = 0, grad: bool = True, boxes=None):
+    """nframes frames from (pos0, vel0, iid0) (N x 3, input order); the state
+    stays in FP64 (fp32=False) or follows the FP32 mirror (fp32=True).
+    boxes: optional per-frame [(ulim, llim), ...] (a sweeping wall).
+    Returns dict(x_final (N x 3 by identity), loss, g_pos0, g_vel0 (N x 3,
+    input order), g_rho0): the loss (1/N) sum |x_final - target|^2 and its
+    adjoint gradients (FP64), when target is given and grad is True."""
+    lib = load()
+    n = int(pos0.shape[0])
+    P = np.ascontiguousarray(np.asarray(pos0, np.float64)[:, :3])
+    V = np.ascontiguousarray(np.asarray(vel0, np.float64)[:, :3])
+    I = np.ascontiguousarray(iid0, np.uint32)
+    xf = np.zeros((n, 3))
+    out = {"x_final": xf}
+    tgt = None if target is None else np.ascontiguousarray(target, np.float64)
+    loss = ctypes.c_double(0.0)
+    gp = gv = None
+    gr = ctypes.c_double(0.0)
+    if tgt is not None and grad:
+        gp, gv = np.zeros((n, 3)), np.zeros((n, 3))
+    bx = None
+    if boxes is not None:
+        if len(boxes) != int(nframes):
+            raise ValueError("boxes needs one (ulim, llim) per frame")
+        bx = (OrcBox * len(boxes))(*[box_c(u, l) for u, l in boxes])
+    rc = lib.orc_rollout(ctypes.byref(params_c(params)), ctypes.byref(box_c(ulim, llim)), bx, n,
+                         _dp(P), _dp(V), _up(I), int(nframes), float(pho0), 1 if fp32 else 0,
+                         int(threads), _dp(tgt), ctypes.byref(loss) if tgt is not None else None,
+                         _dp(xf), _dp(gp), _dp(gv), ctypes.byref(gr) if gp is not None else None)
+    if rc != 0:
+        raise ValueError("orc_rollout rejected its input")
+    if tgt is not None:
+        out["loss"] = loss.value
+    if gp is not None:
+        out.update(g_pos0=gp, g_vel0=gv, g_rho0=gr.value)
+    return out

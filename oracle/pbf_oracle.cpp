@@ -17,6 +17,7 @@
 #include "pbf_oracle.h"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <numeric>
@@ -537,6 +538,572 @@ int density_impl(const orc_params* p, const orc_box* box, int n, const float* xi
   return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Rollout over several frames and its adjoint (reverse-mode derivative).
+//
+// The step is differentiated with its discrete choices held fixed: the sort
+// order, the stale grid, every target's candidate list, which pairs fall
+// inside h / KERNAL_EPS, and which clamps are active (a clamped component
+// passes no derivative).  The adjoint is written in scatter form: for every
+// (target i, candidate j) pair of a pass, the pair term's derivative with
+// respect to x_i is added to i and its derivative with respect to x_j to j,
+// so it assumes nothing about the symmetry of candidate lists (the stale
+// grid makes them asymmetric).  All derivative arithmetic is FP64; the
+// forward state is T (double: the FP64 rollout; float: the GPU's FP32
+// trajectory, operation for operation).
+// ---------------------------------------------------------------------------
+
+template <class T>
+struct FrameTape {
+  std::vector<uint32_t> perm;  // sorted slot -> input index
+  Grid g;
+  std::vector<T> X0;               // sorted start positions (3n)
+  std::vector<std::vector<T>> x;   // x[k]: positions entering iteration k, x[niter] final (3n)
+  std::vector<std::vector<T>> lam, rho, S, D, dp;  // per iteration: n, n, 3n, n, 3n
+  std::vector<T> V;                // velocities before XSPH (3n)
+};
+
+template <class T>
+void consts_with_pho0(Consts<T>& c, const orc_params* p, double pho0) {
+  const double h = (double)p->h;
+  const double spiky = -45.0 / (kPi * std::pow(h, 6));
+  c.pho0 = (T)pho0;
+  c.spiky_rho0 = (T)(spiky / pho0);
+}
+
+// One frame from (X, V, iid) in input order; outputs in sorted order.
+template <class T>
+void frame_forward(const orc_params* p, const orc_box* box, const Consts<T>& c, const int dims[3],
+                   int TH, const std::vector<T>& X, const std::vector<T>& Vin,
+                   const std::vector<uint32_t>& iid, FrameTape<T>& tp, std::vector<T>& Xout,
+                   std::vector<T>& Vout, std::vector<uint32_t>& iid_out) {
+  const int n = (int)iid.size();
+  const T gz = (T)(-p->g);
+  // advect_kernel (fluids/Simulator_kernel.cuh:14-15) as fused multiply-adds;
+  // the keys come from the predicted positions rounded to FP32
+  std::vector<T> xp((size_t)3 * n);
+  std::vector<float> xs((size_t)3 * n);
+  for (int i = 0; i < n; ++i)
+    for (int a = 0; a < 3; ++a) {
+      const T v1 = fma_(c.dt, a == 2 ? gz : (T)0, Vin[(size_t)3 * i + a]);
+      const T y = fma_(c.dt, v1, X[(size_t)3 * i + a]);
+      xp[(size_t)3 * i + a] = y;
+      xs[(size_t)3 * i + a] = (float)y;
+    }
+  std::vector<uint32_t> keys(n), skeys(n);
+  tp.perm.resize(n);
+  orc_cell_keys(xs.data(), 3, n, box, p->h, dims, keys.data());
+  orc_stable_order(keys.data(), n, tp.perm.data());
+  for (int s = 0; s < n; ++s) skeys[s] = keys[tp.perm[s]];
+  build_grid(skeys, n, dims, tp.g);
+  const int NI = p->niter;
+  tp.X0.assign((size_t)3 * n, (T)0);
+  tp.x.assign(NI + 1, std::vector<T>((size_t)3 * n));
+  tp.lam.assign(NI, std::vector<T>(n));
+  tp.rho.assign(NI, std::vector<T>(n));
+  tp.D.assign(NI, std::vector<T>(n));
+  tp.S.assign(NI, std::vector<T>((size_t)3 * n));
+  tp.dp.assign(NI, std::vector<T>((size_t)3 * n));
+  iid_out.resize(n);
+  for (int s = 0; s < n; ++s) {
+    const uint32_t src = tp.perm[s];
+    for (int a = 0; a < 3; ++a) {
+      tp.X0[(size_t)3 * s + a] = X[(size_t)3 * src + a];
+      tp.x[0][(size_t)3 * s + a] = xp[(size_t)3 * src + a];
+    }
+    iid_out[s] = iid[src];
+  }
+  auto collect = [&](const T* xi, std::vector<uint32_t>& cand) {
+    int q[3];
+    query_cell(xi, box, p->h, dims, false, q);
+    cand.clear();
+    for_each_candidate(tp.g, q, [&](uint32_t j) { cand.push_back(j); });
+  };
+  for (int k = 0; k < NI; ++k) {
+    const std::vector<T>& x = tp.x[k];
+    std::vector<T>& lam = tp.lam[k];
+#pragma omp parallel num_threads(TH)
+    {
+      std::vector<uint32_t> cand;
+#pragma omp for schedule(dynamic, 256)
+      for (int i = 0; i < n; ++i) {
+        const T* xi = &x[(size_t)3 * i];
+        collect(xi, cand);
+        Acc<T, 5> a = canonical_sum<T, 5>(cand, [&](Acc<T, 5>& sacc, uint32_t j) {
+          T d[3];
+          const T r2 = dist2(xi, &x[(size_t)3 * j], d);
+          if (r2 < c.h2_outer) {
+            if (r2 < c.h2) {
+              const T t = c.h2 - r2;
+              sacc.v[0] = sacc.v[0] + ((c.poly6 * t) * t) * t;
+            }
+            const T rl = std::sqrt(r2);
+            if (rl >= c.keps && rl < c.h) {
+              const T t = c.h - rl;
+              const T f = ((c.spiky_rho0 * t) * t) / rl;
+              const T ax = f * d[0], ay = f * d[1], az = f * d[2];
+              sacc.v[1] = sacc.v[1] + ax;
+              sacc.v[2] = sacc.v[2] + ay;
+              sacc.v[3] = sacc.v[3] + az;
+              if (j != (uint32_t)i) {
+                sacc.v[4] = fma_(ax, ax, sacc.v[4]);
+                sacc.v[4] = fma_(ay, ay, sacc.v[4]);
+                sacc.v[4] = fma_(az, az, sacc.v[4]);
+              }
+            }
+          }
+        });
+        T r = a.v[0];
+        if (c.k_boundary != (T)0) r = r + c.k_boundary * boundary_density(xi, c);
+        T den = fma_(a.v[1], a.v[1], a.v[4]);
+        den = fma_(a.v[2], a.v[2], den);
+        den = fma_(a.v[3], a.v[3], den);
+        den = den + c.lambda_eps;
+        lam[i] = -(r / c.pho0 - (T)1) / den;
+        tp.rho[k][i] = r;
+        tp.D[k][i] = den;
+        for (int q = 0; q < 3; ++q) tp.S[k][(size_t)3 * i + q] = a.v[1 + q];
+      }
+    }
+    std::vector<T>& xn = tp.x[k + 1];
+#pragma omp parallel num_threads(TH)
+    {
+      std::vector<uint32_t> cand;
+#pragma omp for schedule(dynamic, 256)
+      for (int i = 0; i < n; ++i) {
+        const T* xi = &x[(size_t)3 * i];
+        collect(xi, cand);
+        Acc<T, 3> a = canonical_sum<T, 3>(cand, [&](Acc<T, 3>& acc, uint32_t j) {
+          if (j == (uint32_t)i) return;
+          T d[3];
+          const T r2 = dist2(xi, &x[(size_t)3 * j], d);
+          if (r2 < c.h2_outer) {
+            const T rl = std::sqrt(r2);
+            if (rl >= c.keps && rl < c.h) {
+              T w = 0;
+              if (r2 < c.h2) {
+                const T t2 = c.h2 - r2;
+                w = ((c.poly6 * t2) * t2) * t2;
+              }
+              const T corr = c.coef_corr * pow_corr(w, c);
+              const T t = c.h - rl;
+              const T gs = ((c.spiky * t) * t) / rl;
+              const T sc = ((lam[i] + lam[j]) + corr) * gs;
+              acc.v[0] = fma_(sc, d[0], acc.v[0]);
+              acc.v[1] = fma_(sc, d[1], acc.v[1]);
+              acc.v[2] = fma_(sc, d[2], acc.v[2]);
+            }
+          }
+        });
+        for (int q = 0; q < 3; ++q) {
+          const T v = a.v[q] / c.pho0;
+          tp.dp[k][(size_t)3 * i + q] = v;
+          T y = xi[q] + std::fmin(std::fmax(v, -c.max_dp), c.max_dp);
+          y = std::fmax(std::fmin(y, c.hi[q]), c.lo[q]);
+          xn[(size_t)3 * i + q] = y;
+        }
+      }
+    }
+  }
+  const std::vector<T>& xf = tp.x[NI];
+  tp.V.resize((size_t)3 * n);
+  for (size_t q = 0; q < (size_t)3 * n; ++q) tp.V[q] = (xf[q] - tp.X0[q]) * c.inv_dt;
+  const std::vector<T>& rho = tp.rho[NI - 1];
+  Vout.resize((size_t)3 * n);
+#pragma omp parallel num_threads(TH)
+  {
+    std::vector<uint32_t> cand;
+#pragma omp for schedule(dynamic, 256)
+    for (int i = 0; i < n; ++i) {
+      const T* xi = &xf[(size_t)3 * i];
+      const T* vi = &tp.V[(size_t)3 * i];
+      collect(xi, cand);
+      Acc<T, 3> a = canonical_sum<T, 3>(cand, [&](Acc<T, 3>& acc, uint32_t j) {
+        T d[3];
+        const T r2 = dist2(xi, &xf[(size_t)3 * j], d);
+        if (r2 < c.h2) {
+          const T t = c.h2 - r2;
+          const T w = ((c.poly6 * t) * t) * t;
+          const T* vj = &tp.V[(size_t)3 * j];
+          const T f = ((T)2 * w) / (rho[i] + rho[j]);
+          acc.v[0] = fma_(vj[0] - vi[0], f, acc.v[0]);
+          acc.v[1] = fma_(vj[1] - vi[1], f, acc.v[1]);
+          acc.v[2] = fma_(vj[2] - vi[2], f, acc.v[2]);
+        }
+      });
+      for (int q = 0; q < 3; ++q) Vout[(size_t)3 * i + q] = fma_(c.c_xsph, a.v[q], vi[q]);
+    }
+  }
+  Xout = xf;
+}
+
+// Per-thread accumulation buffers of a scatter loop, summed in thread order
+// (deterministic for a given thread count).
+struct Scatter {
+  int th, len;
+  std::vector<std::vector<double>> buf;
+  Scatter(int threads, int length) : th(threads), len(length), buf(threads) {
+    for (auto& b : buf) b.assign((size_t)length, 0.0);
+  }
+  double* mine() {
+#ifdef _OPENMP
+    return buf[(size_t)omp_get_thread_num()].data();
+#else
+    return buf[0].data();
+#endif
+  }
+  void add_into(std::vector<double>& out) const {
+    for (int t = 0; t < th; ++t)
+      for (int q = 0; q < len; ++q) out[(size_t)q] += buf[(size_t)t][(size_t)q];
+  }
+};
+
+// Pair geometry of x_i - x_j in FP64, with the forward's branch decisions
+// (taken in T exactly as the forward took them).
+template <class T>
+struct Pair {
+  bool in_h2 = false;   // r2 < h2: poly6 terms
+  bool spiky = false;   // r2 < h2_outer && keps <= r < h: gradient terms
+  double d[3], r2 = 0, r = 0;
+};
+
+template <class T>
+inline Pair<T> pair_of(const T* xi, const T* xj, const Consts<T>& c) {
+  Pair<T> pr;
+  T dT[3];
+  const T r2 = dist2(xi, xj, dT);
+  pr.in_h2 = r2 < c.h2;
+  if (r2 < c.h2_outer) {
+    const T rl = std::sqrt(r2);
+    pr.spiky = rl >= c.keps && rl < c.h;
+  }
+  for (int a = 0; a < 3; ++a) pr.d[a] = (double)xi[a] - (double)xj[a];
+  pr.r2 = pr.d[0] * pr.d[0] + pr.d[1] * pr.d[1] + pr.d[2] * pr.d[2];
+  pr.r = std::sqrt(pr.r2);
+  return pr;
+}
+
+// Derivative of the wall term DensityBoundary (fluids/Simulator.cu:139-161)
+// at distance d from a wall: d/dd of (2pi/3)(h-d)^2(h+d) inside (0, h].
+inline double wall_slope(double d, double h, double wall_c) {
+  if (d > h || d <= 0.0) return 0.0;
+  return -wall_c * (h - d) * (h + 3.0 * d);
+}
+
+// Backward pass of one frame: (gX, gV) = adjoints of the frame's outputs
+// (x_final, v after XSPH) in sorted order; returns the adjoints of the
+// frame's inputs (positions, velocities) in input order and adds dL/dpho0.
+template <class T>
+void frame_backward(const orc_params* p, const orc_box* box, const Consts<T>& c,
+                    const int dims[3], int TH, const FrameTape<T>& tp,
+                    const std::vector<double>& gX, const std::vector<double>& gV,
+                    std::vector<double>& gXin, std::vector<double>& gVin, double& g_rho0) {
+  const int n = (int)tp.perm.size();
+  const int NI = p->niter;
+  const double h = (double)c.h, h2 = h * h, poly6 = (double)c.poly6, spiky = (double)c.spiky;
+  const double spiky_rho0 = (double)c.spiky_rho0, pho0 = (double)c.pho0;
+  const double coef = (double)c.coef_corr, ncorr = (double)c.n_corr, eps = (double)c.lambda_eps;
+  const double kb = (double)c.k_boundary, cx = (double)c.c_xsph, inv_dt = (double)c.inv_dt;
+  const double dt = (double)c.dt, maxdp = (double)c.max_dp;
+  auto collect = [&](const T* xi, std::vector<uint32_t>& cand) {
+    int q[3];
+    query_cell(xi, box, p->h, dims, false, q);
+    cand.clear();
+    for_each_candidate(tp.g, q, [&](uint32_t j) { cand.push_back(j); });
+  };
+  auto poly6_w = [&](double r2) {
+    const double t = h2 - r2;
+    return poly6 * t * t * t;
+  };
+  // dW/dx_i = -6 poly6 (h2 - r2)^2 d
+  auto poly6_grad = [&](const double* d, double r2, double out[3]) {
+    const double t = h2 - r2;
+    const double f = -6.0 * poly6 * t * t;
+    for (int a = 0; a < 3; ++a) out[a] = f * d[a];
+  };
+  // (coef, grad) of the spiky kernel: G = k (h-r)^2 d / r and Hess * A
+  auto spiky_hess = [&](double k, const double* d, double r, const double* A, double out[3]) {
+    const double phi = k * (h - r) * (h - r) / r;
+    const double psi = -k * (h - r) * (h + r) / (r * r * r);
+    const double dA = d[0] * A[0] + d[1] * A[1] + d[2] * A[2];
+    for (int a = 0; a < 3; ++a) out[a] = phi * A[a] + psi * dA * d[a];
+  };
+  // d s_corr / d w for s = coef * w^n
+  auto dscorr_dw = [&](double w) {
+    if (c.n_corr_int == 0 || ncorr == 0.0) return 0.0;
+    if (c.n_corr_int > 0) {
+      double r = 1.0;
+      for (int e = 1; e < c.n_corr_int; ++e) r *= w;
+      return coef * ncorr * r;
+    }
+    return w > 0.0 ? coef * ncorr * std::pow(w, ncorr - 1.0) : 0.0;
+  };
+  auto scorr = [&](double w) {
+    if (c.n_corr_int >= 0) {
+      double r = 1.0;
+      for (int e = 0; e < c.n_corr_int; ++e) r *= w;
+      return coef * r;
+    }
+    return coef * std::pow(w, ncorr);
+  };
+
+  std::vector<double> gx(gX);           // adjoint of the current positions
+  std::vector<double> gXs((size_t)3 * n, 0.0);  // adjoint of the sorted start positions
+  std::vector<double> grho_x(n, 0.0);   // adjoint of rho of the last iteration, from XSPH
+  // ---- XSPH (fluids/Simulator_kernel.cuh:196-239) and v = (x - x0)/dt
+  {
+    const std::vector<T>& xf = tp.x[NI];
+    const std::vector<T>& rho = tp.rho[NI - 1];
+    std::vector<double> gVp(gV);  // adjoint of V (identity path)
+    Scatter sv(TH, 3 * n), sx(TH, 3 * n), sr(TH, n);
+#pragma omp parallel num_threads(TH)
+    {
+      std::vector<uint32_t> cand;
+      double* bv = sv.mine();
+      double* bx = sx.mine();
+      double* br = sr.mine();
+#pragma omp for schedule(dynamic, 256)
+      for (int i = 0; i < n; ++i) {
+        const T* xi = &xf[(size_t)3 * i];
+        collect(xi, cand);
+        const double* B = &gV[(size_t)3 * i];
+        for (uint32_t j : cand) {
+          const Pair<T> pr = pair_of(xi, &xf[(size_t)3 * j], c);
+          if (!pr.in_h2) continue;
+          const double W = poly6_w(pr.r2);
+          const double den = (double)rho[i] + (double)rho[j];
+          const double f = 2.0 * cx * W / den;
+          double u = 0.0;
+          for (int a = 0; a < 3; ++a) {
+            bv[3 * j + a] += f * B[a];
+            bv[3 * i + a] -= f * B[a];
+            u += B[a] * ((double)tp.V[3 * (size_t)j + a] - (double)tp.V[3 * (size_t)i + a]);
+          }
+          double P[3];
+          poly6_grad(pr.d, pr.r2, P);
+          const double fx = 2.0 * cx * u / den;
+          for (int a = 0; a < 3; ++a) {
+            bx[3 * i + a] += fx * P[a];
+            bx[3 * j + a] -= fx * P[a];
+          }
+          const double fr = 2.0 * cx * W * u / (den * den);
+          br[i] -= fr;
+          br[j] -= fr;
+        }
+      }
+    }
+    sv.add_into(gVp);
+    sx.add_into(gx);
+    sr.add_into(grho_x);
+    for (size_t q = 0; q < (size_t)3 * n; ++q) {
+      gx[q] += gVp[q] * inv_dt;
+      gXs[q] -= gVp[q] * inv_dt;
+    }
+  }
+  // ---- solver iterations, last to first
+  for (int k = NI - 1; k >= 0; --k) {
+    const std::vector<T>& x = tp.x[k];
+    const std::vector<T>& lam = tp.lam[k];
+    std::vector<double> gxn((size_t)3 * n, 0.0), A((size_t)3 * n, 0.0), glam(n, 0.0);
+    // position update and clamps (fluids/Simulator_kernel.cuh:184-192)
+    for (int i = 0; i < n; ++i)
+      for (int a = 0; a < 3; ++a) {
+        const size_t q = (size_t)3 * i + a;
+        const T v = tp.dp[k][q];
+        const T y = x[q] + std::fmin(std::fmax(v, -c.max_dp), c.max_dp);
+        const double e = (y > c.lo[a] && y < c.hi[a]) ? gx[q] : 0.0;
+        gxn[q] = e;
+        const double dv = (double)v;
+        const double db = (dv > -maxdp && dv < maxdp) ? e : 0.0;
+        g_rho0 += db * (-dv / pho0);
+        A[q] = db / pho0;
+      }
+    // delta-p pair terms (fluids/Simulator_kernel.cuh:153-170)
+    {
+      Scatter sx(TH, 3 * n), sl(TH, n);
+#pragma omp parallel num_threads(TH)
+      {
+        std::vector<uint32_t> cand;
+        double* bx = sx.mine();
+        double* bl = sl.mine();
+#pragma omp for schedule(dynamic, 256)
+        for (int i = 0; i < n; ++i) {
+          const T* xi = &x[(size_t)3 * i];
+          collect(xi, cand);
+          const double* Ai = &A[(size_t)3 * i];
+          for (uint32_t j : cand) {
+            if (j == (uint32_t)i) continue;
+            const Pair<T> pr = pair_of(xi, &x[(size_t)3 * j], c);
+            if (!pr.spiky) continue;
+            const double w = pr.in_h2 ? poly6_w(pr.r2) : 0.0;
+            const double qc = (double)lam[i] + (double)lam[j] + scorr(w);
+            const double phi = spiky * (h - pr.r) * (h - pr.r) / pr.r;
+            double AG = 0.0;
+            for (int a = 0; a < 3; ++a) AG += Ai[a] * phi * pr.d[a];
+            double HA[3];
+            spiky_hess(spiky, pr.d, pr.r, Ai, HA);
+            double ds[3] = {0.0, 0.0, 0.0};
+            if (pr.in_h2) {
+              double P[3];
+              poly6_grad(pr.d, pr.r2, P);
+              const double k1 = dscorr_dw(w);
+              for (int a = 0; a < 3; ++a) ds[a] = k1 * P[a];
+            }
+            bl[i] += AG;
+            bl[j] += AG;
+            for (int a = 0; a < 3; ++a) {
+              const double t = qc * HA[a] + AG * ds[a];
+              bx[3 * i + a] += t;
+              bx[3 * j + a] -= t;
+            }
+          }
+        }
+      }
+      sx.add_into(gxn);
+      sl.add_into(glam);
+    }
+    // lambda = -(rho/pho0 - 1) / D (fluids/Simulator_kernel.cuh:119-123)
+    std::vector<double> grho(n), gD(n);
+    for (int i = 0; i < n; ++i) {
+      const double D = (double)tp.D[k][i], rho = (double)tp.rho[k][i];
+      const double C = rho / pho0 - 1.0;
+      const double gC = -glam[i] / D;
+      gD[i] = glam[i] * C / (D * D);
+      grho[i] = gC / pho0 + (k == NI - 1 ? grho_x[i] : 0.0);
+      g_rho0 += gC * (-rho / (pho0 * pho0)) + gD[i] * (-2.0 * (D - eps) / pho0);
+      if (kb != 0.0) {
+        const T* xi = &x[(size_t)3 * i];
+        for (int a = 0; a < 3; ++a) {
+          const double s = -wall_slope((double)c.ul[a] - (double)xi[a], h, (double)c.wall_c) +
+                           wall_slope((double)xi[a] - (double)c.ll[a], h, (double)c.wall_c);
+          gxn[(size_t)3 * i + a] += grho[i] * kb * s;
+        }
+      }
+    }
+    // density and gradient-norm pair terms (fluids/Simulator_kernel.cuh:78-104)
+    {
+      Scatter sx(TH, 3 * n);
+#pragma omp parallel num_threads(TH)
+      {
+        std::vector<uint32_t> cand;
+        double* bx = sx.mine();
+#pragma omp for schedule(dynamic, 256)
+        for (int i = 0; i < n; ++i) {
+          const T* xi = &x[(size_t)3 * i];
+          collect(xi, cand);
+          double Si[3];
+          for (int a = 0; a < 3; ++a) Si[a] = (double)tp.S[k][(size_t)3 * i + a];
+          for (uint32_t j : cand) {
+            const Pair<T> pr = pair_of(xi, &x[(size_t)3 * j], c);
+            if (pr.in_h2 && grho[i] != 0.0) {
+              double P[3];
+              poly6_grad(pr.d, pr.r2, P);
+              for (int a = 0; a < 3; ++a) {
+                bx[3 * i + a] += grho[i] * P[a];
+                bx[3 * j + a] -= grho[i] * P[a];
+              }
+            }
+            if (pr.spiky && j != (uint32_t)i) {
+              const double phig = spiky_rho0 * (h - pr.r) * (h - pr.r) / pr.r;
+              double Vv[3];
+              for (int a = 0; a < 3; ++a) Vv[a] = phig * pr.d[a] + Si[a];
+              double HV[3];
+              spiky_hess(spiky_rho0, pr.d, pr.r, Vv, HV);
+              for (int a = 0; a < 3; ++a) {
+                const double t = 2.0 * gD[i] * HV[a];
+                bx[3 * i + a] += t;
+                bx[3 * j + a] -= t;
+              }
+            }
+          }
+        }
+      }
+      sx.add_into(gxn);
+    }
+    gx.swap(gxn);
+  }
+  // ---- predict: x[0] = X + dt (V + dt g), un-permuted to input order
+  gXin.assign((size_t)3 * n, 0.0);
+  gVin.assign((size_t)3 * n, 0.0);
+  for (int s = 0; s < n; ++s) {
+    const uint32_t src = tp.perm[s];
+    for (int a = 0; a < 3; ++a) {
+      gXin[(size_t)3 * src + a] = gXs[(size_t)3 * s + a] + gx[(size_t)3 * s + a];
+      gVin[(size_t)3 * src + a] = dt * gx[(size_t)3 * s + a];
+    }
+  }
+}
+
+template <class T>
+int rollout_impl(const orc_params* p, const orc_box* box, const orc_box* boxes, int n,
+                 const double* pos0, const double* vel0, const uint32_t* iid0, int nframes,
+                 double pho0, int threads, const double* target, double* loss, double* x_final,
+                 double* g_pos0, double* g_vel0, double* g_rho0) {
+  if (!p || !box || n < 1 || nframes < 0 || !pos0 || !vel0 || !iid0 || !finite_params(p))
+    return -1;
+  const bool want_grad = g_pos0 || g_vel0 || g_rho0;
+  if ((want_grad || loss) && !target) return -1;
+  for (int s = 0; s < n; ++s)
+    if (iid0[s] >= (uint32_t)n) return -1;
+  const int TH = nthreads(threads);
+  // per frame: its box, grid dims and constants
+  std::vector<orc_box> bx((size_t)nframes, *box);
+  if (boxes)
+    for (int f = 0; f < nframes; ++f) bx[(size_t)f] = boxes[f];
+  std::vector<std::array<int, 3>> dims((size_t)nframes);
+  std::vector<Consts<T>> cs;
+  for (int f = 0; f < nframes; ++f) {
+    if (orc_grid_dims(&bx[(size_t)f], p->h, dims[(size_t)f].data()) != 0) return -1;
+    cs.push_back(make_consts<T>(p, &bx[(size_t)f]));
+    if (pho0 > 0.0) consts_with_pho0(cs.back(), p, pho0);
+  }
+  std::vector<T> X((size_t)3 * n), V((size_t)3 * n);
+  for (size_t q = 0; q < (size_t)3 * n; ++q) {
+    X[q] = (T)pos0[q];
+    V[q] = (T)vel0[q];
+  }
+  std::vector<uint32_t> iid(iid0, iid0 + n);
+  std::vector<FrameTape<T>> tapes(want_grad ? nframes : 1);
+  std::vector<T> Xo, Vo;
+  std::vector<uint32_t> io;
+  for (int f = 0; f < nframes; ++f) {
+    FrameTape<T>& tp = tapes[want_grad ? f : 0];
+    frame_forward(p, &bx[(size_t)f], cs[(size_t)f], dims[(size_t)f].data(), TH, X, V, iid, tp, Xo,
+                  Vo, io);
+    X.swap(Xo);
+    V.swap(Vo);
+    iid.swap(io);
+  }
+  std::vector<double> gX((size_t)3 * n, 0.0), gV((size_t)3 * n, 0.0);
+  double L = 0.0;
+  for (int s = 0; s < n; ++s)
+    for (int a = 0; a < 3; ++a) {
+      const double xv = (double)X[(size_t)3 * s + a];
+      if (x_final) x_final[(size_t)3 * iid[s] + a] = xv;
+      if (target) {
+        const double dlt = xv - target[(size_t)3 * iid[s] + a];
+        L += dlt * dlt;
+        gX[(size_t)3 * s + a] = 2.0 * dlt / n;
+      }
+    }
+  if (loss) *loss = L / n;
+  if (!want_grad) return 0;
+  double gr = 0.0;
+  std::vector<double> gXin, gVin;
+  for (int f = nframes - 1; f >= 0; --f) {
+    frame_backward(p, &bx[(size_t)f], cs[(size_t)f], dims[(size_t)f].data(), TH, tapes[f], gX, gV,
+                   gXin, gVin, gr);
+    gX.swap(gXin);
+    gV.swap(gVin);
+  }
+  for (size_t q = 0; q < (size_t)3 * n; ++q) {
+    if (g_pos0) g_pos0[q] = gX[q];
+    if (g_vel0) g_vel0[q] = gV[q];
+  }
+  if (g_rho0) *g_rho0 = gr;
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -643,5 +1210,15 @@ int orc_density_error_f32(const orc_params* p, const orc_box* box, int n, const 
 }
 
 int orc_max_threads(void) { return nthreads(0); }
+
+int orc_rollout(const orc_params* p, const orc_box* box, const orc_box* boxes, int n,
+                const double* pos0, const double* vel0, const uint32_t* iid0, int nframes,
+                double pho0, int fp32, int threads, const double* target, double* loss,
+                double* x_final, double* g_pos0, double* g_vel0, double* g_rho0) {
+  return fp32 ? rollout_impl<float>(p, box, boxes, n, pos0, vel0, iid0, nframes, pho0, threads,
+                                    target, loss, x_final, g_pos0, g_vel0, g_rho0)
+              : rollout_impl<double>(p, box, boxes, n, pos0, vel0, iid0, nframes, pho0, threads,
+                                     target, loss, x_final, g_pos0, g_vel0, g_rho0);
+}
 
 }  // extern "C"

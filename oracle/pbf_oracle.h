@@ -102,6 +102,25 @@ int orc_density_error(const orc_params* p, const orc_box* box, int n, const floa
 int orc_density_error_f32(const orc_params* p, const orc_box* box, int n, const float* x,
                           int stride, int threads, double out[2], double* err_out);
 
+/*
+ * Rollout of `nframes` frames from (pos0, vel0, iid0) (input order, 3n
+ * doubles each) with the state kept in T between frames (fp32 = 0: FP64, the
+ * reference of the rollout; fp32 = 1: the FP32 mirror, the GPU's trajectory
+ * operation for operation).  boxes (nframes entries, may be NULL = `box`
+ * for every frame) gives each frame's box.  pho0 > 0 overrides p->pho0 (for finite
+ * differences in the rest density).  Loss L = (1/n) sum_i |x_final(iid i) -
+ * target(iid i)|^2 with target (3n, indexed by identity).  x_final (3n, by
+ * identity), loss and the gradients may be NULL; the gradients dL/dpos0,
+ * dL/dvel0 (input order) and dL/dpho0 come from the adjoint of every frame,
+ * evaluated in FP64 along the rollout's own trajectory, with its discrete
+ * choices (order, candidate lists, cutoffs, clamps) held fixed.
+ * Returns 0 or -1.
+ */
+int orc_rollout(const orc_params* p, const orc_box* box, const orc_box* boxes, int n,
+                const double* pos0, const double* vel0, const uint32_t* iid0, int nframes,
+                double pho0, int fp32, int threads, const double* target, double* loss,
+                double* x_final, double* g_pos0, double* g_vel0, double* g_rho0);
+
 /* OpenMP threads the oracle uses by default. */
 int orc_max_threads(void);
 

@@ -1,0 +1,565 @@
+// adjoint.hip — reverse-mode derivative of one PBF frame on gfx950.
+//
+// The frame is differentiated with its discrete choices held fixed (sort
+// order, stale grid, candidate lists, kernel cutoffs, active clamps), along
+// the trajectory the forward kernels computed, which a tape records: per
+// solver iteration k the positions and lambdas xl[k], the lambda pass's
+// (sum of gradients S, denominator D) and densities, and the correction
+// before its clamps.  Derivatives follow the reference's formulas
+// (fluids/Simulator_kernel.cuh:52-239): lambda_i = -(rho_i/pho0 - 1)/D_i with
+// D_i = sum_{j!=i} |g_ij|^2 + |S_i|^2 + eps, g_ij = spikyGrad(x_i-x_j)/pho0;
+// dp_i = sum_{j!=i} (lambda_i + lambda_j + s_corr) spikyGrad / pho0, clamped;
+// XSPH v'_i = v_i + c sum_j 2 (v_j - v_i) W_ij / (rho_i + rho_j).
+//
+// Every backward neighbour pass is a pass policy of pbf_pass.h over target
+// i's candidate list cand(i).  A pair term of target i depends on x_i and
+// x_j; target i gathers d(term)/dx_i of its own terms, and, for each j in
+// cand(i) with i in cand(j) (i's sorted cell next to j's query cell), the
+// derivative of j's term about i with respect to x_i.  The stale grid makes
+// candidate lists asymmetric: when i is not in cand(j), j never visits i, so
+// i scatters d(its term)/dx_j to j with an atomic add into an "extra" array
+// that a later kernel folds in.  Gathered sums follow the canonical order of
+// the forward passes; only the (rare) scattered terms arrive in a
+// hardware-dependent order.
+#include "pbf_adjoint.h"
+#include "pbf_pass.h"
+
+namespace pbf {
+
+namespace {
+
+// Sorted (stale-grid) cell of a particle from its sorted key.
+__device__ __forceinline__ Cell cell_of_key(uint32_t key, const SceneGeo& g) {
+  const uint32_t local = key - g.base;
+  const uint32_t yz = (uint32_t)(g.dy * g.dz);
+  const uint32_t x = local / yz;
+  const uint32_t r = local - x * yz;
+  const uint32_t y = r / (uint32_t)g.dz;
+  return Cell{(int)x, (int)y, (int)(r - y * (uint32_t)g.dz)};
+}
+
+// Whether a particle whose sorted cell is ci is a candidate of the particle
+// at position xj (ci in the 3x3x3 block around xj's query cell).
+__device__ __forceinline__ bool is_candidate_of(const Cell& ci, const float4& xj,
+                                                const StepConsts& c) {
+  const Cell q = cell_of(xj.x, xj.y, xj.z, c.g, c.h);
+  return abs(ci.x - q.x) <= 1 && abs(ci.y - q.y) <= 1 && abs(ci.z - q.z) <= 1;
+}
+
+// dW/dx_i of poly6 at d = x_i - x_j (r2 < h2): -6 poly6 (h2 - r2)^2 d.
+__device__ __forceinline__ float3 poly6_grad(const float3& d, float r2, const StepConsts& c) {
+  const float t = c.h2 - r2;
+  const float f = -6.0f * c.poly6 * t * t;
+  return make_float3(f * d.x, f * d.y, f * d.z);
+}
+
+// Hessian of k (h - r)^2 d / r (the spiky gradient with coefficient k)
+// applied to a: phi a + psi (d.a) d.
+__device__ __forceinline__ float3 spiky_hess(float k, const float3& d, float r, const float3& a,
+                                             const StepConsts& c) {
+  const float u = c.h - r;
+  const float phi = k * u * u / r;
+  const float psi = -k * u * (c.h + r) / (r * r * r);
+  const float da = d.x * a.x + d.y * a.y + d.z * a.z;
+  return make_float3(phi * a.x + psi * da * d.x, phi * a.y + psi * da * d.y,
+                     phi * a.z + psi * da * d.z);
+}
+
+// d s_corr / d w for s_corr = coef_corr * w^n_corr.
+__device__ __forceinline__ float dscorr_dw(float w, const StepConsts& c) {
+  if (c.n_corr_int == 0 || c.n_corr == 0.0f) return 0.0f;
+  if (c.n_corr_int > 0) {
+    float r = 1.0f;
+    for (int e = 1; e < c.n_corr_int; ++e) r = r * w;
+    return c.coef_corr * c.n_corr * r;
+  }
+  return w > 0.0f ? c.coef_corr * c.n_corr * powf(w, c.n_corr - 1.0f) : 0.0f;
+}
+
+__device__ __forceinline__ float dot3(const float3& a, const float3& b) {
+  return a.x * b.x + a.y * b.y + a.z * b.z;
+}
+__device__ __forceinline__ float3 xyz(const float4& v) { return make_float3(v.x, v.y, v.z); }
+
+__device__ __forceinline__ void atomic_add3(float4* p, const float3& v) {
+  atomicAdd(&p->x, v.x);
+  atomicAdd(&p->y, v.y);
+  atomicAdd(&p->z, v.z);
+}
+
+// ---- XSPH -----------------------------------------------------------------
+struct BwdXsphPass {
+  struct Args {
+    const float4* __restrict__ x;    // final positions
+    const float4* __restrict__ vr;   // (v before XSPH, rho of the last lambda pass)
+    const float4* __restrict__ gv;   // adjoint of v after XSPH
+    const uint32_t* __restrict__ key;  // sorted keys
+    float4* __restrict__ out_x;      // gathered (dL/dx, dL/drho)
+    float4* __restrict__ out_v;      // gathered dL/dv (pair terms)
+    float4* __restrict__ ex_x;       // scattered (dL/dx, dL/drho)
+    float4* __restrict__ ex_v;       // scattered dL/dv
+  };
+  struct Target {
+    float4 x, vr, gv;
+    Cell cell;
+    uint32_t i;
+  };
+  struct Acc {
+    float vx = 0.0f, vy = 0.0f, vz = 0.0f, xx = 0.0f, xy = 0.0f, xz = 0.0f, r = 0.0f;
+  };
+  struct Nb {
+    float4 x, vr, gv;
+  };
+  static __device__ __forceinline__ Target target(const Args& a, uint32_t i, const StepConsts& c) {
+    return Target{a.x[i], a.vr[i], a.gv[i], cell_of_key(a.key[i], c.g), i};
+  }
+  static __device__ __forceinline__ float4 load_s(const Args& a, uint32_t j) { return a.x[j]; }
+  static __device__ __forceinline__ Nb load_f(const Args& a, uint32_t j) {
+    return Nb{a.x[j], a.vr[j], a.gv[j]};
+  }
+  static __device__ __forceinline__ bool hit(const Target& t, uint32_t, const float4& xj,
+                                             const StepConsts& c) {
+    float dx, dy, dz;
+    return dist2(t.x, xj, dx, dy, dz) < c.h2;
+  }
+  static __device__ __forceinline__ void add(Acc& s, const Target& t, uint32_t j, const Nb& nb,
+                                             const StepConsts& c, const Args& a) {
+    float dx, dy, dz;
+    const float r2 = dist2(t.x, nb.x, dx, dy, dz);
+    if (!(r2 < c.h2)) return;
+    const float3 d = make_float3(dx, dy, dz);
+    const float W = poly6_of(r2, c);
+    const float3 P = poly6_grad(d, r2, c);
+    const float den = t.vr.w + nb.vr.w;
+    const float f = 2.0f * c.c_xsph * W / den;
+    const float3 dv = make_float3(nb.vr.x - t.vr.x, nb.vr.y - t.vr.y, nb.vr.z - t.vr.z);
+    // own term of v'_i
+    const float3 B = xyz(t.gv);
+    const float u = dot3(B, dv);
+    const float fx = 2.0f * c.c_xsph * u / den;
+    const float fr = 2.0f * c.c_xsph * W * u / (den * den);
+    s.vx -= f * B.x;
+    s.vy -= f * B.y;
+    s.vz -= f * B.z;
+    s.xx += fx * P.x;
+    s.xy += fx * P.y;
+    s.xz += fx * P.z;
+    s.r -= fr;
+    if (is_candidate_of(t.cell, nb.x, c)) {
+      // j's term about i, gathered here
+      const float3 Bj = xyz(nb.gv);
+      const float uj = -dot3(Bj, dv);
+      const float gx = 2.0f * c.c_xsph * uj / den;
+      s.vx += f * Bj.x;
+      s.vy += f * Bj.y;
+      s.vz += f * Bj.z;
+      s.xx += gx * P.x;
+      s.xy += gx * P.y;
+      s.xz += gx * P.z;
+      s.r -= 2.0f * c.c_xsph * W * uj / (den * den);
+    } else if (j != t.i) {
+      // j never visits i: deliver the own term's derivatives about j
+      atomic_add3(&a.ex_v[j], make_float3(f * B.x, f * B.y, f * B.z));
+      atomic_add3(&a.ex_x[j], make_float3(-fx * P.x, -fx * P.y, -fx * P.z));
+      atomicAdd(&a.ex_x[j].w, -fr);
+    }
+  }
+  static __device__ __forceinline__ void accumulate(Acc& tot, const Acc& p) {
+    tot.vx += p.vx;
+    tot.vy += p.vy;
+    tot.vz += p.vz;
+    tot.xx += p.xx;
+    tot.xy += p.xy;
+    tot.xz += p.xz;
+    tot.r += p.r;
+  }
+  static __device__ __forceinline__ void finish(const Args& a, const Target& t, const Acc& s,
+                                                const StepConsts&, const SceneGeo&) {
+    a.out_x[t.i] = make_float4(s.xx, s.xy, s.xz, s.r);
+    a.out_v[t.i] = make_float4(s.vx, s.vy, s.vz, 0.0f);
+  }
+};
+
+// ---- delta-p ----------------------------------------------------------------
+struct BwdDeltaPass {
+  struct Args {
+    const float4* __restrict__ xl;   // (x, lambda) of the iteration
+    const float4* __restrict__ A;    // adjoint of the correction sum (before / pho0)
+    const uint32_t* __restrict__ key;
+    float4* __restrict__ out;        // gathered (dL/dx, dL/dlambda)
+    float4* __restrict__ ex;         // scattered (dL/dx, dL/dlambda)
+  };
+  struct Target {
+    float4 x, A;
+    Cell cell;
+    uint32_t i;
+  };
+  struct Acc {
+    float x = 0.0f, y = 0.0f, z = 0.0f, l = 0.0f;
+  };
+  struct Nb {
+    float4 x, A;
+  };
+  static __device__ __forceinline__ Target target(const Args& a, uint32_t i, const StepConsts& c) {
+    return Target{a.xl[i], a.A[i], cell_of_key(a.key[i], c.g), i};
+  }
+  static __device__ __forceinline__ float4 load_s(const Args& a, uint32_t j) { return a.xl[j]; }
+  static __device__ __forceinline__ Nb load_f(const Args& a, uint32_t j) {
+    return Nb{a.xl[j], a.A[j]};
+  }
+  static __device__ __forceinline__ bool hit(const Target& t, uint32_t j, const float4& xj,
+                                             const StepConsts& c) {
+    float dx, dy, dz;
+    return j != t.i && dist2(t.x, xj, dx, dy, dz) < c.h2_outer;
+  }
+  static __device__ __forceinline__ void add(Acc& s, const Target& t, uint32_t j, const Nb& nb,
+                                             const StepConsts& c, const Args& a) {
+    if (j == t.i) return;
+    float dx, dy, dz;
+    const float r2 = dist2(t.x, nb.x, dx, dy, dz);
+    if (!(r2 < c.h2_outer)) return;
+    const float r = sqrtf(r2);
+    if (!(r >= c.keps && r < c.h)) return;
+    const float3 d = make_float3(dx, dy, dz);
+    const float w = r2 < c.h2 ? poly6_of(r2, c) : 0.0f;
+    const float q = (t.x.w + nb.x.w) + c.coef_corr * pow_corr(w, c);
+    const float u = c.h - r;
+    const float phi = c.spiky * u * u / r;
+    const float3 G = make_float3(phi * d.x, phi * d.y, phi * d.z);
+    float3 ds = make_float3(0.0f, 0.0f, 0.0f);
+    if (r2 < c.h2) {
+      const float k1 = dscorr_dw(w, c);
+      const float3 P = poly6_grad(d, r2, c);
+      ds = make_float3(k1 * P.x, k1 * P.y, k1 * P.z);
+    }
+    // own term: A_i . (q G(d))
+    const float3 Ai = xyz(t.A);
+    const float AG = dot3(Ai, G);
+    const float3 HA = spiky_hess(c.spiky, d, r, Ai, c);
+    const float3 gi = make_float3(q * HA.x + AG * ds.x, q * HA.y + AG * ds.y, q * HA.z + AG * ds.z);
+    s.x += gi.x;
+    s.y += gi.y;
+    s.z += gi.z;
+    s.l += AG;
+    if (is_candidate_of(t.cell, nb.x, c)) {
+      // j's term about i: A_j . (q G(x_j - x_i)) = -q A_j . G(d)
+      const float3 Aj = xyz(nb.A);
+      const float AGj = dot3(Aj, G);
+      const float3 HAj = spiky_hess(c.spiky, d, r, Aj, c);
+      s.x -= q * HAj.x + AGj * ds.x;
+      s.y -= q * HAj.y + AGj * ds.y;
+      s.z -= q * HAj.z + AGj * ds.z;
+      s.l -= AGj;
+    } else {
+      atomic_add3(&a.ex[j], make_float3(-gi.x, -gi.y, -gi.z));
+      atomicAdd(&a.ex[j].w, AG);
+    }
+  }
+  static __device__ __forceinline__ void accumulate(Acc& tot, const Acc& p) {
+    tot.x += p.x;
+    tot.y += p.y;
+    tot.z += p.z;
+    tot.l += p.l;
+  }
+  static __device__ __forceinline__ void finish(const Args& a, const Target& t, const Acc& s,
+                                                const StepConsts&, const SceneGeo&) {
+    a.out[t.i] = make_float4(s.x, s.y, s.z, s.l);
+  }
+};
+
+// ---- lambda (density and gradient norms) ------------------------------------
+struct BwdLambdaPass {
+  struct Args {
+    const float4* __restrict__ xr;   // (x, dL/drho)
+    const float4* __restrict__ sd;   // (S, dL/dD)
+    const uint32_t* __restrict__ key;
+    float4* __restrict__ gx;         // adjoint of x: += gathered
+    float4* __restrict__ ex;         // scattered dL/dx
+  };
+  struct Target {
+    float4 x, sd;
+    Cell cell;
+    uint32_t i;
+  };
+  struct Acc {
+    float x = 0.0f, y = 0.0f, z = 0.0f;
+  };
+  struct Nb {
+    float4 x, sd;
+  };
+  static __device__ __forceinline__ Target target(const Args& a, uint32_t i, const StepConsts& c) {
+    return Target{a.xr[i], a.sd[i], cell_of_key(a.key[i], c.g), i};
+  }
+  static __device__ __forceinline__ float4 load_s(const Args& a, uint32_t j) { return a.xr[j]; }
+  static __device__ __forceinline__ Nb load_f(const Args& a, uint32_t j) {
+    return Nb{a.xr[j], a.sd[j]};
+  }
+  static __device__ __forceinline__ bool hit(const Target& t, uint32_t, const float4& xj,
+                                             const StepConsts& c) {
+    float dx, dy, dz;
+    return dist2(t.x, xj, dx, dy, dz) < c.h2_outer;
+  }
+  static __device__ __forceinline__ void add(Acc& s, const Target& t, uint32_t j, const Nb& nb,
+                                             const StepConsts& c, const Args& a) {
+    float dx, dy, dz;
+    const float r2 = dist2(t.x, nb.x, dx, dy, dz);
+    if (!(r2 < c.h2_outer)) return;
+    const float3 d = make_float3(dx, dy, dz);
+    const bool recip = is_candidate_of(t.cell, nb.x, c);
+    float3 gi = make_float3(0.0f, 0.0f, 0.0f);  // own terms' d/dx_i
+    float3 gr = make_float3(0.0f, 0.0f, 0.0f);  // j's terms' d/dx_i
+    if (r2 < c.h2) {
+      const float3 P = poly6_grad(d, r2, c);
+      gi = make_float3(t.x.w * P.x, t.x.w * P.y, t.x.w * P.z);
+      gr = make_float3(nb.x.w * P.x, nb.x.w * P.y, nb.x.w * P.z);
+    }
+    const float r = sqrtf(r2);
+    if (r >= c.keps && r < c.h && j != t.i) {
+      const float u = c.h - r;
+      const float phig = c.spiky_rho0 * u * u / r;
+      const float3 g = make_float3(phig * d.x, phig * d.y, phig * d.z);
+      const float3 Vi = make_float3(g.x + t.sd.x, g.y + t.sd.y, g.z + t.sd.z);
+      const float3 Hi = spiky_hess(c.spiky_rho0, d, r, Vi, c);
+      const float ki = 2.0f * t.sd.w;
+      gi = make_float3(gi.x + ki * Hi.x, gi.y + ki * Hi.y, gi.z + ki * Hi.z);
+      const float3 Vj = make_float3(g.x - nb.sd.x, g.y - nb.sd.y, g.z - nb.sd.z);
+      const float3 Hj = spiky_hess(c.spiky_rho0, d, r, Vj, c);
+      const float kj = 2.0f * nb.sd.w;
+      gr = make_float3(gr.x + kj * Hj.x, gr.y + kj * Hj.y, gr.z + kj * Hj.z);
+    }
+    s.x += gi.x;
+    s.y += gi.y;
+    s.z += gi.z;
+    if (recip) {
+      s.x += gr.x;
+      s.y += gr.y;
+      s.z += gr.z;
+    } else if (j != t.i) {
+      atomic_add3(&a.ex[j], make_float3(-gi.x, -gi.y, -gi.z));
+    }
+  }
+  static __device__ __forceinline__ void accumulate(Acc& tot, const Acc& p) {
+    tot.x += p.x;
+    tot.y += p.y;
+    tot.z += p.z;
+  }
+  static __device__ __forceinline__ void finish(const Args& a, const Target& t, const Acc& s,
+                                                const StepConsts&, const SceneGeo&) {
+    const float4 g = a.gx[t.i];
+    a.gx[t.i] = make_float4(g.x + s.x, g.y + s.y, g.z + s.z, 0.0f);
+  }
+};
+
+// ---- pointwise kernels -------------------------------------------------------
+
+// After the XSPH pass: dL/dx_final, dL/dx_start (from v = (x - x0)/dt) and
+// dL/drho of the last lambda pass; clears the extras.
+__global__ __launch_bounds__(kBlock) void k_bwd_velocity(
+    int n, StepConsts c, const float4* __restrict__ gx_out, const float4* __restrict__ gv_out,
+    const float4* __restrict__ out_x, const float4* __restrict__ out_v, float4* __restrict__ ex_x,
+    float4* __restrict__ ex_v, float4* __restrict__ gx, float4* __restrict__ gx0,
+    float* __restrict__ grho) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const float4 ox = out_x[i], ov = out_v[i], ex = ex_x[i], ev = ex_v[i], go = gx_out[i],
+               gvo = gv_out[i];
+  const float vx = gvo.x + ov.x + ev.x, vy = gvo.y + ov.y + ev.y, vz = gvo.z + ov.z + ev.z;
+  gx[i] = make_float4(go.x + ox.x + ex.x + vx * c.inv_dt, go.y + ox.y + ex.y + vy * c.inv_dt,
+                      go.z + ox.z + ex.z + vz * c.inv_dt, 0.0f);
+  gx0[i] = make_float4(-vx * c.inv_dt, -vy * c.inv_dt, -vz * c.inv_dt, 0.0f);
+  grho[i] = ox.w + ex.w;
+  ex_x[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  ex_v[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+// Position update of iteration k (fluids/Simulator_kernel.cuh:184-192):
+// x' = clamp_box(x + clamp(dp, +-MAX_DP)).  gx = dL/dx' -> gxn = identity
+// part of dL/dx, A = dL/d(correction sum), r0 += dL/dpho0 through the / pho0.
+__global__ __launch_bounds__(kBlock) void k_bwd_clamp(int n, StepConsts c,
+                                                      const float4* __restrict__ gx,
+                                                      const float4* __restrict__ xl,
+                                                      const float4* __restrict__ dp,
+                                                      float4* __restrict__ gxn,
+                                                      float4* __restrict__ A,
+                                                      double* __restrict__ r0) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const float4 g = gx[i], x = xl[i], u = dp[i];
+  const float gv[3] = {g.x, g.y, g.z}, xv[3] = {x.x, x.y, x.z}, uv[3] = {u.x, u.y, u.z};
+  const float lo[3] = {c.g.lox, c.g.loy, c.g.loz}, hi[3] = {c.g.hix, c.g.hiy, c.g.hiz};
+  float e[3], a[3];
+  double rr = 0.0;
+  for (int q = 0; q < 3; ++q) {
+    const float y = xv[q] + fminf(fmaxf(uv[q], -c.max_dp), c.max_dp);
+    e[q] = (y > lo[q] && y < hi[q]) ? gv[q] : 0.0f;
+    const float db = (uv[q] > -c.max_dp && uv[q] < c.max_dp) ? e[q] : 0.0f;
+    rr += (double)db * (-(double)uv[q] / (double)c.pho0);
+    a[q] = db / c.pho0;
+  }
+  gxn[i] = make_float4(e[0], e[1], e[2], 0.0f);
+  A[i] = make_float4(a[0], a[1], a[2], 0.0f);
+  r0[i] += rr;
+}
+
+// DensityBoundary slope (fluids/Simulator.cu:139-161) at wall distance d.
+__device__ __forceinline__ float wall_slope(float d, const StepConsts& c) {
+  if (d > c.h || d <= 0.0f) return 0.0f;
+  return -c.wall_c * (c.h - d) * (c.h + 3.0f * d);
+}
+
+// lambda = -(rho/pho0 - 1)/D: from dL/dlambda (gathered + scattered) to
+// dL/drho, dL/dD; folds the delta-p pair terms into gxn; writes the lambda
+// pass's per-particle inputs; clears the extras.
+__global__ __launch_bounds__(kBlock) void k_bwd_lambda_pre(
+    int n, StepConsts c, int last, const float4* __restrict__ out, float4* __restrict__ ex,
+    const float4* __restrict__ xl, const float4* __restrict__ sd, const float* __restrict__ rho,
+    const float* __restrict__ grho_x, float4* __restrict__ gxn, float4* __restrict__ xr,
+    float4* __restrict__ sdb, double* __restrict__ r0) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const float4 o = out[i], e = ex[i], x = xl[i], s = sd[i];
+  const float gl = o.w + e.w;
+  const float D = s.w, rh = rho[i];
+  const float C = rh / c.pho0 - 1.0f;
+  const float gC = -gl / D;
+  const float gD = gl * C / (D * D);
+  float gr = gC / c.pho0;
+  if (last) gr += grho_x[i];
+  r0[i] += (double)gC * (-(double)rh / ((double)c.pho0 * (double)c.pho0)) +
+           (double)gD * (-2.0 * ((double)D - (double)c.lambda_eps) / (double)c.pho0);
+  float4 g = gxn[i];
+  g.x += o.x + e.x;
+  g.y += o.y + e.y;
+  g.z += o.z + e.z;
+  if (c.k_boundary != 0.0f) {
+    const float k = gr * c.k_boundary;
+    g.x += k * (wall_slope(x.x - c.g.llx, c) - wall_slope(c.g.ulx - x.x, c));
+    g.y += k * (wall_slope(x.y - c.g.lly, c) - wall_slope(c.g.uly - x.y, c));
+    g.z += k * (wall_slope(x.z - c.g.llz, c) - wall_slope(c.g.ulz - x.z, c));
+  }
+  gxn[i] = g;
+  xr[i] = make_float4(x.x, x.y, x.z, gr);
+  sdb[i] = make_float4(s.x, s.y, s.z, gD);
+  ex[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+// gx = gxn + scattered extras; clears the extras.
+__global__ __launch_bounds__(kBlock) void k_bwd_fold(int n, const float4* __restrict__ gxn,
+                                                     float4* __restrict__ ex,
+                                                     float4* __restrict__ gx) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const float4 g = gxn[i], e = ex[i];
+  gx[i] = make_float4(g.x + e.x, g.y + e.y, g.z + e.z, 0.0f);
+  ex[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+// Prediction x[0] = x + dt (v + dt g) (fluids/Simulator_kernel.cuh:14-15),
+// back to the frame's input order through the sort permutation.
+template <int L>
+__global__ __launch_bounds__(kBlock) void k_bwd_predict(int n, StepConsts c,
+                                                        const uint32_t* __restrict__ sval,
+                                                        const float4* __restrict__ gx,
+                                                        const float4* __restrict__ gx0,
+                                                        float* __restrict__ gpos,
+                                                        float* __restrict__ gvel) {
+  const int s = blockIdx.x * kBlock + threadIdx.x;
+  if (s >= n) return;
+  const uint32_t src = sval[s] < (uint32_t)n ? sval[s] : 0u;
+  const float4 g = gx[s], g0 = gx0[s];
+  store_vec<L>(gpos, src, g0.x + g.x, g0.y + g.y, g0.z + g.z);
+  store_vec<L>(gvel, src, c.dt * g.x, c.dt * g.y, c.dt * g.z);
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_f4(float4* __restrict__ a, int n) {
+  const int stride = gridDim.x * kBlock;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += stride)
+    a[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_f64(double* __restrict__ a, int n) {
+  const int stride = gridDim.x * kBlock;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) a[i] = 0.0;
+}
+
+// Layout conversion of a caller vector array to float4.
+template <int L>
+__global__ __launch_bounds__(kBlock) void k_to_f4(const float* __restrict__ a, int n,
+                                                  float4* __restrict__ out) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  out[i] = load_vec<L>(a, (uint32_t)i);
+}
+
+// Sum of n doubles by one workgroup in a fixed order.
+__global__ __launch_bounds__(kBlock) void k_sum_f64(const double* __restrict__ a, int n,
+                                                    double* __restrict__ out) {
+  __shared__ double sh[kBlock];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += kBlock) s += a[i];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = kBlock / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = sh[0];
+}
+
+int fill_blocks(int n) {
+  const int b = (n + kBlock - 1) / kBlock;
+  return b < 1 ? 1 : (b > 4096 ? 4096 : b);
+}
+
+}  // namespace
+
+void launch_bwd_xsph(const BwdFrame& f, int n, const StepConsts& c, const uint32_t* cs,
+                     const HeavyLists& hl, int pass, const Shapes& sh, hipStream_t st) {
+  launch_pass<BwdXsphPass>(BwdXsphPass::Args{f.x_final, f.vr, f.gv_out, f.key, f.out_x, f.out_v,
+                                             f.ex_x, f.ex_v},
+                           n, c, Scenes{}, cs, hl, pass, sh, st);
+  k_bwd_velocity<<<blocks_for(n), kBlock, 0, st>>>(n, c, f.gx_out, f.gv_out, f.out_x, f.out_v,
+                                                   f.ex_x, f.ex_v, f.gx, f.gx0, f.grho_x);
+}
+
+void launch_bwd_iteration(const BwdFrame& f, const BwdIter& it, int last, int n,
+                          const StepConsts& c, const uint32_t* cs, const HeavyLists& hl, int pass,
+                          const Shapes& sh, hipStream_t st) {
+  k_bwd_clamp<<<blocks_for(n), kBlock, 0, st>>>(n, c, f.gx, it.xl, it.dp, f.gxn, f.A, f.r0);
+  launch_pass<BwdDeltaPass>(BwdDeltaPass::Args{it.xl, f.A, f.key, f.out_x, f.ex_x}, n, c,
+                            Scenes{}, cs, hl, pass, sh, st);
+  k_bwd_lambda_pre<<<blocks_for(n), kBlock, 0, st>>>(n, c, last, f.out_x, f.ex_x, it.xl, it.sd,
+                                                     it.rho, f.grho_x, f.gxn, f.xr, f.sdb, f.r0);
+  launch_pass<BwdLambdaPass>(BwdLambdaPass::Args{f.xr, f.sdb, f.key, f.gxn, f.ex_x}, n, c,
+                             Scenes{}, cs, hl, pass + 1, sh, st);
+  k_bwd_fold<<<blocks_for(n), kBlock, 0, st>>>(n, f.gxn, f.ex_x, f.gx);
+}
+
+void launch_bwd_predict(int layout, const BwdFrame& f, const uint32_t* sval, int n,
+                        const StepConsts& c, float* gpos, float* gvel, hipStream_t st) {
+  if (layout == 3)
+    k_bwd_predict<3><<<blocks_for(n), kBlock, 0, st>>>(n, c, sval, f.gx, f.gx0, gpos, gvel);
+  else
+    k_bwd_predict<4><<<blocks_for(n), kBlock, 0, st>>>(n, c, sval, f.gx, f.gx0, gpos, gvel);
+}
+
+void launch_fill_f4(float4* a, int n, hipStream_t st) {
+  k_fill_f4<<<fill_blocks(n), kBlock, 0, st>>>(a, n);
+}
+
+void launch_fill_f64(double* a, int n, hipStream_t st) {
+  k_fill_f64<<<fill_blocks(n), kBlock, 0, st>>>(a, n);
+}
+
+void launch_to_f4(int layout, const float* a, int n, float4* out, hipStream_t st) {
+  if (layout == 3)
+    k_to_f4<3><<<blocks_for(n), kBlock, 0, st>>>(a, n, out);
+  else
+    k_to_f4<4><<<blocks_for(n), kBlock, 0, st>>>(a, n, out);
+}
+
+void launch_sum_f64(const double* a, int n, double* out, hipStream_t st) {
+  k_sum_f64<<<1, kBlock, 0, st>>>(a, n, out);
+}
+
+}  // namespace pbf

@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "pbf.h"
+#include "pbf_adjoint.h"
 #include "pbf_launch.h"
 
 namespace {
@@ -105,6 +106,28 @@ struct pbf_solver {
 
   Timer timer;
   std::string err;
+};
+
+// Checkpoints and backward scratch of a differentiated run (one allocation).
+struct pbf_tape {
+  int device = 0;
+  int n = 0, nframes = 0, niter = 0, layout = PBF_LAYOUT_FLOAT4;
+  bool forward_done = false;
+  void* arena = nullptr;
+  std::vector<float*> cpos, cvel;    // per checkpoint 0..nframes (caller layout)
+  std::vector<uint32_t*> ciid;
+  float *wpos = nullptr, *wvel = nullptr, *wnpos = nullptr, *wnvel = nullptr;
+  uint32_t* wiid = nullptr;
+  std::vector<float4*> xl, sd, dp;   // per iteration
+  std::vector<float*> rho;
+  float4 *out_x = nullptr, *out_v = nullptr, *ex_x = nullptr, *ex_v = nullptr, *gx = nullptr,
+         *gxn = nullptr, *gx0 = nullptr, *A = nullptr, *xr = nullptr, *sdb = nullptr,
+         *gxo = nullptr, *gvo = nullptr;
+  float* grho_x = nullptr;
+  float *gp[2] = {nullptr, nullptr}, *gv[2] = {nullptr, nullptr};  // inter-frame adjoints
+  double *r0 = nullptr, *r0_sum = nullptr;
+  pbf_params prm{};                 // parameters of the forward run
+  std::vector<pbf_vec3> ul, ll;     // box of each frame of the forward run
 };
 
 namespace {
@@ -321,6 +344,19 @@ void drop_graphs(pbf_solver* s) {
   s->graphs.clear();
 }
 
+// What a tape keeps of one frame's forward pass (all null outside a tape):
+// per iteration k the (x, lambda) array, (S, D), densities and the
+// correction before its clamps; and, set by enqueue_step, where the sorted
+// keys and the sort permutation ended up.
+struct Record {
+  float4** xl = nullptr;
+  float4** sd = nullptr;
+  float** rho = nullptr;
+  float4** dp = nullptr;
+  const uint32_t* skey = nullptr;
+  const uint32_t* sval = nullptr;
+};
+
 // Caller arrays of one step.
 struct StepArrays {
   float *pos, *npos, *vel, *nvel;
@@ -348,9 +384,10 @@ int step_consts(pbf_solver* s, pbf::StepConsts& c) {
 }
 
 // The launch sequence of one frame, on validated arguments.  timed: record
-// the phase events (never inside a graph capture).
+// the phase events (never inside a graph capture).  rec: keep the frame's
+// iterations for a backward pass.
 void enqueue_step(pbf_solver* s, const StepArrays& a, const pbf::StepConsts& c, hipStream_t st,
-                  bool timed) {
+                  bool timed, Record* rec = nullptr) {
   auto tb = [&](int ph) {
     if (timed) tbegin(s, ph, st);
   };
@@ -373,28 +410,37 @@ void enqueue_step(pbf_solver* s, const StepArrays& a, const pbf::StepConsts& c, 
                                           key_bits_for(c.cells), s->hist, s->totals, st);
   te(PBF_PHASE_SORT);
   const uint32_t* sval = which ? s->val1 : s->val0;
+  if (rec) {
+    rec->skey = which ? s->key1 : s->key0;
+    rec->sval = sval;
+  }
   // memset + computeGridRange (Simulator.cu:200-204), fused with the payload gather
   tb(PBF_PHASE_GRID);
   pbf::launch_cell_starts(s->cnt, c.cells, s->tile_sum, s->cs, st);
   pbf::launch_permute(L, sval, a.pos, s->xp, a.iid, n, s->x0s, s->xa, s->iid_s, st);
   te(PBF_PHASE_GRID);
   // niter x correctDensity (Simulator.cpp:62-66, Simulator.cu:213-249);
-  // xl (x, lambda) aliases xp, which is dead after the permute.
-  float4* xl = s->xp;
+  // xl (x, lambda) aliases xp, which is dead after the permute (a tape keeps
+  // one xl per iteration instead).
   int pass = 0;
-  for (int it = 0; it < s->prm.niter; ++it) {
+  const int NI = s->prm.niter;
+  for (int it = 0; it < NI; ++it) {
+    float4* xl = rec ? rec->xl[it] : s->xp;
+    float* rho = rec ? rec->rho[it] : (it + 1 == NI ? s->rho : nullptr);
     tb(PBF_PHASE_LAMBDA);
-    pbf::launch_lambda(s->xa, s->cs, n, c, scn, xl, it + 1 == s->prm.niter ? s->rho : nullptr,
-                       s->hl, pass++, sh, st);
+    pbf::launch_lambda(s->xa, s->cs, n, c, scn, xl, rho, rec ? rec->sd[it] : nullptr, s->hl,
+                       pass++, sh, st);
     te(PBF_PHASE_LAMBDA);
     tb(PBF_PHASE_DELTA);
-    pbf::launch_delta(xl, s->cs, n, c, scn, s->xa, s->hl, pass++, sh, st);
+    pbf::launch_delta(xl, s->cs, n, c, scn, s->xa, rec ? rec->dp[it] : nullptr, s->hl, pass++, sh,
+                      st);
     te(PBF_PHASE_DELTA);
   }
   // updateVelocity (Simulator.cpp:72, Simulator.cu:267-274); vr aliases xp.
   float4* vr = s->xp;
   tb(PBF_PHASE_VEL);
-  pbf::launch_velocity(L, s->xa, s->x0s, s->iid_s, s->rho, n, c, vr, a.pos, a.vel, a.iid, st);
+  pbf::launch_velocity(L, s->xa, s->x0s, s->iid_s, rec ? rec->rho[NI - 1] : s->rho, n, c, vr,
+                       a.pos, a.vel, a.iid, st);
   te(PBF_PHASE_VEL);
   // correctVelocity (Simulator.cpp:77, Simulator.cu:251-265)
   tb(PBF_PHASE_XSPH);
@@ -1050,6 +1096,285 @@ int pbf_get_timing(pbf_solver* s, double ms_total[PBF_NUM_PHASES],
     t.ms[p] = 0.0;
     t.cnt[p] = 0;
   }
+  return PBF_OK;
+}
+
+
+int pbf_tape_create(pbf_solver* s, int n, int nframes, pbf_tape** out) {
+  if (!out) return fail(s, PBF_ERR_INVALID_ARGUMENT, "out is NULL");
+  *out = nullptr;
+  if (!s) return fail(nullptr, PBF_ERR_INVALID_ARGUMENT, "solver is NULL");
+  if (s->nscenes) return fail(s, PBF_ERR_STATE, "tapes are for single-scene solvers");
+  if (n < 1 || n > s->capacity)
+    return fail(s, PBF_ERR_INVALID_ARGUMENT, "n must be in [1, capacity = %d], got %d", s->capacity,
+                n);
+  if (nframes < 1 || nframes > 100000)
+    return fail(s, PBF_ERR_INVALID_ARGUMENT, "nframes must be in [1, 100000], got %d", nframes);
+  DeviceGuard g(s->device);
+  if (!g.ok) return fail(s, PBF_ERR_HIP, "hipSetDevice(%d) failed", s->device);
+  pbf_tape* t = new (std::nothrow) pbf_tape();
+  if (!t) return fail(s, PBF_ERR_INVALID_ARGUMENT, "out of host memory");
+  t->device = s->device;
+  t->n = n;
+  t->nframes = nframes;
+  t->niter = s->prm.niter;
+  t->layout = s->layout;
+  const size_t N = (size_t)n, L = (size_t)s->layout, NI = (size_t)t->niter;
+  size_t off = 0;
+  auto carve = [&](size_t bytes) {
+    const size_t o = off;
+    off = align_up(off + bytes);
+    return o;
+  };
+  std::vector<size_t> o_cp, o_cv, o_ci;
+  for (int f = 0; f <= nframes; ++f) {
+    o_cp.push_back(carve(4 * L * N));
+    o_cv.push_back(carve(4 * L * N));
+    o_ci.push_back(carve(4 * N));
+  }
+  const size_t o_wp = carve(4 * L * N), o_wv = carve(4 * L * N), o_wnp = carve(4 * L * N),
+               o_wnv = carve(4 * L * N), o_wi = carve(4 * N);
+  std::vector<size_t> o_xl, o_sd, o_dp, o_rho;
+  for (size_t k = 0; k < NI; ++k) {
+    o_xl.push_back(carve(16 * N));
+    o_sd.push_back(carve(16 * N));
+    o_dp.push_back(carve(16 * N));
+    o_rho.push_back(carve(4 * N));
+  }
+  size_t o_f4[12];
+  for (auto& o : o_f4) o = carve(16 * N);
+  const size_t o_grx = carve(4 * N), o_gp0 = carve(4 * L * N), o_gp1 = carve(4 * L * N),
+               o_gv0 = carve(4 * L * N), o_gv1 = carve(4 * L * N), o_r0 = carve(8 * N),
+               o_r0s = carve(8);
+  hipError_t e = hipMalloc(&t->arena, off);
+  if (e != hipSuccess) {
+    delete t;
+    return fail_hip(s, e, "hipMalloc of the tape");
+  }
+  char* b = (char*)t->arena;
+  for (int f = 0; f <= nframes; ++f) {
+    t->cpos.push_back((float*)(b + o_cp[f]));
+    t->cvel.push_back((float*)(b + o_cv[f]));
+    t->ciid.push_back((uint32_t*)(b + o_ci[f]));
+  }
+  t->wpos = (float*)(b + o_wp);
+  t->wvel = (float*)(b + o_wv);
+  t->wnpos = (float*)(b + o_wnp);
+  t->wnvel = (float*)(b + o_wnv);
+  t->wiid = (uint32_t*)(b + o_wi);
+  for (size_t k = 0; k < NI; ++k) {
+    t->xl.push_back((float4*)(b + o_xl[k]));
+    t->sd.push_back((float4*)(b + o_sd[k]));
+    t->dp.push_back((float4*)(b + o_dp[k]));
+    t->rho.push_back((float*)(b + o_rho[k]));
+  }
+  float4** f4[12] = {&t->out_x, &t->out_v, &t->ex_x, &t->ex_v, &t->gx, &t->gxn,
+                     &t->gx0,   &t->A,     &t->xr,   &t->sdb,  &t->gxo, &t->gvo};
+  for (int q = 0; q < 12; ++q) *f4[q] = (float4*)(b + o_f4[q]);
+  t->grho_x = (float*)(b + o_grx);
+  t->gp[0] = (float*)(b + o_gp0);
+  t->gp[1] = (float*)(b + o_gp1);
+  t->gv[0] = (float*)(b + o_gv0);
+  t->gv[1] = (float*)(b + o_gv1);
+  t->r0 = (double*)(b + o_r0);
+  t->r0_sum = (double*)(b + o_r0s);
+  // the scatter arrays start (and stay between kernels) all zero
+  pbf::launch_fill_f4(t->ex_x, n, nullptr);
+  pbf::launch_fill_f4(t->ex_v, n, nullptr);
+  if ((e = hipDeviceSynchronize()) != hipSuccess) {
+    (void)hipFree(t->arena);
+    delete t;
+    return fail_hip(s, e, "clearing the tape");
+  }
+  *out = t;
+  return PBF_OK;
+}
+
+void pbf_tape_destroy(pbf_tape* t) {
+  if (!t) return;
+  {
+    DeviceGuard g(t->device);
+    if (t->arena) (void)hipFree(t->arena);
+  }
+  delete t;
+}
+
+int pbf_tape_state(const pbf_tape* t, int frame, float** pos, float** vel, uint32_t** iid) {
+  if (!t) return fail(nullptr, PBF_ERR_INVALID_ARGUMENT, "tape is NULL");
+  if (frame < 0 || frame > t->nframes)
+    return fail(nullptr, PBF_ERR_INVALID_ARGUMENT, "frame must be in [0, %d], got %d", t->nframes,
+                frame);
+  if (pos) *pos = t->cpos[(size_t)frame];
+  if (vel) *vel = t->cvel[(size_t)frame];
+  if (iid) *iid = t->ciid[(size_t)frame];
+  return PBF_OK;
+}
+
+int pbf_tape_read(const pbf_tape* t, int frame, float* pos, float* vel, uint32_t* iid,
+                  void* stream) {
+  if (!t) return fail(nullptr, PBF_ERR_INVALID_ARGUMENT, "tape is NULL");
+  if (frame < 0 || frame > t->nframes)
+    return fail(nullptr, PBF_ERR_INVALID_ARGUMENT, "frame must be in [0, %d], got %d", t->nframes,
+                frame);
+  DeviceGuard g(t->device);
+  hipStream_t st = (hipStream_t)stream;
+  const size_t vb = (size_t)t->n * (size_t)t->layout * 4, ib = (size_t)t->n * 4;
+  hipError_t e = hipSuccess;
+  if (pos) e = hipMemcpyAsync(pos, t->cpos[(size_t)frame], vb, hipMemcpyDeviceToDevice, st);
+  if (e == hipSuccess && vel)
+    e = hipMemcpyAsync(vel, t->cvel[(size_t)frame], vb, hipMemcpyDeviceToDevice, st);
+  if (e == hipSuccess && iid)
+    e = hipMemcpyAsync(iid, t->ciid[(size_t)frame], ib, hipMemcpyDeviceToDevice, st);
+  if (e != hipSuccess) return fail_hip(nullptr, e, "tape read");
+  return PBF_OK;
+}
+
+int pbf_tape_forward(pbf_solver* s, pbf_tape* t, const float* pos, const float* vel,
+                     const uint32_t* iid, int first_frame, const pbf_sweep* sweep,
+                     void* stream) {
+  if (!s) return fail(nullptr, PBF_ERR_INVALID_ARGUMENT, "solver is NULL");
+  if (!t) return fail(s, PBF_ERR_INVALID_ARGUMENT, "tape is NULL");
+  if (t->device != s->device || t->layout != s->layout || t->niter != s->prm.niter)
+    return fail(s, PBF_ERR_STATE, "tape made for another solver state (device, layout or niter)");
+  int rc;
+  if ((rc = check_vec_ptr(s, pos, "pos")) || (rc = check_vec_ptr(s, vel, "vel"))) return rc;
+  if (!iid) return fail(s, PBF_ERR_INVALID_ARGUMENT, "iid is NULL");
+  DeviceGuard g(s->device);
+  if (!g.ok) return fail(s, PBF_ERR_HIP, "hipSetDevice(%d) failed", s->device);
+  hipStream_t st = (hipStream_t)stream;
+  const size_t vb = (size_t)t->n * (size_t)t->layout * 4, ib = (size_t)t->n * 4;
+  hipError_t e;
+  t->forward_done = false;
+  if ((e = hipMemcpyAsync(t->cpos[0], pos, vb, hipMemcpyDeviceToDevice, st)) != hipSuccess ||
+      (e = hipMemcpyAsync(t->cvel[0], vel, vb, hipMemcpyDeviceToDevice, st)) != hipSuccess ||
+      (e = hipMemcpyAsync(t->ciid[0], iid, ib, hipMemcpyDeviceToDevice, st)) != hipSuccess)
+    return fail_hip(s, e, "tape checkpoint copy");
+  t->ul.assign((size_t)t->nframes, pbf_vec3{});
+  t->ll.assign((size_t)t->nframes, pbf_vec3{});
+  for (int f = 0; f < t->nframes; ++f) {
+    if (sweep) {
+      pbf_vec3 u, l;
+      pbf_sweep_box(sweep, first_frame + f, &u, &l);
+      if ((rc = pbf_set_lim(s, u, l))) return rc;
+    }
+    t->ul[(size_t)f] = s->ulim;
+    t->ll[(size_t)f] = s->llim;
+    // the step clobbers its inputs: step a copy of checkpoint f
+    if ((e = hipMemcpyAsync(t->wpos, t->cpos[f], vb, hipMemcpyDeviceToDevice, st)) != hipSuccess ||
+        (e = hipMemcpyAsync(t->wvel, t->cvel[f], vb, hipMemcpyDeviceToDevice, st)) != hipSuccess ||
+        (e = hipMemcpyAsync(t->ciid[f + 1], t->ciid[f], ib, hipMemcpyDeviceToDevice, st)) !=
+            hipSuccess)
+      return fail_hip(s, e, "tape checkpoint copy");
+    if ((rc = pbf_step(s, t->wpos, t->cpos[f + 1], t->wvel, t->cvel[f + 1], t->ciid[f + 1], t->n,
+                       stream)))
+      return rc;
+  }
+  t->prm = s->prm;
+  t->forward_done = true;
+  return PBF_OK;
+}
+
+int pbf_tape_backward(pbf_solver* s, pbf_tape* t, const float* gpos, const float* gvel,
+                      float* gpos0, float* gvel0, double* grho0, void* stream) {
+  if (!s) return fail(nullptr, PBF_ERR_INVALID_ARGUMENT, "solver is NULL");
+  if (!t) return fail(s, PBF_ERR_INVALID_ARGUMENT, "tape is NULL");
+  if (!t->forward_done) return fail(s, PBF_ERR_STATE, "pbf_tape_forward has not run on this tape");
+  if (t->device != s->device) return fail(s, PBF_ERR_STATE, "tape made on another device");
+  if (std::memcmp(&t->prm, &s->prm, sizeof(pbf_params)) != 0 || t->layout != s->layout)
+    return fail(s, PBF_ERR_STATE, "parameters changed since the forward run");
+  int rc;
+  if ((rc = check_vec_ptr(s, gpos0, "gpos0")) || (rc = check_vec_ptr(s, gvel0, "gvel0"))) return rc;
+  if ((gpos && (rc = check_vec_ptr(s, gpos, "gpos"))) || (gvel && (rc = check_vec_ptr(s, gvel, "gvel"))))
+    return rc;
+  DeviceGuard g(s->device);
+  if (!g.ok) return fail(s, PBF_ERR_HIP, "hipSetDevice(%d) failed", s->device);
+  hipStream_t st = (hipStream_t)stream;
+  const int n = t->n, L = t->layout, NI = t->niter;
+  const pbf_vec3 ul_saved = s->ulim, ll_saved = s->llim;
+  const size_t vb = (size_t)n * (size_t)L * 4, ib = (size_t)n * 4;
+  hipError_t e;
+  // seeds: dL/d(final state), caller layout
+  if (gpos) {
+    e = hipMemcpyAsync(t->gp[0], gpos, vb, hipMemcpyDeviceToDevice, st);
+  } else {
+    e = hipMemsetAsync(t->gp[0], 0, vb, st);
+  }
+  if (e == hipSuccess)
+    e = gvel ? hipMemcpyAsync(t->gv[0], gvel, vb, hipMemcpyDeviceToDevice, st)
+             : hipMemsetAsync(t->gv[0], 0, vb, st);
+  if (e != hipSuccess) return fail_hip(s, e, "seeding the backward pass");
+  pbf::launch_fill_f64(t->r0, n, st);
+  std::vector<float4*> xl(t->xl), sd(t->sd), dp(t->dp);
+  std::vector<float*> rho(t->rho);
+  Record rec;
+  rec.xl = xl.data();
+  rec.sd = sd.data();
+  rec.dp = dp.data();
+  rec.rho = rho.data();
+  int cur = 0;
+  for (int f = t->nframes - 1; f >= 0; --f) {
+    // frame f under its own box
+    s->ulim = t->ul[(size_t)f];
+    s->llim = t->ll[(size_t)f];
+    pbf::StepConsts c;
+    if ((rc = step_consts(s, c))) {
+      s->ulim = ul_saved;
+      s->llim = ll_saved;
+      return rc;
+    }
+    // recompute frame f from its checkpoint, keeping its iterations
+    if ((e = hipMemcpyAsync(t->wpos, t->cpos[f], vb, hipMemcpyDeviceToDevice, st)) != hipSuccess ||
+        (e = hipMemcpyAsync(t->wvel, t->cvel[f], vb, hipMemcpyDeviceToDevice, st)) != hipSuccess ||
+        (e = hipMemcpyAsync(t->wiid, t->ciid[f], ib, hipMemcpyDeviceToDevice, st)) != hipSuccess)
+      return fail_hip(s, e, "tape checkpoint copy");
+    if (s->cnt_dirty) pbf::launch_fill_u32(s->cnt, s->max_cells, 0u, st);
+    s->cnt_dirty = true;
+    const StepArrays arr{t->wpos, t->wnpos, t->wvel, t->wnvel, t->wiid, n};
+    enqueue_step(s, arr, c, st, false, &rec);
+    // adjoint of the frame
+    pbf::launch_to_f4(L, t->gp[cur], n, t->gxo, st);
+    pbf::launch_to_f4(L, t->gv[cur], n, t->gvo, st);
+    pbf::BwdFrame bf{};
+    bf.x_final = s->xa;
+    bf.vr = s->xp;
+    bf.key = rec.skey;
+    bf.gx_out = t->gxo;
+    bf.gv_out = t->gvo;
+    bf.out_x = t->out_x;
+    bf.out_v = t->out_v;
+    bf.ex_x = t->ex_x;
+    bf.ex_v = t->ex_v;
+    bf.gx = t->gx;
+    bf.gxn = t->gxn;
+    bf.gx0 = t->gx0;
+    bf.A = t->A;
+    bf.xr = t->xr;
+    bf.sdb = t->sdb;
+    bf.grho_x = t->grho_x;
+    bf.r0 = t->r0;
+    int pass = 2 * NI + 1;  // the recompute used neighbour passes 0 .. 2 niter
+    pbf::launch_bwd_xsph(bf, n, c, s->cs, s->hl, pass++, s->shapes, st);
+    for (int k = NI - 1; k >= 0; --k) {
+      const pbf::BwdIter it{t->xl[(size_t)k], t->sd[(size_t)k], t->rho[(size_t)k],
+                            t->dp[(size_t)k]};
+      pbf::launch_bwd_iteration(bf, it, k == NI - 1, n, c, s->cs, s->hl, pass, s->shapes, st);
+      pass += 2;
+    }
+    float* op = f == 0 ? gpos0 : t->gp[cur ^ 1];
+    float* ov = f == 0 ? gvel0 : t->gv[cur ^ 1];
+    pbf::launch_bwd_predict(L, bf, rec.sval, n, c, op, ov, st);
+    cur ^= 1;
+    if ((e = hipGetLastError()) != hipSuccess) return fail_hip(s, e, "kernel launch");
+    s->cnt_dirty = false;
+  }
+  s->ulim = ul_saved;
+  s->llim = ll_saved;
+  pbf::launch_sum_f64(t->r0, n, t->r0_sum, st);
+  double host = 0.0;
+  if ((e = hipMemcpyAsync(&host, t->r0_sum, sizeof(double), hipMemcpyDeviceToHost, st)) != hipSuccess)
+    return fail_hip(s, e, "hipMemcpyAsync");
+  if ((e = hipStreamSynchronize(st)) != hipSuccess) return fail_hip(s, e, "hipStreamSynchronize");
+  if (grho0) *grho0 = host;
   return PBF_OK;
 }
 

@@ -53,12 +53,15 @@ void launch_permute(int layout, const uint32_t* sval, const float* pos, const fl
                     hipStream_t st);
 void launch_permute_pos(const uint32_t* sval, const float4* xp, int n, float4* xa,
                         hipStream_t st);
+// rho (may be null) receives the densities; sd and dp (null outside a tape)
+// receive (sum of lambda gradients, denominator) and the correction / pho0
+// before the clamps, per particle.
 void launch_lambda(const float4* xa, const uint32_t* cs, int n, const StepConsts& c,
-                   const Scenes& scn, float4* xl, float* rho, const HeavyLists& hl, int pass,
-                   const Shapes& sh, hipStream_t st);
+                   const Scenes& scn, float4* xl, float* rho, float4* sd, const HeavyLists& hl,
+                   int pass, const Shapes& sh, hipStream_t st);
 void launch_delta(const float4* xl, const uint32_t* cs, int n, const StepConsts& c,
-                  const Scenes& scn, float4* xa, const HeavyLists& hl, int pass, const Shapes& sh,
-                  hipStream_t st);
+                  const Scenes& scn, float4* xa, float4* dp, const HeavyLists& hl, int pass,
+                  const Shapes& sh, hipStream_t st);
 void launch_velocity(int layout, const float4* xa, const float4* x0s, const uint32_t* iid_s,
                      const float* rho, int n, const StepConsts& c, float4* vr, float* pos,
                      float* vel, uint32_t* iid, hipStream_t st);

@@ -1,0 +1,235 @@
+// pbf_pass.h — the generic neighbour-pass kernels of the gfx950 PBF step,
+// instantiated by each translation unit over its pass policies.
+//
+// A pass policy P defines, for one neighbour pass:
+//   Args                 kernel arguments (device arrays)
+//   Target, Acc          a target's own data; its running sum (<= 8 floats)
+//   target(a, i, c)      load target i
+//   load_s(a, j)         candidate data for the screening test
+//   load_f(a, j)         candidate data for the pair term
+//   hit(t, j, d, c)      cheap test: false only if the pair adds exactly 0
+//   add(acc, t, j, d, c, a) the pair term (the body of the reference's loop)
+//   accumulate(tot, p)   tot += a chunk's partial sum
+//   finish(a, t, s, c, g) store the result of target t
+// Each neighbour pass is three kernels:
+//   light   one work-item per target over the cell-sorted order; workgroups
+//           remapped so each XCD's L2 serves one contiguous particle range; a
+//           target with more than kLightMax candidates is appended to the
+//           heavy list instead;
+//   chunks  one work-item per kChunk-candidate chunk of the heavy targets;
+//   combine one work-item per heavy target, adding its chunk sums in order.
+// Both follow the canonical summation order of pbf_device.h.
+#pragma once
+
+#include "pbf_device.h"
+#include "pbf_launch.h"
+
+namespace pbf {
+namespace {
+
+constexpr int kBlock = 256;
+
+inline int blocks_for(long long n) { return (int)((n + kBlock - 1) / kBlock); }
+
+// ---------------------------------------------------------------------------
+// Generic neighbour-pass kernels over a pass policy.
+// ---------------------------------------------------------------------------
+
+// Light: one work-item per target; the workgroup size is a launch shape
+// (64, 128 or 256), the column tables live in dynamic LDS.  A target with
+// more than kLightMax candidates is heavy: it gets one chunk item per kChunk
+// candidates and a record, appended with one atomic per wavefront each.
+// counters[pass & 1] belong to this pass; the kernel clears the other
+// parity's counters for the next pass (their previous users finished earlier
+// on the stream).  If the chunk list is full the lane sums its target itself,
+// still chunk by chunk, so the canonical order holds either way.
+template <class P, class S>
+__global__ __launch_bounds__(kBlock) void k_light(typename P::Args a, int n, StepConsts c, S sc,
+                                                  const uint32_t* __restrict__ cs, HeavyLists hl,
+                                                  int pass) {
+  extern __shared__ uint2 seq_dyn[];
+  __shared__ __align__(16) uint32_t key_sh[kBlock];
+  __shared__ uint32_t order_sh[kBlock];
+  uint32_t* cnt = hl.counters + 4 * (pass & 1);
+  if (blockIdx.x == 0 && threadIdx.x < 4) hl.counters[4 * ((pass + 1) & 1) + threadIdx.x] = 0u;
+  const int tid = threadIdx.x;
+  const int nb = (int)blockDim.x;
+  const int block_base = xcd_block_index() * nb;
+  const int i = block_base + tid;
+  const bool active = i < n;
+  const SeqTable st{&seq_dyn[tid], nb};
+  typename P::Target t{};
+  uint32_t m = 0;
+  if (active) {
+    t = P::target(a, (uint32_t)i, c);
+    const SceneGeo g = sc.at((uint32_t)i, c);
+    m = build_seq(st, cell_of(t.x.x, t.x.y, t.x.z, g, c.h), cs, g);
+  }
+  const bool is_heavy = active && m > kLightMax;
+  const uint32_t nch = is_heavy ? (m + kChunk - 1) / kChunk : 0u;
+  bool in_lane = false;  // chunk list full: this lane sums its chunks itself
+  if (__ballot(is_heavy) != 0ull) {
+    // Wave-aggregated allocation.  Partials are target-major (lane l owns
+    // [first, first + nch)); the work items are chunk-major over the wave
+    // (chunk 0 of every heavy lane, then chunk 1, ...), so the lanes of a
+    // chunk kernel wavefront work on the same chunk of neighbouring targets,
+    // which in a crowded cell share their candidate sequence: the candidate
+    // loads coalesce instead of being gathered lane by lane.
+    uint32_t item_total;
+    const uint32_t item_off = wave_exclusive_sum(nch, &item_total);
+    uint32_t item_base = 0, rec_base = 0;
+    if ((threadIdx.x & 63) == 0) item_base = atomicAdd(&cnt[1], item_total);
+    item_base = __shfl(item_base, 0);
+    const uint32_t first = item_base + item_off;
+    const bool wave_fits = item_base + item_total <= hl.item_cap;
+    uint32_t maxnch = nch;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const uint32_t o = __shfl_xor(maxnch, off);
+      maxnch = o > maxnch ? o : maxnch;
+    }
+    const uint64_t lt = (1ull << (threadIdx.x & 63)) - 1ull;
+    uint32_t pos = item_base;
+    for (uint32_t ch = 0; ch < maxnch; ++ch) {
+      const bool has = nch > ch;
+      const uint64_t bm = __ballot(has);
+      const uint32_t u = pos + (uint32_t)__popcll(bm & lt);
+      if (has && u < hl.item_cap)
+        hl.items[u] = wave_fits ? make_uint4((uint32_t)i, ch, first + ch, 0u)
+                                : make_uint4(~0u, 0u, 0u, 0u);
+      pos += (uint32_t)__popcll(bm);
+    }
+    const uint64_t hm = __ballot(is_heavy);
+    if (wave_fits) {
+      if ((threadIdx.x & 63) == 0) rec_base = atomicAdd(&cnt[0], (uint32_t)__popcll(hm));
+      rec_base = __shfl(rec_base, 0);
+      if (is_heavy)
+        hl.recs[rec_base + (uint32_t)__popcll(hm & lt)] = make_uint4((uint32_t)i, first, nch, 0u);
+    } else {
+      in_lane = is_heavy;
+    }
+  }
+  // Lanes take the block's remaining targets in order of candidate count (a
+  // stable rank in LDS), so each wavefront gets targets of similar cost
+  // instead of waiting on its longest lane.  The targets stay in this
+  // block's particle range, whose neighbourhoods overlap.  Each target's sum
+  // is unchanged: only the lane that computes it moves.
+  const uint32_t key = active && (!is_heavy || in_lane) ? m + 1u : 0u;
+  key_sh[tid] = key;
+  __syncthreads();
+  uint32_t rank = 0;
+  for (int j = 0; j < nb; j += 4) {
+    const uint4 k4 = *reinterpret_cast<const uint4*>(&key_sh[j]);
+    rank += (k4.x < key || (k4.x == key && j < tid)) ? 1u : 0u;
+    rank += (k4.y < key || (k4.y == key && j + 1 < tid)) ? 1u : 0u;
+    rank += (k4.z < key || (k4.z == key && j + 2 < tid)) ? 1u : 0u;
+    rank += (k4.w < key || (k4.w == key && j + 3 < tid)) ? 1u : 0u;
+  }
+  order_sh[rank] = (uint32_t)tid;
+  __syncthreads();
+  const int src = (int)order_sh[tid];
+  const uint32_t ks = key_sh[src];
+  if (ks == 0u) return;  // idle slot, or a target handed to the chunk kernels
+  const uint32_t ms = ks - 1u;
+  const uint32_t ti = (uint32_t)(block_base + src);
+  const SeqTable sts{&seq_dyn[src], nb};
+  t = P::target(a, ti, c);
+  auto ls = [&](uint32_t j) { return P::load_s(a, j); };
+  auto lf = [&](uint32_t j) { return P::load_f(a, j); };
+  auto hit = [&](uint32_t j, const auto& d) { return P::hit(t, j, d, c); };
+  typename P::Acc s;
+  if (ms <= kLightMax) {
+    sweep_seq(sts, 0u, ms, ls, lf, hit, [&](uint32_t j, const auto& d) { P::add(s, t, j, d, c, a); });
+  } else {
+    // chunk list full: this lane sums the target's chunks in order
+    const uint32_t nc = (ms + kChunk - 1) / kChunk;
+    for (uint32_t ch = 0; ch < nc; ++ch) {
+      typename P::Acc part;
+      const uint32_t s0 = ch * kChunk;
+      const uint32_t s1 = s0 + kChunk < ms ? s0 + kChunk : ms;
+      sweep_seq(sts, s0, s1, ls, lf, hit,
+                [&](uint32_t j, const auto& d) { P::add(part, t, j, d, c, a); });
+      P::accumulate(s, part);
+    }
+  }
+  P::finish(a, t, s, c, sc.at(ti, c));
+}
+
+// One work-item per chunk item (i, chunk, partial slot): the partial sum of
+// kChunk consecutive candidates of a heavy target.  Grid-stride over the
+// items; the grid size is a launch shape.
+template <class P, class S>
+__global__ __launch_bounds__(kBlock) void k_chunks(typename P::Args a, StepConsts c, S sc,
+                                                   const uint32_t* __restrict__ cs, HeavyLists hl,
+                                                   int pass) {
+  __shared__ uint2 seq[10][kBlock];
+  const uint32_t* cnt = hl.counters + 4 * (pass & 1);
+  const uint32_t total = cnt[1] < hl.item_cap ? cnt[1] : hl.item_cap;
+  const uint32_t stride = gridDim.x * kBlock;
+  const SeqTable st{&seq[0][threadIdx.x], kBlock};
+  for (uint32_t u = (uint32_t)xcd_block_index() * kBlock + threadIdx.x; u < total; u += stride) {
+    const uint4 it = hl.items[u];
+    if (it.x == ~0u) continue;
+    const typename P::Target t = P::target(a, it.x, c);
+    const SceneGeo g = sc.at(it.x, c);
+    const uint32_t m = build_seq(st, cell_of(t.x.x, t.x.y, t.x.z, g, c.h), cs, g);
+    typename P::Acc part;
+    const uint32_t s0 = it.y * kChunk;
+    const uint32_t s1 = s0 + kChunk < m ? s0 + kChunk : m;
+    sweep_seq(
+        st, s0, s1, [&](uint32_t j) { return P::load_s(a, j); },
+        [&](uint32_t j) { return P::load_f(a, j); },
+        [&](uint32_t j, const auto& d) { return P::hit(t, j, d, c); },
+        [&](uint32_t j, const auto& d) { P::add(part, t, j, d, c, a); });
+    static_assert(sizeof(part) <= kPartialWords * sizeof(float), "partial too large");
+    *reinterpret_cast<typename P::Acc*>(hl.partials + (size_t)it.z * kPartialWords) = part;
+  }
+}
+
+// One work-item per heavy target: chunk partials added in chunk order.
+template <class P, class S>
+__global__ __launch_bounds__(kBlock) void k_combine(typename P::Args a, StepConsts c, S sc,
+                                                    HeavyLists hl, int pass) {
+  const uint32_t* cnt = hl.counters + 4 * (pass & 1);
+  const uint32_t nrec = cnt[0];
+  const uint32_t stride = gridDim.x * kBlock;
+  for (uint32_t r = blockIdx.x * kBlock + threadIdx.x; r < nrec; r += stride) {
+    const uint4 rec = hl.recs[r];
+    const typename P::Target t = P::target(a, rec.x, c);
+    typename P::Acc s;
+    for (uint32_t k = 0; k < rec.z; ++k)
+      P::accumulate(
+          s, *reinterpret_cast<const typename P::Acc*>(hl.partials +
+                                                       (size_t)(rec.y + k) * kPartialWords));
+    P::finish(a, t, s, c, sc.at(rec.x, c));
+  }
+}
+
+int grid_for_items(long long items, int max_blocks) {
+  const long long b = (items + kBlock - 1) / kBlock;
+  return (int)(b < 1 ? 1 : (b > max_blocks ? max_blocks : b));
+}
+
+template <class P, class S>
+void launch_pass_s(const typename P::Args& a, int n, const StepConsts& c, const S& sc,
+                   const uint32_t* cs, const HeavyLists& hl, int pass, const Shapes& sh,
+                   hipStream_t st) {
+  const int lb = sh.light_block;
+  k_light<P, S><<<(n + lb - 1) / lb, lb, 10 * lb * sizeof(uint2), st>>>(a, n, c, sc, cs, hl, pass);
+  k_chunks<P, S><<<grid_for_items(hl.item_cap, sh.chunk_blocks), kBlock, 0, st>>>(a, c, sc, cs,
+                                                                                    hl, pass);
+  k_combine<P, S><<<grid_for_items(n, 512), kBlock, 0, st>>>(a, c, sc, hl, pass);
+}
+
+template <class P>
+void launch_pass(const typename P::Args& a, int n, const StepConsts& c, const Scenes& scn,
+                 const uint32_t* cs, const HeavyLists& hl, int pass, const Shapes& sh,
+                 hipStream_t st) {
+  if (scn.geo)
+    launch_pass_s<P>(a, n, c, SceneTable{scn.geo, scn.sid}, cs, hl, pass, sh, st);
+  else
+    launch_pass_s<P>(a, n, c, OneScene{}, cs, hl, pass, sh, st);
+}
+
+}  // namespace
+}  // namespace pbf

@@ -57,7 +57,7 @@ struct LambdaPass {
     return dist2(t.x, xj, dx, dy, dz) < c.h2_outer;
   }
   static __device__ __forceinline__ void add(Acc& s, const Target& t, uint32_t j, const float4& xj,
-                                             const StepConsts& c) {
+                                             const StepConsts& c, const Args&) {
     float dx, dy, dz;
     const float r2 = dist2(t.x, xj, dx, dy, dz);
     if (r2 < c.h2_outer) {
@@ -130,7 +130,7 @@ struct DeltaPass {
     return j != t.i && dist2(t.x, xj, dx, dy, dz) < c.h2_outer;
   }
   static __device__ __forceinline__ void add(Acc& s, const Target& t, uint32_t j, const float4& xj,
-                                             const StepConsts& c) {
+                                             const StepConsts& c, const Args&) {
     if (j == t.i) return;
     float dx, dy, dz;
     const float r2 = dist2(t.x, xj, dx, dy, dz);
@@ -205,7 +205,7 @@ struct XsphPass {
     return dist2(t.x, xj, dx, dy, dz) < c.h2;
   }
   static __device__ __forceinline__ void add(Acc& s, const Target& t, uint32_t, const Nb& nb,
-                                             const StepConsts& c) {
+                                             const StepConsts& c, const Args&) {
     float dx, dy, dz;
     const float r2 = dist2(t.x, nb.x, dx, dy, dz);
     if (r2 < c.h2) {
@@ -255,7 +255,7 @@ struct DensityPass {
     return dist2(t.x, xj, dx, dy, dz) < c.h2;
   }
   static __device__ __forceinline__ void add(Acc& s, const Target& t, uint32_t, const float4& xj,
-                                             const StepConsts& c) {
+                                             const StepConsts& c, const Args&) {
     float dx, dy, dz;
     const float r2 = dist2(t.x, xj, dx, dy, dz);
     if (r2 < c.h2) s.rho = s.rho + poly6_of(r2, c);

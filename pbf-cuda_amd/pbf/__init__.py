@@ -6,5 +6,5 @@ over the C ABI in libpbf.so; `scenes` builds seeded particle scenes;
 """
 from ._lib import PBFError, LIB_PATH, load  # noqa: F401
 from .params import FluidParams, dam_params, reference_defaults  # noqa: F401
-from .simulator import BatchedSimulator, Simulator, Sweep  # noqa: F401
+from .simulator import BatchedSimulator, Simulator, Sweep, Tape  # noqa: F401
 from . import scenes  # noqa: F401

@@ -103,6 +103,15 @@ _SIGNATURES = [
     ("pbf_autotune", ctypes.c_int,
      [_P, _P, _P, _P, ctypes.c_int, ctypes.c_int, _P, ctypes.POINTER(PbfShapes),
       ctypes.POINTER(ctypes.c_double)]),
+    ("pbf_tape_create", ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_P)]),
+    ("pbf_tape_destroy", None, [_P]),
+    ("pbf_tape_forward", ctypes.c_int,
+     [_P, _P, _P, _P, _P, ctypes.c_int, ctypes.POINTER(PbfSweep), _P]),
+    ("pbf_tape_state", ctypes.c_int,
+     [_P, ctypes.c_int, ctypes.POINTER(_P), ctypes.POINTER(_P), ctypes.POINTER(_P)]),
+    ("pbf_tape_read", ctypes.c_int, [_P, ctypes.c_int, _P, _P, _P, _P]),
+    ("pbf_tape_backward", ctypes.c_int,
+     [_P, _P, _P, _P, _P, _P, ctypes.POINTER(ctypes.c_double), _P]),
     ("pbf_last_error", ctypes.c_char_p, [_P]),
     ("pbf_status_string", ctypes.c_char_p, [ctypes.c_int]),
     ("pbf_abi_version", ctypes.c_int, []),

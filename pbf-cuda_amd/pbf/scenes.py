@@ -149,6 +149,24 @@ def batch_scenes(nscenes: int = 64, seed: int = SEED_BASE + 3, h: float = 0.1,
     return out
 
 
+def differentiable(seed: int = SEED_BASE + 4, h: float = 0.1) -> Scene:
+    """The differentiated workload: a 32x16x32 jittered lattice (16,384
+    particles, spacing 0.5h, jitter +-0.05h) in the floor corner of a
+    40h x 20h x 40h box, initial velocities N(0, 0.5^2) per component."""
+    return dataclasses.replace(block((32, 16, 32), (40 * h, 20 * h, 40 * h), seed, h,
+                                     vel_sigma=0.5), name="differentiable_16k")
+
+
+def target_cloud(sc: Scene, seed: int = SEED_BASE + 5) -> np.ndarray:
+    """A seeded target point per particle identity (N x 3, FP64): uniform in
+    the lower half of the scene's box."""
+    u = uniform01(seed, 0, 3 * sc.n).reshape(sc.n, 3)
+    lo = np.asarray(sc.llim, np.float64)
+    hi = np.asarray(sc.ulim, np.float64)
+    hi = lo + (hi - lo) * np.array([1.0, 1.0, 0.5])
+    return lo + u * (hi - lo)
+
+
 def concat(scenes: Sequence[Scene], layout: int):
     """The batched arrays of `scenes`: pos, vel (N_total, layout), iid, and
     the per-scene counts, boxes and offsets."""

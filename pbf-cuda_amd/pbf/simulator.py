@@ -234,6 +234,65 @@ class Simulator:
         return (int(best.light_block), int(best.chunk_blocks)), float(ms.value)
 
 
+class Tape:
+    """Checkpointed run of `nframes` frames of a single-scene Simulator and
+    its backward pass (pbf_tape_*)."""
+
+    def __init__(self, sim: "Simulator", nparticle: int, nframes: int):
+        self._lib = _lib.load()
+        self.sim = sim
+        self.n = int(nparticle)
+        self.nframes = int(nframes)
+        self._t = ctypes.c_void_p()
+        h = sim._handle()
+        _lib.check(self._lib.pbf_tape_create(h, self.n, self.nframes, ctypes.byref(self._t)), h)
+
+    def close(self) -> None:
+        if self._t:
+            self._lib.pbf_tape_destroy(self._t)
+            self._t = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def forward(self, pos, vel, iid, stream=None, first_frame: int = 0,
+                sweep: Optional[Sweep] = None) -> None:
+        """Runs the frames from (pos, vel, iid) (not modified); with a sweep,
+        frame k runs in sweep.box(first_frame + k)."""
+        L, n = self.sim.layout, self.n
+        h = self.sim._handle()
+        _lib.check(self._lib.pbf_tape_forward(h, self._t, _ptr(pos, L, "pos", n, True),
+                                              _ptr(vel, L, "vel", n, True),
+                                              _ptr(iid, L, "iid", n, False), int(first_frame),
+                                              ctypes.byref(sweep.c) if sweep else None,
+                                              _stream_handle(stream)), h)
+
+    def read(self, frame: int, pos=None, vel=None, iid=None, stream=None) -> None:
+        """Copies checkpoint `frame` (0 = inputs, nframes = final state)."""
+        L, n = self.sim.layout, self.n
+        _lib.check(self._lib.pbf_tape_read(
+            self._t, int(frame), _ptr(pos, L, "pos", n, True) if pos is not None else None,
+            _ptr(vel, L, "vel", n, True) if vel is not None else None,
+            _ptr(iid, L, "iid", n, False) if iid is not None else None, _stream_handle(stream)),
+            None)
+
+    def backward(self, gpos, gvel, gpos0, gvel0, stream=None) -> float:
+        """gpos/gvel: dL/d(final state) in the final order (None = 0); fills
+        gpos0/gvel0 = dL/d(initial state); returns dL/dpho0.  Synchronises."""
+        L, n = self.sim.layout, self.n
+        g = ctypes.c_double(0.0)
+        h = self.sim._handle()
+        _lib.check(self._lib.pbf_tape_backward(
+            h, self._t, _ptr(gpos, L, "gpos", n, True) if gpos is not None else None,
+            _ptr(gvel, L, "gvel", n, True) if gvel is not None else None,
+            _ptr(gpos0, L, "gpos0", n, True), _ptr(gvel0, L, "gvel0", n, True), ctypes.byref(g),
+            _stream_handle(stream)), h)
+        return float(g.value)
+
+
 class BatchedSimulator(Simulator):
     """Many independent box scenes stepped together (pbf_create_batched).
 

@@ -1,0 +1,214 @@
+"""GPU tests of the differentiable rollout (pbf_tape_* through the C ABI and
+pbf.autograd):
+
+* the tape's forward run equals plain steps bit for bit;
+* the GPU adjoint (FP32 kernels) against the oracle's FP64 adjoint along the
+  same FP32 trajectory (the oracle's rollout with fp32=True replays the GPU's
+  forward exactly), so the comparison isolates the backward arithmetic;
+* the GPU gradient against FP64 central finite differences of the FP64
+  rollout (reference constants, where the rollout is smooth at the step);
+* the autograd.Function wrapper gives the same gradients as the C ABI.
+The oracle's adjoint itself is pinned by finite differences in
+tests/test_adjoint_oracle.py.
+"""
+import numpy as np
+import pytest
+import torch
+
+from conftest import from_dev, to_dev
+from pbf import params as P
+from pbf import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _scene(counts=(8, 8, 8), box=(1.0, 1.0, 1.0), seed=5):
+    return scenes.block(counts, box, seed=seed, vel_sigma=0.5)
+
+
+def _target(sc, seed=1):
+    rng = np.random.default_rng(seed)
+    return sc.pos.astype(np.float64) + rng.normal(0, 0.05, (sc.n, 3))
+
+
+def gpu_grad(pbf_mod, sc, p, nframes, target, layout=4):
+    """Loss gradients from the C ABI: (g_pos0, g_vel0 as N x 3 in input order,
+    g_rho0, final positions by identity)."""
+    n = sc.n
+    sim = pbf_mod.Simulator(p, sc.ulim, sc.llim, capacity=n, layout=layout)
+    tape = pbf_mod.Tape(sim, n, nframes)
+    pos, vel = sc.as_layout(layout)
+    dev = "cuda:0"
+    tape.forward(to_dev(pos, dev), to_dev(vel, dev), to_dev(sc.iid, dev))
+    xf = torch.zeros(n * layout, device=dev)
+    iidf = torch.zeros(n, dtype=torch.int32, device=dev)
+    tape.read(nframes, pos=xf, iid=iidf)
+    torch.cuda.synchronize()
+    X = from_dev(xf).reshape(n, layout)[:, :3].astype(np.float64)
+    I = from_dev(iidf).astype(np.int64)
+    g = np.zeros((n, layout), np.float32)
+    g[:, :3] = 2.0 * (X - target[I]) / n
+    gp0 = torch.zeros(n * layout, device=dev)
+    gv0 = torch.zeros(n * layout, device=dev)
+    grho = tape.backward(to_dev(g, dev), None, gp0, gv0)
+    x_by_iid = np.zeros((n, 3))
+    x_by_iid[I] = X
+    return (from_dev(gp0).reshape(n, layout)[:, :3].astype(np.float64),
+            from_dev(gv0).reshape(n, layout)[:, :3].astype(np.float64), grho, x_by_iid)
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def test_tape_forward_equals_steps(pbf_mod, device):
+    from test_gpu_parity import Run
+
+    p = P.dam_params()
+    sc = _scene()
+    run = Run(pbf_mod, sc, p, 4)
+    sim = pbf_mod.Simulator(p, sc.ulim, sc.llim, capacity=sc.n)
+    tape = pbf_mod.Tape(sim, sc.n, 5)
+    pos, vel = sc.as_layout(4)
+    tape.forward(to_dev(pos, device), to_dev(vel, device), to_dev(sc.iid, device))
+    for f in range(1, 6):
+        g = run.step()
+        xp = torch.zeros(sc.n * 4, device=device)
+        xv = torch.zeros(sc.n * 4, device=device)
+        xi = torch.zeros(sc.n, dtype=torch.int32, device=device)
+        tape.read(f, xp, xv, xi)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(from_dev(xp).reshape(-1, 4), g["npos"])
+        np.testing.assert_array_equal(from_dev(xv).reshape(-1, 4), g["nvel"])
+        np.testing.assert_array_equal(from_dev(xi).astype(np.uint32), g["iid"])
+
+
+@pytest.mark.parametrize("pname,nframes,layout", [("reference", 3, 4), ("reference", 3, 3),
+                                                  ("dam", 1, 4)])
+def test_adjoint_matches_oracle_on_the_same_trajectory(pbf_mod, orc, device, pname, nframes,
+                                                       layout):
+    p = P.dam_params() if pname == "dam" else P.reference_defaults()
+    sc = _scene()
+    tgt = _target(sc)
+    print(f"adjoint {pname} frames {nframes} layout {layout} n {sc.n}", flush=True)
+    gp, gv, gr, xf = gpu_grad(pbf_mod, sc, p, nframes, tgt, layout)
+    o = orc.rollout(p, sc.ulim, sc.llim, sc.pos, sc.vel, sc.iid, nframes, tgt, fp32=True)
+    np.testing.assert_array_equal(xf.astype(np.float32), o["x_final"].astype(np.float32))
+    assert rel(gv, o["g_vel0"]) < 1e-4, rel(gv, o["g_vel0"])
+    assert rel(gp, o["g_pos0"]) < 1e-4, rel(gp, o["g_pos0"])
+    assert abs(gr - o["g_rho0"]) < 1e-3 * abs(o["g_rho0"]), (gr, o["g_rho0"])
+
+
+def test_adjoint_with_scattered_pairs(pbf_mod, orc, device):
+    """Fast particles leave their stale cells, so candidate lists become
+    asymmetric and the scatter path carries terms; still equal to the oracle."""
+    p = P.reference_defaults()
+    sc = scenes.block((8, 8, 8), (1.0, 1.0, 1.0), seed=9, vel_sigma=6.0)
+    tgt = _target(sc)
+    gp, gv, gr, _ = gpu_grad(pbf_mod, sc, p, 2, tgt)
+    o = orc.rollout(p, sc.ulim, sc.llim, sc.pos, sc.vel, sc.iid, 2, tgt, fp32=True)
+    assert rel(gv, o["g_vel0"]) < 1e-4, rel(gv, o["g_vel0"])
+    assert rel(gp, o["g_pos0"]) < 1e-4, rel(gp, o["g_pos0"])
+
+
+def test_gpu_gradient_against_fp64_central_differences(pbf_mod, orc, device):
+    """GPU gradient (FP32 trajectory and adjoint) against FP64 central
+    differences (step 1e-4) of the FP64 rollout: within 1e-3 relative."""
+    p = P.reference_defaults()
+    sc = _scene()
+    tgt = _target(sc)
+    nf = 5
+    gp, gv, gr, _ = gpu_grad(pbf_mod, sc, p, nf, tgt)
+    rng = np.random.default_rng(11)
+    loss = lambda v=sc.vel, rho0=0.0: orc.rollout(p, sc.ulim, sc.llim, sc.pos, v, sc.iid, nf, tgt,
+                                                  pho0=rho0, grad=False)["loss"]
+    for _ in range(3):
+        d = rng.normal(size=(sc.n, 3))
+        fd = (loss(sc.vel + 1e-4 * d) - loss(sc.vel - 1e-4 * d)) / 2e-4
+        an = float((gv * d).sum())
+        assert abs(fd - an) <= 1e-3 * abs(fd), (fd, an)
+    rho0 = float(np.float32(p.pho0))
+    fd = (loss(rho0=rho0 * (1 + 1e-4)) - loss(rho0=rho0 * (1 - 1e-4))) / (2e-4 * rho0)
+    assert abs(fd - gr) <= 1e-3 * abs(fd), (fd, gr)
+
+
+def test_autograd_function(pbf_mod, orc, device):
+    from pbf.autograd import rollout
+
+    p = P.reference_defaults()
+    sc = _scene()
+    tgt = _target(sc)
+    n = sc.n
+    sim = pbf_mod.Simulator(p, sc.ulim, sc.llim, capacity=n)
+    tape = pbf_mod.Tape(sim, n, 3)
+    pos, vel = sc.as_layout(4)
+    pos0 = to_dev(pos, device).view(n, 4).requires_grad_(True)
+    vel0 = to_dev(vel, device).view(n, 4).requires_grad_(True)
+    rho0 = torch.tensor(float(np.float32(p.pho0)), dtype=torch.float64, device=device,
+                        requires_grad=True)
+    iid0 = to_dev(sc.iid, device)
+    x = rollout(sim, tape, pos0, vel0, rho0, iid0)
+    loss = ((x.double() - torch.from_numpy(tgt).to(device)) ** 2).sum() / n
+    loss.backward()
+    o = orc.rollout(p, sc.ulim, sc.llim, sc.pos, sc.vel, sc.iid, 3, tgt, fp32=True)
+    assert abs(loss.item() - o["loss"]) < 1e-6 * o["loss"]
+    gv = vel0.grad.cpu().numpy()[:, :3].astype(np.float64)
+    gp = pos0.grad.cpu().numpy()[:, :3].astype(np.float64)
+    assert rel(gv, o["g_vel0"]) < 1e-4 and rel(gp, o["g_pos0"]) < 1e-4
+    assert abs(rho0.grad.item() - o["g_rho0"]) < 1e-3 * abs(o["g_rho0"])
+
+
+def test_tape_argument_errors(pbf_mod, device):
+    p = P.reference_defaults()
+    sc = _scene()
+    sim = pbf_mod.Simulator(p, sc.ulim, sc.llim, capacity=sc.n)
+    with pytest.raises(pbf_mod.PBFError):
+        pbf_mod.Tape(sim, sc.n + 1, 3)
+    with pytest.raises(pbf_mod.PBFError):
+        pbf_mod.Tape(sim, sc.n, 0)
+    tape = pbf_mod.Tape(sim, sc.n, 2)
+    g = torch.zeros(sc.n * 4, device=device)
+    with pytest.raises(pbf_mod.PBFError) as e:
+        tape.backward(None, None, g, g.clone())
+    assert e.value.code == 5  # no forward run yet
+    pos, vel = sc.as_layout(4)
+    tape.forward(to_dev(pos, device), to_dev(vel, device), to_dev(sc.iid, device))
+    sim.loadParams(p.replace(c_XSPH=0.1))
+    with pytest.raises(pbf_mod.PBFError) as e:
+        tape.backward(None, None, g, g.clone())
+    assert e.value.code == 5  # parameters changed
+    bsim = pbf_mod.BatchedSimulator(p, [sc.ulim], [sc.llim], [sc.n])
+    with pytest.raises(pbf_mod.PBFError) as e:
+        pbf_mod.Tape(bsim, sc.n, 2)
+    assert e.value.code == 5
+
+
+def test_adjoint_with_a_moving_wall_matches_oracle(pbf_mod, orc, device):
+    p = P.reference_defaults()
+    sc = _scene()
+    tgt = _target(sc)
+    sw = pbf_mod.Sweep(sc.ulim, sc.llim, a_ulim=(-0.75, 0.0, 0.0), w=0.5)
+    n = sc.n
+    sim = pbf_mod.Simulator(p, sc.ulim, sc.llim, capacity=n)
+    tape = pbf_mod.Tape(sim, n, 3)
+    pos, vel = sc.as_layout(4)
+    tape.forward(to_dev(pos, device), to_dev(vel, device), to_dev(sc.iid, device), sweep=sw)
+    assert sim.getLim()[0][0] == pytest.approx(sw.box(2)[0][0])
+    xf = torch.zeros(n * 4, device=device)
+    iidf = torch.zeros(n, dtype=torch.int32, device=device)
+    tape.read(3, pos=xf, iid=iidf)
+    torch.cuda.synchronize()
+    X = from_dev(xf).reshape(n, 4)[:, :3].astype(np.float64)
+    I = from_dev(iidf).astype(np.int64)
+    g = np.zeros((n, 4), np.float32)
+    g[:, :3] = 2.0 * (X - tgt[I]) / n
+    gp0 = torch.zeros(n * 4, device=device)
+    gv0 = torch.zeros(n * 4, device=device)
+    tape.backward(to_dev(g, device), None, gp0, gv0)
+    gv = from_dev(gv0).reshape(n, 4)[:, :3].astype(np.float64)
+    o = orc.rollout(p, sc.ulim, sc.llim, sc.pos, sc.vel, sc.iid, 3, tgt, fp32=True,
+                    boxes=[sw.box(f) for f in range(3)])
+    x_by_iid = np.zeros((n, 3))
+    x_by_iid[I] = X
+    np.testing.assert_array_equal(x_by_iid.astype(np.float32), o["x_final"].astype(np.float32))
+    assert rel(gv, o["g_vel0"]) < 1e-4, rel(gv, o["g_vel0"])

@@ -6,12 +6,16 @@ known answers, brute force, and invariants chosen so that a plausible slip
 them.  CPU only."""
 import math
 
+import os
+
 import numpy as np
 import pytest
 from scipy import integrate
 
 from pbf import params as P
 from pbf import scenes
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 PI_REF = 3.14159265359  # fluids/helper.h:4
 
@@ -322,14 +326,20 @@ def test_multiframe_density_error_reference_defaults(orc):
 @pytest.mark.slow
 def test_multiframe_density_error_dam_constants_within_fp64_ensemble(orc):
     """The dam constants are chaotic over 200 frames: FP64 runs that differ
-    by 1 ulp of input spread by several percent.  The FP32 sequence's
-    frame-averaged mean and max density errors must lie within 1% of the
-    envelope of an FP64 ensemble."""
+    by 1 ulp of input spread by about +-5% (mean) and +-15% (max).  The FP32
+    sequence's frame-averaged mean and max density errors must lie within 1%
+    of the envelope of an 8-member FP64 ensemble.  The ensemble is frozen in
+    tests/golden/density_ensemble_dam.json (minutes of CPU); its unperturbed
+    member is re-run here, which pins the fixture to the oracle."""
+    import json
+
+    fx = json.load(open(os.path.join(GOLDEN, "density_ensemble_dam.json")))
+    ens = np.array(fx["fp64_mean_max"])
+    assert ens.shape == (8, 2) and fx["frames"] == 200
     sc = scenes.double_dam()
     p = P.dam_params()
-    pos, _ = sc.as_layout(4)
-    ens = np.array([_density_run(orc, p, sc, 200, False, _perturb_ulp(pos, s) if s else None)
-                    for s in (0, 1, 2)])
+    a0 = _density_run(orc, p, sc, 200, False)
+    np.testing.assert_allclose(a0, ens[0], rtol=1e-12)
     b = _density_run(orc, p, sc, 200, True)
     lo, hi = ens.min(0), ens.max(0)
     assert np.all(b >= 0.99 * lo) and np.all(b <= 1.01 * hi), (b, ens)

@@ -44,6 +44,12 @@ def parse_args():
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batched rate")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the backward/forward ratio")
+    ap.add_argument("--sweep", action="store_true",
+                    help="also run the 4,194,304-particle sweeping-wall workload: 120 frames, "
+                         "6 iterations, backward over the last 10 frames")
+    ap.add_argument("--copy-only", action="store_true",
+                    help="only run a 256 MiB device-to-device copy 3 times (calibrates the "
+                         "FETCH_SIZE / WRITE_SIZE counters against known bytes)")
     return ap.parse_args()
 
 
@@ -221,6 +227,74 @@ def adjoint_ratio(pbf, p, dev, stream, reps=3):
                         "recomputed"}
 
 
+def sweep_workload(pbf, dev, stream, frames=120, back=10):
+    """The sweeping-wall workload: 4,194,304 particles, the +x wall moving as
+    160h + 20h sin(2 pi f / 120), dam constants with 6 iterations.  Frames
+    0 .. frames-back-1 run plainly (timed), the last `back` frames run on a
+    checkpointing tape, then dL/d(state at frame frames-back) is computed
+    through them (timed), L = mean squared distance to a seeded cloud."""
+    import numpy as np
+    import torch
+
+    from pbf import scenes
+
+    p = pbf.dam_params(niter=6)
+    sc = scenes.sweep_dam()
+    sw = pbf.Sweep(*scenes.sweep_of_sweep_dam())
+    n = sc.n
+    tgt = torch.from_numpy(scenes.target_cloud(sc)).to(dev)
+    sim = pbf.Simulator(p, sc.ulim, sc.llim, capacity=n)
+    pos, vel = sc.as_layout(4)
+    a = [torch.from_numpy(pos.ravel()).to(dev), torch.from_numpy(vel.ravel()).to(dev)]
+    b = [torch.empty_like(a[0]), torch.empty_like(a[1])]
+    iid = torch.from_numpy(sc.iid.astype(np.int32)).to(dev)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    plain = frames - back
+    with torch.cuda.stream(stream):
+        torch.cuda.synchronize()
+        e0.record(stream)
+        which = sim.run_frames(a[0], a[1], b[0], b[1], iid, n, 0, plain, sw, stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms_plain = e0.elapsed_time(e1) / plain
+        cur = b if which else a
+        free0 = torch.cuda.mem_get_info()[0]
+        tape = pbf.Tape(sim, n, back)
+        tape_bytes = free0 - torch.cuda.mem_get_info()[0]
+        torch.cuda.synchronize()
+        e0.record(stream)
+        tape.forward(cur[0], cur[1], iid, stream, first_frame=plain, sweep=sw)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms_tape = e0.elapsed_time(e1)
+        xf = torch.empty(n * 4, device=dev)
+        iidf = torch.empty(n, dtype=torch.int32, device=dev)
+        tape.read(back, pos=xf, iid=iidf, stream=stream)
+        g = torch.zeros(n, 4, device=dev)
+        g[:, :3] = (2.0 * (xf.view(n, 4)[:, :3].double() - tgt[iidf.long()]) / n).float()
+        gp0 = torch.empty(n * 4, device=dev)
+        gv0 = torch.empty(n * 4, device=dev)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        grho = tape.backward(g.view(-1), None, gp0, gv0, stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms_back = e0.elapsed_time(e1)
+    gv = gv0.view(n, 4)[:, :3]
+    return {"particles": n, "frames": frames, "niter": p.niter,
+            "plain_ms_per_frame": ms_plain,
+            "plain_rate": n * p.niter / (ms_plain * 1e-3),
+            "tape_forward_ms": ms_tape, "backward_ms": ms_back,
+            "backward_forward_ratio": ms_back / ms_tape,
+            "tape_device_bytes": int(tape_bytes),
+            "gradients_finite": bool(torch.isfinite(gv).all().item()) and bool(np.isfinite(grho)),
+            "workload": "sweep_dam_4m: 256x64x256 jittered lattice in a 160h x 64h x 160h box, "
+                        "+x wall 160h + 20h sin(2 pi f/120), dam constants, 6 iterations; "
+                        f"frames 0..{plain - 1} plain, {plain}..{frames - 1} on a tape, "
+                        "backward through the last 10"}
+
+
 def main():
     args = parse_args()
     if args.gpus != 1:
@@ -235,6 +309,14 @@ def main():
     from pbf import scenes
 
     dev = torch.device("cuda:0")
+    if args.copy_only:
+        a = torch.ones(64 << 20, device=dev)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": a.numel() * 4, "copies": 3}), flush=True)
+        return
     p = pbf.dam_params(niter=4)
     sc = scenes.single_dam()
     n = sc.n
@@ -311,6 +393,13 @@ def main():
         print(f"adjoint: forward {adjoint['forward_ms']:.2f} ms, backward "
               f"{adjoint['backward_ms']:.2f} ms", file=sys.stderr, flush=True)
 
+    sweep = None
+    if args.sweep:
+        sweep = sweep_workload(pbf, dev, stream)
+        print(f"sweep: {sweep['plain_ms_per_frame']:.2f} ms/frame, backward "
+              f"{sweep['backward_ms']:.1f} ms / tape forward {sweep['tape_forward_ms']:.1f} ms",
+              file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
         cur = eager.current()
@@ -348,6 +437,8 @@ def main():
         "batched_rate": batched,
         "backward_forward_ratio": adjoint,
     }
+    if sweep is not None:
+        out["sweep_workload"] = sweep
     print(json.dumps(out), flush=True)
 
 

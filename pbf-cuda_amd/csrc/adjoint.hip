@@ -210,7 +210,7 @@ struct BwdDeltaPass {
   static __device__ __forceinline__ bool hit(const Target& t, uint32_t j, const float4& xj,
                                              const StepConsts& c) {
     float dx, dy, dz;
-    return j != t.i && dist2(t.x, xj, dx, dy, dz) < c.h2_outer;
+    return (j != t.i) & (dist2(t.x, xj, dx, dy, dz) < c.h2_outer);
   }
   static __device__ __forceinline__ void add(Acc& s, const Target& t, uint32_t j, const Nb& nb,
                                              const StepConsts& c, const Args& a) {

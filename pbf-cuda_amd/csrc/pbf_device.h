@@ -95,28 +95,6 @@ __device__ __forceinline__ uint32_t cell_id(const Cell& q, const SceneGeo& g) {
   return g.base + (uint32_t)((q.x * g.dy + q.y) * g.dz + q.z);
 }
 
-// Candidate neighbours of a particle whose query cell is q, in the
-// reference's order (fluids/Simulator_kernel.cuh:78-88): dx, then dy, then
-// dz, j ascending.  The three cells (x,y,z-1..z+1) of one column are
-// consecutive cell ids, so their particles form one contiguous sorted range,
-// read from the cell-start table cs (cells + 1 entries; cs[c] = first sorted
-// slot whose cell id is >= c): [cs[first cell], cs[last cell + 1]).  Walking
-// that range is the same j sequence as walking the three cells in turn, and
-// costs two neighbouring loads instead of three cell lookups.
-// column_range(q, k) is column k = 3*(dx+1) + (dy+1); outside the grid or
-// empty, it is an empty range.
-__device__ __forceinline__ uint2 column_range(const Cell& q, int k,
-                                              const uint32_t* __restrict__ cs,
-                                              const SceneGeo& g) {
-  const int X = q.x + k / 3 - 1;
-  const int Y = q.y + k % 3 - 1;
-  if (X < 0 || X >= g.dx || Y < 0 || Y >= g.dy) return make_uint2(0u, 0u);
-  const uint32_t col = g.base + (uint32_t)((X * g.dy + Y) * g.dz);
-  const int z0 = q.z > 0 ? q.z - 1 : 0;
-  const int z1 = q.z + 1 < g.dz ? q.z + 1 : g.dz - 1;
-  return make_uint2(cs[col + z0], cs[col + z1 + 1]);
-}
-
 // Canonical summation order, shared with the host oracle's FP32 mirror: a
 // target's candidate sequence is summed sequentially when it has at most
 // kLightMax candidates (the "light" kernels, one work-item per target);
@@ -139,21 +117,46 @@ struct SeqTable {
   __device__ __forceinline__ uint2 at(int k) const { return tab[k * stride]; }
 };
 
-// Builds the table from the nine column ranges around query cell q and
-// returns the candidate count m.
+// Candidate neighbours of a particle whose query cell is q, in the
+// reference's order (fluids/Simulator_kernel.cuh:78-88): dx, then dy, then
+// dz, j ascending.  The three cells (x,y,z-1..z+1) of one column are
+// consecutive cell ids, so their particles form one contiguous sorted range,
+// read from the cell-start table cs (cells + 1 entries; cs[c] = first sorted
+// slot whose cell id is >= c): [cs[first cell], cs[last cell + 1]).  Walking
+// that range is the same j sequence as walking the three cells in turn.
+// build_seq builds the table from the nine column ranges around query cell q
+// (column k = 3*(dx+1) + (dy+1); outside the grid: empty) and returns the
+// candidate count m.  The 18 cell-start loads are issued
+// together (a column outside the grid reads a valid dummy address and is
+// then treated as empty), so building the table costs one memory round trip,
+// not one per column.
 __device__ __forceinline__ uint32_t build_seq(const SeqTable& st, const Cell& q,
                                               const uint32_t* __restrict__ cs,
                                               const SceneGeo& g) {
+  uint32_t lo[9], hi[9];
+  bool in[9];
+  const int z0 = q.z > 0 ? q.z - 1 : 0;
+  const int z1 = q.z + 1 < g.dz ? q.z + 1 : g.dz - 1;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    const int X = q.x + k / 3 - 1;
+    const int Y = q.y + k % 3 - 1;
+    in[k] = (X >= 0) & (X < g.dx) & (Y >= 0) & (Y < g.dy);
+    const uint32_t col = g.base + (in[k] ? (uint32_t)((X * g.dy + Y) * g.dz) : 0u);
+    lo[k] = cs[col + z0];
+    hi[k] = cs[col + z1 + 1];
+  }
+  // branch-free compaction: every column writes its entry at the next free
+  // index, which only advances past non-empty columns (an empty column's
+  // entry is overwritten by the next one or by the closing entry)
   uint32_t m = 0;
   int nk = 0;
 #pragma unroll
   for (int k = 0; k < 9; ++k) {
-    const uint2 r = column_range(q, k, cs, g);
-    if (r.y > r.x) {
-      st.tab[nk * st.stride] = make_uint2(r.x, m);
-      ++nk;
-      m += r.y - r.x;
-    }
+    const uint32_t len = (in[k] & (hi[k] > lo[k])) ? hi[k] - lo[k] : 0u;
+    st.tab[nk * st.stride] = make_uint2(lo[k], m);
+    nk += len ? 1 : 0;
+    m += len;
   }
   st.tab[nk * st.stride] = make_uint2(0u, m);
   return m;
@@ -197,6 +200,10 @@ struct SeqCursor {
 // one or two per column), then the hits are reloaded four at a time and
 // accumulated in ascending order.  Candidates failing hit() contribute
 // exactly nothing in add(), so the sums equal the plain in-order sum.
+// The screening is branch-free after the loads (all eight tests are
+// evaluated and masked), so the compiler keeps the eight loads in flight
+// together instead of sinking some of them into conditional blocks that
+// wait for each load in turn; hit() must be branch-free as well.
 template <class LoadS, class LoadF, class Hit, class Add>
 __device__ __forceinline__ void sweep_seq(const SeqTable& st, uint32_t u0, uint32_t u1,
                                           LoadS&& load_s, LoadF&& load_f, Hit&& hit, Add&& add) {
@@ -216,10 +223,13 @@ __device__ __forceinline__ void sweep_seq(const SeqTable& st, uint32_t u0, uint3
       decltype(load_s(0u)) d[8];
 #pragma unroll
       for (int t = 0; t < 8; ++t) d[t] = load_s(j[t]);
+      uint32_t bits = 0;
 #pragma unroll
-      for (int t = 0; t < 8; ++t)
-        if (u + t < be && hit(j[t], d[t])) mask |= 1u << (u - bb + t);
+      for (int t = 0; t < 8; ++t) bits |= (hit(j[t], d[t]) ? 1u : 0u) << t;
+      const uint32_t valid = be - u >= 8u ? 0xFFu : (1u << (be - u)) - 1u;
+      mask |= (bits & valid) << (u - bb);
     }
+    uint32_t last = 0;
 #pragma unroll 1
     while (mask) {
       uint32_t j[4];
@@ -227,13 +237,15 @@ __device__ __forceinline__ void sweep_seq(const SeqTable& st, uint32_t u0, uint3
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
         h[t] = mask != 0u;
-        const uint32_t bit = h[t] ? (uint32_t)__builtin_ctz(mask) : 0u;
+        // past the last hit, repeat it (the cursor only moves forward)
+        const uint32_t bit = h[t] ? (uint32_t)__builtin_ctz(mask) : last;
+        last = bit;
         mask &= mask - 1u;
-        j[t] = h[t] ? c2.slot(bb + bit) : 0u;
+        j[t] = c2.slot(bb + bit);
       }
       decltype(load_f(0u)) d[4];
 #pragma unroll
-      for (int t = 0; t < 4; ++t) d[t] = load_f(h[t] ? j[t] : j[0]);
+      for (int t = 0; t < 4; ++t) d[t] = load_f(j[t]);
 #pragma unroll
       for (int t = 0; t < 4; ++t)
         if (h[t]) add(j[t], d[t]);
@@ -268,6 +280,12 @@ constexpr int kPartialWords = 8;
 // x^n for the s_corr exponent: repeated multiplication for small integer n
 // (the reference default n=4), powf otherwise.
 __device__ __forceinline__ float pow_corr(float w, const StepConsts& c) {
+  if (c.n_corr_int == 4) {
+    // the reference default; exactly the product the loop below forms for
+    // n = 4 (1 * ((w*w) * (w*w)))
+    const float w2 = w * w;
+    return w2 * w2;
+  }
   if (c.n_corr_int >= 0) {
     float r = 1.0f, b = w;
     int e = c.n_corr_int;
@@ -297,6 +315,25 @@ __device__ __forceinline__ float dist2(const float4& a, const float4& b, float& 
   r2 = fmaf(dy, dy, r2);
   r2 = fmaf(dz, dz, r2);
   return r2;
+}
+
+// The same with the second position as a float3 (screening loads of x, y, z
+// only).
+__device__ __forceinline__ float dist2(const float4& a, const float3& b, float& dx, float& dy,
+                                       float& dz) {
+  dx = a.x - b.x;
+  dy = a.y - b.y;
+  dz = a.z - b.z;
+  float r2 = dx * dx;
+  r2 = fmaf(dy, dy, r2);
+  r2 = fmaf(dz, dz, r2);
+  return r2;
+}
+
+// x, y, z of element j of a float4 array (a 12-byte load).
+__device__ __forceinline__ float3 load_xyz(const float4* __restrict__ a, uint32_t j) {
+  const float* p = reinterpret_cast<const float*>(a + j);
+  return make_float3(p[0], p[1], p[2]);
 }
 
 // poly6 numerator times its coefficient, for r2 < h2: poly6 * (h2-r2)^3.

@@ -28,6 +28,7 @@ namespace pbf {
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kRankBuckets = 64;  // one wavefront scans them
 
 inline int blocks_for(long long n) { return (int)((n + kBlock - 1) / kBlock); }
 
@@ -48,8 +49,9 @@ __global__ __launch_bounds__(kBlock) void k_light(typename P::Args a, int n, Ste
                                                   const uint32_t* __restrict__ cs, HeavyLists hl,
                                                   int pass) {
   extern __shared__ uint2 seq_dyn[];
-  __shared__ __align__(16) uint32_t key_sh[kBlock];
+  __shared__ uint32_t key_sh[kBlock];
   __shared__ uint32_t order_sh[kBlock];
+  __shared__ uint32_t bucket_sh[kRankBuckets];
   uint32_t* cnt = hl.counters + 4 * (pass & 1);
   if (blockIdx.x == 0 && threadIdx.x < 4) hl.counters[4 * ((pass + 1) & 1) + threadIdx.x] = 0u;
   const int tid = threadIdx.x;
@@ -109,23 +111,28 @@ __global__ __launch_bounds__(kBlock) void k_light(typename P::Args a, int n, Ste
       in_lane = is_heavy;
     }
   }
-  // Lanes take the block's remaining targets in order of candidate count (a
-  // stable rank in LDS), so each wavefront gets targets of similar cost
-  // instead of waiting on its longest lane.  The targets stay in this
-  // block's particle range, whose neighbourhoods overlap.  Each target's sum
-  // is unchanged: only the lane that computes it moves.
+  // Lanes take the block's remaining targets in order of candidate count, so
+  // each wavefront gets targets of similar cost instead of waiting on its
+  // longest lane: a counting sort over buckets of 8 candidates in LDS
+  // (bucket 0: idle slots and targets handed to the chunk kernels).  The
+  // targets stay in this block's particle range, whose neighbourhoods
+  // overlap.  Each target's sum is unchanged: only the lane that computes it
+  // moves, so the order within a bucket (set by LDS atomics) does not matter.
   const uint32_t key = active && (!is_heavy || in_lane) ? m + 1u : 0u;
   key_sh[tid] = key;
+  if (tid < kRankBuckets) bucket_sh[tid] = 0u;
   __syncthreads();
-  uint32_t rank = 0;
-  for (int j = 0; j < nb; j += 4) {
-    const uint4 k4 = *reinterpret_cast<const uint4*>(&key_sh[j]);
-    rank += (k4.x < key || (k4.x == key && j < tid)) ? 1u : 0u;
-    rank += (k4.y < key || (k4.y == key && j + 1 < tid)) ? 1u : 0u;
-    rank += (k4.z < key || (k4.z == key && j + 2 < tid)) ? 1u : 0u;
-    rank += (k4.w < key || (k4.w == key && j + 3 < tid)) ? 1u : 0u;
+  const uint32_t bucket =
+      key == 0u ? 0u : 1u + min((key - 1u) >> 3, (uint32_t)kRankBuckets - 2u);
+  const uint32_t within = atomicAdd(&bucket_sh[bucket], 1u);
+  __syncthreads();
+  if (tid < kRankBuckets) {  // the first wavefront scans the bucket counts
+    uint32_t total;
+    const uint32_t cnt_b = bucket_sh[tid];
+    bucket_sh[tid] = wave_exclusive_sum(cnt_b, &total);
   }
-  order_sh[rank] = (uint32_t)tid;
+  __syncthreads();
+  order_sh[bucket_sh[bucket] + within] = (uint32_t)tid;
   __syncthreads();
   const int src = (int)order_sh[tid];
   const uint32_t ks = key_sh[src];

@@ -49,9 +49,11 @@ struct LambdaPass {
   static __device__ __forceinline__ Target target(const Args& a, uint32_t i, const StepConsts&) {
     return Target{a.x[i], i};
   }
-  static __device__ __forceinline__ float4 load_s(const Args& a, uint32_t j) { return a.x[j]; }
+  static __device__ __forceinline__ float3 load_s(const Args& a, uint32_t j) {
+    return load_xyz(a.x, j);
+  }
   static __device__ __forceinline__ float4 load_f(const Args& a, uint32_t j) { return a.x[j]; }
-  static __device__ __forceinline__ bool hit(const Target& t, uint32_t, const float4& xj,
+  static __device__ __forceinline__ bool hit(const Target& t, uint32_t, const float3& xj,
                                              const StepConsts& c) {
     float dx, dy, dz;
     return dist2(t.x, xj, dx, dy, dz) < c.h2_outer;
@@ -122,12 +124,14 @@ struct DeltaPass {
   static __device__ __forceinline__ Target target(const Args& a, uint32_t i, const StepConsts&) {
     return Target{a.xl[i], i};
   }
-  static __device__ __forceinline__ float4 load_s(const Args& a, uint32_t j) { return a.xl[j]; }
+  static __device__ __forceinline__ float3 load_s(const Args& a, uint32_t j) {
+    return load_xyz(a.xl, j);
+  }
   static __device__ __forceinline__ float4 load_f(const Args& a, uint32_t j) { return a.xl[j]; }
-  static __device__ __forceinline__ bool hit(const Target& t, uint32_t j, const float4& xj,
+  static __device__ __forceinline__ bool hit(const Target& t, uint32_t j, const float3& xj,
                                              const StepConsts& c) {
     float dx, dy, dz;
-    return j != t.i && dist2(t.x, xj, dx, dy, dz) < c.h2_outer;
+    return (j != t.i) & (dist2(t.x, xj, dx, dy, dz) < c.h2_outer);
   }
   static __device__ __forceinline__ void add(Acc& s, const Target& t, uint32_t j, const float4& xj,
                                              const StepConsts& c, const Args&) {
@@ -195,11 +199,13 @@ struct XsphPass {
   static __device__ __forceinline__ Target target(const Args& a, uint32_t i, const StepConsts&) {
     return Target{a.xa[i], a.vr[i], i};
   }
-  static __device__ __forceinline__ float4 load_s(const Args& a, uint32_t j) { return a.xa[j]; }
+  static __device__ __forceinline__ float3 load_s(const Args& a, uint32_t j) {
+    return load_xyz(a.xa, j);
+  }
   static __device__ __forceinline__ Nb load_f(const Args& a, uint32_t j) {
     return Nb{a.xa[j], a.vr[j]};
   }
-  static __device__ __forceinline__ bool hit(const Target& t, uint32_t, const float4& xj,
+  static __device__ __forceinline__ bool hit(const Target& t, uint32_t, const float3& xj,
                                              const StepConsts& c) {
     float dx, dy, dz;
     return dist2(t.x, xj, dx, dy, dz) < c.h2;

@@ -157,7 +157,7 @@ def differentiable(seed: int = SEED_BASE + 4, h: float = 0.1) -> Scene:
                                      vel_sigma=0.5), name="differentiable_16k")
 
 
-def target_cloud(sc: Scene, seed: int = SEED_BASE + 5) -> np.ndarray:
+def target_cloud(sc: Scene, seed: int = SEED_BASE + 0x40) -> np.ndarray:
     """A seeded target point per particle identity (N x 3, FP64): uniform in
     the lower half of the scene's box."""
     u = uniform01(seed, 0, 3 * sc.n).reshape(sc.n, 3)
@@ -165,6 +165,22 @@ def target_cloud(sc: Scene, seed: int = SEED_BASE + 5) -> np.ndarray:
     hi = np.asarray(sc.ulim, np.float64)
     hi = lo + (hi - lo) * np.array([1.0, 1.0, 0.5])
     return lo + u * (hi - lo)
+
+
+def sweep_dam(seed: int = SEED_BASE + 5, h: float = 0.1) -> Scene:
+    """The large sweeping-wall workload: a 256x64x256 jittered lattice
+    (4,194,304 particles, spacing 0.5h, jitter +-0.05h) in the floor corner
+    of a resting box 160h x 64h x 160h; the moving wall is sweep_of_sweep_dam()."""
+    return dataclasses.replace(single_dam(seed, h, counts=(256, 64, 256),
+                                          box_cells=(160, 64, 160)), name="sweep_dam_4m")
+
+
+def sweep_of_sweep_dam(h: float = 0.1, frames_per_period: int = 120):
+    """The +x wall of sweep_dam moves as ulim.x = 160h + 20h sin(2 pi f / 120)
+    (period 2 s at dt = 1/60): (ulim0, llim0, a_ulim, a_llim, w, start)."""
+    ul = (float(np.float32(160 * h)), float(np.float32(64 * h)), float(np.float32(160 * h)))
+    return ul, (0.0, 0.0, 0.0), (20 * h, 0.0, 0.0), (0.0, 0.0, 0.0), \
+        2.0 * np.pi / frames_per_period, 0
 
 
 def concat(scenes: Sequence[Scene], layout: int):

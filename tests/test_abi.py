@@ -212,3 +212,36 @@ def test_batch_scene_generator(pbf_mod):
     assert all(np.array_equal(a.pos, b.pos) for a, b in zip(scs, again))
     pos, vel, iid, counts, ul, ll, offs = scenes.concat(scs, 4)
     assert pos.shape == (ns.sum(), 4) and list(offs[1:]) == list(np.cumsum(ns)[:-1])
+
+
+def test_cpp_mirror_header_compiles_and_links(tmp_path):
+    """include/pbf_simulator.hpp (the C++ mirror of the reference's
+    Simulator) compiles against include/pbf.h and links with libpbf.so; the
+    program is built, not run (no device here)."""
+    import shutil
+    import subprocess
+
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("no g++")
+    src = tmp_path / "use.cpp"
+    src.write_text(
+        '#include "pbf_simulator.hpp"\n'
+        "int main() {\n"
+        "  pbf_params p{4, 6378.f, 9.8f, 0.1f, 1.f / 60, 600.f, 0.02f, 0.1f, 4.f, 0.f, 0.5f};\n"
+        "  try {\n"
+        "    pbf::Simulator sim(p, pbf_vec3{1, 1, 1}, pbf_vec3{0, 0, 0}, 100);\n"
+        "    pbf::Tape tape(sim, 100, 2);\n"
+        "    sim.setGraph(true);\n"
+        "    sim.setShapes(128, 1024);\n"
+        "  } catch (const pbf::Error& e) {\n"
+        "    return e.code();\n"
+        "  }\n"
+        "  return 0;\n"
+        "}\n")
+    libdir = os.path.join(ROOT, "pbf-cuda_amd", "pbf")
+    out = tmp_path / "use"
+    r = subprocess.run([gxx, "-std=c++17", "-I", os.path.join(ROOT, "include"), str(src), "-o",
+                        str(out), "-L", libdir, "-lpbf", "-L/opt/rocm/lib", "-lamdhip64",
+                        f"-Wl,-rpath,{libdir}"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-2000:]

@@ -162,14 +162,27 @@ __device__ __forceinline__ uint32_t build_seq(const SeqTable& st, const Cell& q,
   return m;
 }
 
-// Maps increasing sequence positions to sorted slots (u must be < m and
-// must not decrease between calls).
+// Maps increasing sequence positions to sorted slots (positions must be < m
+// and must not decrease between calls).  The cursor holds the current column
+// (first slot b0, positions [c0, n0)) and the next one (first slot b1,
+// positions [n0, n1)); a column exists after the current one exactly when
+// n0 < m, so the lookahead never reads past the table's closing entry.
 struct SeqCursor {
   SeqTable st;
   int k;
-  uint32_t b, cum, next;
-  __device__ __forceinline__ void init(const SeqTable& t, uint32_t u) {
+  uint32_t m, b0, c0, n0, b1, n1;
+  __device__ __forceinline__ void load_next() {
+    if (n0 < m) {
+      b1 = st.at(k + 1).x;
+      n1 = st.at(k + 2).y;
+    } else {
+      b1 = 0u;
+      n1 = n0;
+    }
+  }
+  __device__ __forceinline__ void init(const SeqTable& t, uint32_t u, uint32_t total) {
     st = t;
+    m = total;
     k = 0;
     uint2 e0 = st.at(0), e1 = st.at(1);
     while (u >= e1.y) {  // only when starting inside the sequence
@@ -177,9 +190,51 @@ struct SeqCursor {
       e0 = e1;
       e1 = st.at(k + 1);
     }
-    b = e0.x;
-    cum = e0.y;
-    next = e1.y;
+    b0 = e0.x;
+    c0 = e0.y;
+    n0 = e1.y;
+    load_next();
+  }
+  __device__ __forceinline__ void shift() {
+    ++k;
+    b0 = b1;
+    c0 = n0;
+    n0 = n1;
+    load_next();
+  }
+  __device__ __forceinline__ uint32_t slot(uint32_t u) {
+    while (u >= n0) shift();
+    return b0 + (u - c0);
+  }
+  // Slots of positions p0 .. p0+7, clamped to last_valid: branch-free while
+  // the batch spans at most the current and the next column.
+  __device__ __forceinline__ void slots8(uint32_t p0, uint32_t last_valid, uint32_t j[8]) {
+    const uint32_t last = p0 + 7u < last_valid ? p0 + 7u : last_valid;
+    if (last < n1) {
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        const uint32_t p = p0 + t < last_valid ? p0 + t : last_valid;
+        j[t] = p < n0 ? b0 + (p - c0) : b1 + (p - n0);
+      }
+      if (last >= n0) shift();
+    } else {
+#pragma unroll
+      for (int t = 0; t < 8; ++t) j[t] = slot(p0 + t < last_valid ? p0 + t : last_valid);
+    }
+  }
+};
+
+// The reload cursor: positions only (no lookahead), fewer registers.
+struct SeqCursor1 {
+  SeqTable st;
+  int k;
+  uint32_t b, cum, next;
+  __device__ __forceinline__ void from(const SeqCursor& c) {
+    st = c.st;
+    k = c.k;
+    b = c.b0;
+    cum = c.c0;
+    next = c.n0;
   }
   __device__ __forceinline__ uint32_t slot(uint32_t u) {
     while (u >= next) {
@@ -193,7 +248,8 @@ struct SeqCursor {
   }
 };
 
-// Visits sequence positions [u0, u1) of a candidate sequence in order,
+// Visits sequence positions [u0, u1) of a candidate sequence of m positions
+// in order,
 // calling add() only for candidates passing the cheap test hit().  Blocks of
 // 32 positions are screened eight loads at a time (across column
 // boundaries, so a sparse neighbourhood costs a few memory round trips, not
@@ -205,12 +261,14 @@ struct SeqCursor {
 // together instead of sinking some of them into conditional blocks that
 // wait for each load in turn; hit() must be branch-free as well.
 template <class LoadS, class LoadF, class Hit, class Add>
-__device__ __forceinline__ void sweep_seq(const SeqTable& st, uint32_t u0, uint32_t u1,
-                                          LoadS&& load_s, LoadF&& load_f, Hit&& hit, Add&& add) {
+__device__ __forceinline__ void sweep_seq(const SeqTable& st, uint32_t m, uint32_t u0,
+                                          uint32_t u1, LoadS&& load_s, LoadF&& load_f, Hit&& hit,
+                                          Add&& add) {
   if (u0 >= u1) return;
-  SeqCursor c1, c2;
-  c1.init(st, u0);
-  c2 = c1;
+  SeqCursor c1;
+  c1.init(st, u0, m);
+  SeqCursor1 c2;
+  c2.from(c1);
 #pragma unroll 1
   for (uint32_t bb = u0; bb < u1; bb += 32) {
     const uint32_t be = u1 - bb < 32u ? u1 : bb + 32u;
@@ -218,8 +276,7 @@ __device__ __forceinline__ void sweep_seq(const SeqTable& st, uint32_t u0, uint3
 #pragma unroll 1
     for (uint32_t u = bb; u < be; u += 8) {
       uint32_t j[8];
-#pragma unroll
-      for (int t = 0; t < 8; ++t) j[t] = c1.slot(u + t < be ? u + t : be - 1);
+      c1.slots8(u, be - 1u, j);
       decltype(load_s(0u)) d[8];
 #pragma unroll
       for (int t = 0; t < 8; ++t) d[t] = load_s(j[t]);

@@ -210,7 +210,8 @@ struct BwdDeltaPass {
   static __device__ __forceinline__ bool hit(const Target& t, uint32_t j, const float4& xj,
                                              const StepConsts& c) {
     float dx, dy, dz;
-    return (j != t.i) & (dist2(t.x, xj, dx, dy, dz) < c.h2_outer);
+    const float r2 = dist2(t.x, xj, dx, dy, dz);
+    return (j != t.i) & (r2 < c.h2_outer) & (r2 >= c.keps2_lo);
   }
   static __device__ __forceinline__ void add(Acc& s, const Target& t, uint32_t j, const Nb& nb,
                                              const StepConsts& c, const Args& a) {

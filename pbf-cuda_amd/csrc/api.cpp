@@ -240,6 +240,10 @@ int make_consts(pbf_solver* s, const pbf_params& p, pbf_vec3 ul, pbf_vec3 ll,
   c.h2 = p.h * p.h;
   c.h2_outer = c.h2 * 1.0001f;
   c.keps = kKernelEps;
+  // a strict lower bound of the pairs whose gradient term can be non-zero:
+  // sqrt is monotone and correctly rounded, so r2 < 0.99 keps^2 gives
+  // sqrtf(r2) <= sqrtf(0.99 keps^2) < keps
+  c.keps2_lo = (float)(0.99 * (double)kKernelEps * (double)kKernelEps);
   const double h = (double)p.h;
   // getPoly6 / getSpikyGrad coefficients (fluids/Simulator.cu:77-83, :94-98),
   // evaluated in FP64 and stored as FP32.

@@ -28,6 +28,7 @@ struct StepConsts {
   float h, h2;             // kernel radius, h*h
   float h2_outer;          // prefilter bound: r2 >= h2_outer => both kernels are 0
   float keps;              // KERNAL_EPS
+  float keps2_lo;          // 0.99 keps^2: r2 < keps2_lo => sqrtf(r2) < keps (no gradient)
   float poly6;             // 315/(64 pi h^9)
   float spiky;             // -45/(pi h^6)
   float spiky_rho0;        // spiky / pho0 (the lambda gradient, Simulator_kernel.cuh:92)

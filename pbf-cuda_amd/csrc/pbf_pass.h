@@ -204,10 +204,21 @@ __global__ __launch_bounds__(kBlock) void k_combine(typename P::Args a, StepCons
     const uint4 rec = hl.recs[r];
     const typename P::Target t = P::target(a, rec.x, c);
     typename P::Acc s;
-    for (uint32_t k = 0; k < rec.z; ++k)
-      P::accumulate(
-          s, *reinterpret_cast<const typename P::Acc*>(hl.partials +
-                                                       (size_t)(rec.y + k) * kPartialWords));
+    using PAcc = typename P::Acc;
+    const float* base = hl.partials + (size_t)rec.y * kPartialWords;
+    auto part = [&](uint32_t k) {
+      return *reinterpret_cast<const PAcc*>(base + (size_t)k * kPartialWords);
+    };
+    // eight loads in flight, then the adds in chunk order
+    uint32_t k = 0;
+    for (; k + 8u <= rec.z; k += 8u) {
+      PAcc p8[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) p8[q] = part(k + q);
+#pragma unroll
+      for (int q = 0; q < 8; ++q) P::accumulate(s, p8[q]);
+    }
+    for (; k < rec.z; ++k) P::accumulate(s, part(k));
     P::finish(a, t, s, c, sc.at(rec.x, c));
   }
 }

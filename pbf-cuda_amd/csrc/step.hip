@@ -64,7 +64,9 @@ struct LambdaPass {
     const float r2 = dist2(t.x, xj, dx, dy, dz);
     if (r2 < c.h2_outer) {
       if (r2 < c.h2) s.rho = s.rho + poly6_of(r2, c);
-      const float r = sqrtf(r2);
+      // coincident pairs (piles clamped into a corner) have no gradient term:
+      // skip their square root
+      const float r = r2 >= c.keps2_lo ? sqrtf(r2) : 0.0f;
       if (r >= c.keps && r < c.h) {
         const float u = c.h - r;
         const float f = ((c.spiky_rho0 * u) * u) / r;
@@ -131,7 +133,9 @@ struct DeltaPass {
   static __device__ __forceinline__ bool hit(const Target& t, uint32_t j, const float3& xj,
                                              const StepConsts& c) {
     float dx, dy, dz;
-    return (j != t.i) & (dist2(t.x, xj, dx, dy, dz) < c.h2_outer);
+    // pairs closer than KERNAL_EPS (coincident piles) add nothing either
+    const float r2 = dist2(t.x, xj, dx, dy, dz);
+    return (j != t.i) & (r2 < c.h2_outer) & (r2 >= c.keps2_lo);
   }
   static __device__ __forceinline__ void add(Acc& s, const Target& t, uint32_t j, const float4& xj,
                                              const StepConsts& c, const Args&) {

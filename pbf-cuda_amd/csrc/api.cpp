@@ -9,6 +9,7 @@
 // Simulator.cpp:44-79), explicit parameters instead of the GUIParams
 // singleton (Simulator.cpp:101-115), and status codes instead of exit()
 // (checkCudaErrors).
+#include <dlfcn.h>
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -37,6 +38,36 @@ constexpr int kMaxParticles = 1 << 30;
 constexpr int kMaxScenes = 65535;
 
 thread_local std::string g_create_error;
+
+// roctx ranges around the step and its phases, for rocprofv3 --marker-trace.
+// The rocprofiler-sdk roctx library is looked up at run time; without it the
+// markers are no-ops (no link-time dependency).
+struct Roctx {
+  using Push = int (*)(const char*);
+  using Pop = int (*)();
+  Push push = nullptr;
+  Pop pop = nullptr;
+  Roctx() {
+    void* h = dlopen("librocprofiler-sdk-roctx.so.1", RTLD_NOW | RTLD_LOCAL);
+    if (!h) return;
+    push = reinterpret_cast<Push>(dlsym(h, "roctxRangePushA"));
+    pop = reinterpret_cast<Pop>(dlsym(h, "roctxRangePop"));
+    if (!push || !pop) push = nullptr;
+  }
+};
+const Roctx& roctx() {
+  static const Roctx r;
+  return r;
+}
+void mark_push(const char* name) {
+  if (roctx().push) roctx().push(name);
+}
+void mark_pop() {
+  if (roctx().push) roctx().pop();
+}
+const char* const kPhaseName[PBF_NUM_PHASES] = {"pbf.predict", "pbf.sort",     "pbf.grid",
+                                                "pbf.lambda",  "pbf.delta",    "pbf.velocity",
+                                                "pbf.xsph",    "pbf.step"};
 
 std::string vformat(const char* fmt, va_list ap) {
   char buf[512];
@@ -393,10 +424,12 @@ int step_consts(pbf_solver* s, pbf::StepConsts& c) {
 void enqueue_step(pbf_solver* s, const StepArrays& a, const pbf::StepConsts& c, hipStream_t st,
                   bool timed, Record* rec = nullptr) {
   auto tb = [&](int ph) {
+    mark_push(kPhaseName[ph]);
     if (timed) tbegin(s, ph, st);
   };
   auto te = [&](int ph) {
     if (timed) tend(s, ph, st);
+    mark_pop();
   };
   const int L = s->layout;
   const int n = a.n;
@@ -1308,6 +1341,10 @@ int pbf_tape_backward(pbf_solver* s, pbf_tape* t, const float* gpos, const float
              : hipMemsetAsync(t->gv[0], 0, vb, st);
   if (e != hipSuccess) return fail_hip(s, e, "seeding the backward pass");
   pbf::launch_fill_f64(t->r0, n, st);
+  mark_push("pbf.tape_backward");
+  struct PopOnExit {
+    ~PopOnExit() { mark_pop(); }
+  } pop_on_exit;
   std::vector<float4*> xl(t->xl), sd(t->sd), dp(t->dp);
   std::vector<float*> rho(t->rho);
   Record rec;

@@ -12,8 +12,10 @@ inputs resident in HBM, bracketed by torch.cuda.synchronize(), timed with
 HIP events on the launch stream.  The launch shapes come from the library's
 autotuner run on the warmed-up state; the timed steps replay a recorded HIP
 graph (the eager figure is reported beside it, from a second simulation
-over the same frames).  Also measured: the 64-scene batched rate.  One JSON
-line on stdout.
+over the same frames).  Also measured: the 64-scene batched rate, the
+backward/forward ratio of the differentiable rollout, and the step time of
+one GPU's share of an 8-way x-slab split (slab plus one-cell halo, stepped
+as its own scene).  One JSON line on stdout.
 """
 import argparse
 import json
@@ -44,6 +46,8 @@ def parse_args():
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batched rate")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the backward/forward ratio")
+    ap.add_argument("--no-slab", action="store_true",
+                    help="skip the one-slab share of an 8-way x-slab split")
     ap.add_argument("--sweep", action="store_true",
                     help="also run the 4,194,304-particle sweeping-wall workload: 120 frames, "
                          "6 iterations, backward over the last 10 frames")
@@ -227,6 +231,63 @@ def adjoint_ratio(pbf, p, dev, stream, reps=3):
                         "recomputed"}
 
 
+def slab_share(pbf, p, pos, vel, iid, ulim, llim, shapes, dev, stream, parts=8, warmup=3,
+               steps=20):
+    """One GPU's share of an x-slab decomposition, stepped at its real shape:
+    the particles are split into `parts` slabs of whole h-cells in x with
+    balanced particle counts, and the slab with the most particles plus a
+    one-cell halo on each side is stepped as a scene of its own (box = the
+    slab and halo cells).  The halo exchange, migration and the slab walls'
+    physics are not modelled; this times the kernels on one share."""
+    import numpy as np
+    import torch
+
+    h = p.h
+    n = pos.shape[0]
+    dims_x = int(np.ceil((ulim[0] - llim[0]) / h))
+    cx = np.clip(((pos[:, 0] - llim[0]) / h).astype(np.int64), 0, dims_x - 1)
+    per_cell = np.bincount(cx, minlength=dims_x)
+    cum = np.cumsum(per_cell)
+    cuts = [0] + [int(np.searchsorted(cum, k * n / parts)) + 1 for k in range(1, parts)] + [dims_x]
+    best = None
+    for a, b in zip(cuts[:-1], cuts[1:]):
+        own = int(per_cell[a:b].sum())
+        if best is None or own > best[2]:
+            best = (a, b, own)
+    a, b, own = best
+    lo, hi = max(a - 1, 0), min(b + 1, dims_x)
+    sel = (cx >= lo) & (cx < hi)
+    m = int(sel.sum())
+    sl_ll = (llim[0] + lo * h, llim[1], llim[2])
+    sl_ul = (llim[0] + hi * h, ulim[1], ulim[2])
+    P = np.ascontiguousarray(pos[sel], dtype=np.float32)
+    V = np.ascontiguousarray(vel[sel], dtype=np.float32)
+    I = np.arange(m, dtype=np.int32)
+    sim = pbf.Simulator(p, sl_ul, sl_ll, capacity=m)
+    sim.set_shapes(*shapes)
+    sim.set_graph(True)
+    bufs = [[torch.from_numpy(P.ravel()).to(dev), torch.from_numpy(V.ravel()).to(dev)],
+            [torch.empty(m * 4, device=dev), torch.empty(m * 4, device=dev)]]
+    ids = torch.from_numpy(I).to(dev)
+
+    class _Run:
+        f = 0
+
+        def step(self, st):
+            x, y = bufs[self.f % 2], bufs[(self.f + 1) % 2]
+            sim.step(x[0], y[0], x[1], y[1], ids, m, st)
+            self.f += 1
+
+    run = _Run()
+    with torch.cuda.stream(stream):
+        for _ in range(warmup):
+            run.step(stream)
+        ms = timed(run, steps, stream)
+    return {"parts": parts, "x_cells": [a, b], "slab_particles": own, "halo_particles": m - own,
+            "ms_per_step": ms, "rate": own * p.niter / (ms * 1e-3), "niter": p.niter,
+            "mode": "hip graph replay", "unit": "particle-solver-iters/s (slab particles)"}
+
+
 def sweep_workload(pbf, dev, stream, frames=120, back=10):
     """The sweeping-wall workload: 4,194,304 particles, the +x wall moving as
     160h + 20h sin(2 pi f / 120), dam constants with 6 iterations.  Frames
@@ -387,6 +448,24 @@ def main():
         print(f"batched: {ms_b:.4f} ms/step, {rate:.4g} /s", file=sys.stderr, flush=True)
         del br
 
+    # one GPU's share of an 8-way x-slab split of the single dam (its state
+    # after the timed frames) and of the 4M sweep dam (its initial state)
+    slab = None
+    if not args.no_slab:
+        cur = eager.current()
+        P4 = cur[0].cpu().numpy().reshape(n, 4)
+        V4 = cur[1].cpu().numpy().reshape(n, 4)
+        slab = {"single_dam_1m": slab_share(pbf, p, P4, V4, None, sc.ulim, sc.llim, shapes, dev,
+                                            stream)}
+        sd = scenes.sweep_dam()
+        p6 = pbf.dam_params(niter=6)
+        sp, sv = sd.as_layout(4)
+        slab["sweep_dam_4m"] = slab_share(pbf, p6, sp, sv, None, sd.ulim, sd.llim, shapes, dev,
+                                          stream)
+        print(f"slab: 1M share {slab['single_dam_1m']['ms_per_step']:.3f} ms/step, "
+              f"4M share {slab['sweep_dam_4m']['ms_per_step']:.3f} ms/step", file=sys.stderr,
+              flush=True)
+
     adjoint = None
     if not args.no_adjoint:
         adjoint = adjoint_ratio(pbf, p, dev, stream)
@@ -436,6 +515,7 @@ def main():
         "phase_ms": phase_ms,
         "batched_rate": batched,
         "backward_forward_ratio": adjoint,
+        "slab_share": slab,
     }
     if sweep is not None:
         out["sweep_workload"] = sweep

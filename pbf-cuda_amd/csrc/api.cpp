@@ -438,23 +438,25 @@ void enqueue_step(pbf_solver* s, const StepArrays& a, const pbf::StepConsts& c, 
   tb(PBF_PHASE_STEP);
   // advect() + getGridId (Simulator.cpp:50, Simulator.cu:165-193)
   tb(PBF_PHASE_PREDICT);
-  pbf::launch_predict_key(L, a.pos, a.vel, n, c, scn, s->xp, s->key0, s->val0, s->cnt,
-                          s->counters, st);
+  pbf::launch_predict_key(L, a.pos, a.vel, n, c, scn, s->xp, s->key0, s->val0, s->counters, st);
   te(PBF_PHASE_PREDICT);
   // thrust::sort_by_key (Simulator.cu:196-198)
   tb(PBF_PHASE_SORT);
   const int which = pbf::radix_sort_pairs(s->key0, s->val0, s->key1, s->val1, n,
                                           key_bits_for(c.cells), s->hist, s->totals, st);
   te(PBF_PHASE_SORT);
+  const uint32_t* skey = which ? s->key1 : s->key0;
   const uint32_t* sval = which ? s->val1 : s->val0;
   if (rec) {
-    rec->skey = which ? s->key1 : s->key0;
+    rec->skey = skey;
     rec->sval = sval;
   }
-  // memset + computeGridRange (Simulator.cu:200-204), fused with the payload gather
+  // memset + computeGridRange (Simulator.cu:200-204): the payload gather
+  // counts the particles per cell, the scan turns the counts into cell starts
   tb(PBF_PHASE_GRID);
+  pbf::launch_permute(L, skey, sval, a.pos, s->xp, a.iid, n, s->x0s, s->xa, s->iid_s, s->cnt,
+                      st);
   pbf::launch_cell_starts(s->cnt, c.cells, s->tile_sum, s->cs, st);
-  pbf::launch_permute(L, sval, a.pos, s->xp, a.iid, n, s->x0s, s->xa, s->iid_s, st);
   te(PBF_PHASE_GRID);
   // niter x correctDensity (Simulator.cpp:62-66, Simulator.cu:213-249);
   // xl (x, lambda) aliases xp, which is dead after the permute (a tape keeps
@@ -1082,12 +1084,12 @@ int pbf_density_error(pbf_solver* s, const float* pos, int n, void* stream, doub
   const pbf::Scenes scn = scenes_of(s);
   if (s->cnt_dirty) pbf::launch_fill_u32(s->cnt, s->max_cells, 0u, st);
   s->cnt_dirty = true;
-  pbf::launch_key_only(s->layout, pos, n, c, scn, s->xp, s->key0, s->val0, s->cnt, s->counters,
-                       st);
+  pbf::launch_key_only(s->layout, pos, n, c, scn, s->xp, s->key0, s->val0, s->counters, st);
   const int which = pbf::radix_sort_pairs(s->key0, s->val0, s->key1, s->val1, n,
                                           key_bits_for(c.cells), s->hist, s->totals, st);
+  pbf::launch_permute_pos(which ? s->key1 : s->key0, which ? s->val1 : s->val0, s->xp, n, s->xa,
+                          s->cnt, st);
   pbf::launch_cell_starts(s->cnt, c.cells, s->tile_sum, s->cs, st);
-  pbf::launch_permute_pos(which ? s->val1 : s->val0, s->xp, n, s->xa, st);
   pbf::launch_density_error(s->xa, s->cs, n, c, scn, s->dens_err, s->dens_partial, s->dens_out,
                             s->hl, s->shapes, st);
   hipError_t e = hipGetLastError();

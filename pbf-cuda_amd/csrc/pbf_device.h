@@ -252,7 +252,7 @@ struct SeqCursor1 {
 // Visits sequence positions [u0, u1) of a candidate sequence of m positions
 // in order,
 // calling add() only for candidates passing the cheap test hit().  Blocks of
-// 32 positions are screened eight loads at a time (across column
+// 64 positions are screened eight loads at a time (across column
 // boundaries, so a sparse neighbourhood costs a few memory round trips, not
 // one or two per column), then the hits are reloaded four at a time and
 // accumulated in ascending order.  Candidates failing hit() contribute
@@ -271,9 +271,9 @@ __device__ __forceinline__ void sweep_seq(const SeqTable& st, uint32_t m, uint32
   SeqCursor1 c2;
   c2.from(c1);
 #pragma unroll 1
-  for (uint32_t bb = u0; bb < u1; bb += 32) {
-    const uint32_t be = u1 - bb < 32u ? u1 : bb + 32u;
-    uint32_t mask = 0;
+  for (uint32_t bb = u0; bb < u1; bb += 64) {
+    const uint32_t be = u1 - bb < 64u ? u1 : bb + 64u;
+    uint64_t mask = 0;
 #pragma unroll 1
     for (uint32_t u = bb; u < be; u += 8) {
       uint32_t j[8];
@@ -285,7 +285,7 @@ __device__ __forceinline__ void sweep_seq(const SeqTable& st, uint32_t m, uint32
 #pragma unroll
       for (int t = 0; t < 8; ++t) bits |= (hit(j[t], d[t]) ? 1u : 0u) << t;
       const uint32_t valid = be - u >= 8u ? 0xFFu : (1u << (be - u)) - 1u;
-      mask |= (bits & valid) << (u - bb);
+      mask |= (uint64_t)(bits & valid) << (u - bb);
     }
     uint32_t last = 0;
 #pragma unroll 1
@@ -296,7 +296,7 @@ __device__ __forceinline__ void sweep_seq(const SeqTable& st, uint32_t m, uint32
       for (int t = 0; t < 4; ++t) {
         h[t] = mask != 0u;
         // past the last hit, repeat it (the cursor only moves forward)
-        const uint32_t bit = h[t] ? (uint32_t)__builtin_ctz(mask) : last;
+        const uint32_t bit = h[t] ? (uint32_t)__builtin_ctzll(mask) : last;
         last = bit;
         mask &= mask - 1u;
         j[t] = c2.slot(bb + bit);

@@ -38,9 +38,9 @@ int radix_sort_pairs(uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, int
 // the parity-0 counters, each light kernel clears the other parity's).
 void launch_predict_key(int layout, const float* pos, const float* vel, int n, const StepConsts& c,
                         const Scenes& scn, float4* xp, uint32_t* key, uint32_t* val,
-                        uint32_t* cnt, uint32_t* counters, hipStream_t st);
+                        uint32_t* counters, hipStream_t st);
 void launch_key_only(int layout, const float* pos, int n, const StepConsts& c, const Scenes& scn,
-                     float4* xp, uint32_t* key, uint32_t* val, uint32_t* cnt, uint32_t* counters,
+                     float4* xp, uint32_t* key, uint32_t* val, uint32_t* counters,
                      hipStream_t st);
 void launch_fill_u32(uint32_t* a, long long n, uint32_t v, hipStream_t st);
 // Cell-start table from the per-cell counts (which it clears): cs holds
@@ -48,11 +48,14 @@ void launch_fill_u32(uint32_t* a, long long n, uint32_t v, hipStream_t st);
 long long scan_tiles(long long cells);
 void launch_cell_starts(uint32_t* cnt, long long cells, uint32_t* tile_sum, uint32_t* cs,
                         hipStream_t st);
-void launch_permute(int layout, const uint32_t* sval, const float* pos, const float4* xp,
-                    const uint32_t* iid, int n, float4* x0s, float4* xa, uint32_t* iid_s,
-                    hipStream_t st);
-void launch_permute_pos(const uint32_t* sval, const float4* xp, int n, float4* xa,
-                        hipStream_t st);
+// The payload gathers after the sort also count the particles per cell from
+// the sorted keys skey into cnt (all zero beforehand), the input of
+// launch_cell_starts.
+void launch_permute(int layout, const uint32_t* skey, const uint32_t* sval, const float* pos,
+                    const float4* xp, const uint32_t* iid, int n, float4* x0s, float4* xa,
+                    uint32_t* iid_s, uint32_t* cnt, hipStream_t st);
+void launch_permute_pos(const uint32_t* skey, const uint32_t* sval, const float4* xp, int n,
+                        float4* xa, uint32_t* cnt, hipStream_t st);
 // rho (may be null) receives the densities; sd and dp (null outside a tape)
 // receive (sum of lambda gradients, denominator) and the correction / pho0
 // before the clamps, per particle.

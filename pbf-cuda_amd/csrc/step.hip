@@ -286,12 +286,18 @@ struct DensityPass {
 // Streaming kernels.
 // ---------------------------------------------------------------------------
 
-// cnt[k] += 1 for every valid lane, with one atomic per run of equal keys
-// among consecutive lanes (particles arrive in the previous frame's cell
-// order, so runs are common and piled cells would otherwise serialise
-// thousands of atomics on one address).  All lanes of the wave call it.
-__device__ __forceinline__ void count_cell(uint32_t k, bool valid, uint32_t* __restrict__ cnt) {
+// Particles per cell, counted from the sorted keys (the input of the
+// cell-start scan).  Equal keys are contiguous after the sort, so a run that
+// lies inside one wavefront is that cell's only run and its first lane
+// stores the count; a run touching a wavefront boundary (at most two per
+// wavefront) may continue in the neighbouring wavefront, and each of its
+// parts is added atomically.  The counts must be zero beforehand (the scan
+// re-zeroes them).
+__device__ __forceinline__ void count_sorted(const uint32_t* __restrict__ skey, int s, int n,
+                                             uint32_t* __restrict__ cnt) {
   const int lane = threadIdx.x & 63;
+  const bool valid = s < n;
+  const uint32_t k = valid ? skey[s] : 0u;
   const uint32_t prev = __shfl_up(k, 1);
   const uint64_t vm = __ballot(valid);
   const bool head = valid && (lane == 0 || prev != k);
@@ -300,7 +306,14 @@ __device__ __forceinline__ void count_cell(uint32_t k, bool valid, uint32_t* __r
     // run = [lane, next head or end of the valid lanes)
     const uint64_t later = lane == 63 ? 0ull : (hm >> (lane + 1)) << (lane + 1);
     const int end = later ? __ffsll((unsigned long long)later) - 1 : 64 - __clzll(vm);
-    atomicAdd(&cnt[k], (uint32_t)(end - lane));
+    const int len = end - lane;
+    const int after = s + len;  // sorted index just past the run
+    const bool open_left = lane == 0 && s > 0 && skey[s - 1] == k;
+    const bool open_right = !later && after < n && skey[after] == k;
+    if (open_left || open_right)
+      atomicAdd(&cnt[k], (uint32_t)len);
+    else
+      cnt[k] = (uint32_t)len;
   }
 }
 
@@ -308,8 +321,7 @@ __device__ __forceinline__ void count_cell(uint32_t k, bool valid, uint32_t* __r
 // (fluids/Simulator.cu:56-73).  vel += dt*g and npos = pos + dt*vel as fused
 // multiply-adds; the updated velocity itself is not stored, since the
 // reference overwrites it before any read (fluids/Simulator.cu:267-274).
-// Also counts particles per cell (the input of the cell-start scan) and
-// clears the heavy counters of the first neighbour pass.
+// Also clears the heavy counters of the first neighbour pass.
 template <int L, class S>
 __global__ __launch_bounds__(kBlock) void k_predict_key(const float* __restrict__ pos,
                                                         const float* __restrict__ vel, int n,
@@ -317,7 +329,6 @@ __global__ __launch_bounds__(kBlock) void k_predict_key(const float* __restrict_
                                                         float4* __restrict__ xp,
                                                         uint32_t* __restrict__ key,
                                                         uint32_t* __restrict__ val,
-                                                        uint32_t* __restrict__ cnt,
                                                         uint32_t* __restrict__ counters) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i < 4) counters[i] = 0u;
@@ -333,7 +344,6 @@ __global__ __launch_bounds__(kBlock) void k_predict_key(const float* __restrict_
   const float z = fmaf(c.dt, vz, p.z);
   const SceneGeo g = sc.at((uint32_t)ii, c);
   const uint32_t k = cell_id(cell_of(x, y, z, g, c.h), g);
-  count_cell(k, i < n, cnt);
   if (i >= n) return;
   xp[i] = make_float4(x, y, z, 0.0f);
   key[i] = k;
@@ -346,7 +356,6 @@ __global__ __launch_bounds__(kBlock) void k_key_only(const float* __restrict__ p
                                                      StepConsts c, S sc, float4* __restrict__ xp,
                                                      uint32_t* __restrict__ key,
                                                      uint32_t* __restrict__ val,
-                                                     uint32_t* __restrict__ cnt,
                                                      uint32_t* __restrict__ counters) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i < 4) counters[i] = 0u;
@@ -355,7 +364,6 @@ __global__ __launch_bounds__(kBlock) void k_key_only(const float* __restrict__ p
   const float4 p = load_vec<L>(pos, ii);
   const SceneGeo g = sc.at((uint32_t)ii, c);
   const uint32_t k = cell_id(cell_of(p.x, p.y, p.z, g, c.h), g);
-  count_cell(k, i < n, cnt);
   if (i >= n) return;
   xp[i] = make_float4(p.x, p.y, p.z, 0.0f);
   key[i] = k;
@@ -467,14 +475,19 @@ __global__ __launch_bounds__(kBlock) void k_scan_cells(uint32_t* __restrict__ cn
 }
 
 // Gather of the sorted payload (the zip permutation of
-// fluids/Simulator.cu:196-198).  Indices < n hold for any sorted index
-// array; the guard keeps a corrupted one from reaching out of bounds.
+// fluids/Simulator.cu:196-198), with the per-cell counts of the sorted keys
+// (computeGridRange's input, fluids/Simulator.cu:201-204).  Indices < n
+// hold for any sorted index array; the guard keeps a corrupted one from
+// reaching out of bounds.
 template <int L>
 __global__ __launch_bounds__(kBlock) void k_permute(
-    const uint32_t* __restrict__ sval, const float* __restrict__ pos,
-    const float4* __restrict__ xp, const uint32_t* __restrict__ iid, int n,
-    float4* __restrict__ x0s, float4* __restrict__ xa, uint32_t* __restrict__ iid_s) {
+    const uint32_t* __restrict__ skey, const uint32_t* __restrict__ sval,
+    const float* __restrict__ pos, const float4* __restrict__ xp,
+    const uint32_t* __restrict__ iid, int n, float4* __restrict__ x0s,
+    float4* __restrict__ xa, uint32_t* __restrict__ iid_s, uint32_t* __restrict__ cnt) {
   const int s = blockIdx.x * kBlock + threadIdx.x;
+  if (blockIdx.x * kBlock >= n) return;  // whole workgroup past the end
+  count_sorted(skey, s, n, cnt);
   if (s >= n) return;
   const uint32_t src = sval[s] < (uint32_t)n ? sval[s] : 0u;
   x0s[s] = load_vec<L>(pos, src);
@@ -482,10 +495,14 @@ __global__ __launch_bounds__(kBlock) void k_permute(
   iid_s[s] = iid[src];
 }
 
-__global__ __launch_bounds__(kBlock) void k_permute_pos(const uint32_t* __restrict__ sval,
+__global__ __launch_bounds__(kBlock) void k_permute_pos(const uint32_t* __restrict__ skey,
+                                                        const uint32_t* __restrict__ sval,
                                                         const float4* __restrict__ xp, int n,
-                                                        float4* __restrict__ xa) {
+                                                        float4* __restrict__ xa,
+                                                        uint32_t* __restrict__ cnt) {
   const int s = blockIdx.x * kBlock + threadIdx.x;
+  if (blockIdx.x * kBlock >= n) return;
+  count_sorted(skey, s, n, cnt);
   if (s >= n) return;
   xa[s] = xp[sval[s] < (uint32_t)n ? sval[s] : 0u];
 }
@@ -577,49 +594,49 @@ __global__ __launch_bounds__(kRedBlock) void k_reduce_partials(const double* __r
 
 template <int L, class S>
 void predict_key_s(const float* pos, const float* vel, int n, const StepConsts& c, const S& sc,
-                   float4* xp, uint32_t* key, uint32_t* val, uint32_t* cnt, uint32_t* counters,
+                   float4* xp, uint32_t* key, uint32_t* val, uint32_t* counters,
                    hipStream_t st) {
-  k_predict_key<L, S><<<blocks_for(n), kBlock, 0, st>>>(pos, vel, n, c, sc, xp, key, val, cnt,
+  k_predict_key<L, S><<<blocks_for(n), kBlock, 0, st>>>(pos, vel, n, c, sc, xp, key, val,
                                                         counters);
 }
 
 template <int L, class S>
 void key_only_s(const float* pos, int n, const StepConsts& c, const S& sc, float4* xp,
-                uint32_t* key, uint32_t* val, uint32_t* cnt, uint32_t* counters, hipStream_t st) {
-  k_key_only<L, S><<<blocks_for(n), kBlock, 0, st>>>(pos, n, c, sc, xp, key, val, cnt, counters);
+                uint32_t* key, uint32_t* val, uint32_t* counters, hipStream_t st) {
+  k_key_only<L, S><<<blocks_for(n), kBlock, 0, st>>>(pos, n, c, sc, xp, key, val, counters);
 }
 
 void launch_predict_key(int layout, const float* pos, const float* vel, int n, const StepConsts& c,
                         const Scenes& scn, float4* xp, uint32_t* key, uint32_t* val,
-                        uint32_t* cnt, uint32_t* counters, hipStream_t st) {
+                        uint32_t* counters, hipStream_t st) {
   const SceneTable tab{scn.geo, scn.sid};
   if (layout == 3) {
     if (scn.geo)
-      predict_key_s<3>(pos, vel, n, c, tab, xp, key, val, cnt, counters, st);
+      predict_key_s<3>(pos, vel, n, c, tab, xp, key, val, counters, st);
     else
-      predict_key_s<3>(pos, vel, n, c, OneScene{}, xp, key, val, cnt, counters, st);
+      predict_key_s<3>(pos, vel, n, c, OneScene{}, xp, key, val, counters, st);
   } else {
     if (scn.geo)
-      predict_key_s<4>(pos, vel, n, c, tab, xp, key, val, cnt, counters, st);
+      predict_key_s<4>(pos, vel, n, c, tab, xp, key, val, counters, st);
     else
-      predict_key_s<4>(pos, vel, n, c, OneScene{}, xp, key, val, cnt, counters, st);
+      predict_key_s<4>(pos, vel, n, c, OneScene{}, xp, key, val, counters, st);
   }
 }
 
 void launch_key_only(int layout, const float* pos, int n, const StepConsts& c, const Scenes& scn,
-                     float4* xp, uint32_t* key, uint32_t* val, uint32_t* cnt, uint32_t* counters,
+                     float4* xp, uint32_t* key, uint32_t* val, uint32_t* counters,
                      hipStream_t st) {
   const SceneTable tab{scn.geo, scn.sid};
   if (layout == 3) {
     if (scn.geo)
-      key_only_s<3>(pos, n, c, tab, xp, key, val, cnt, counters, st);
+      key_only_s<3>(pos, n, c, tab, xp, key, val, counters, st);
     else
-      key_only_s<3>(pos, n, c, OneScene{}, xp, key, val, cnt, counters, st);
+      key_only_s<3>(pos, n, c, OneScene{}, xp, key, val, counters, st);
   } else {
     if (scn.geo)
-      key_only_s<4>(pos, n, c, tab, xp, key, val, cnt, counters, st);
+      key_only_s<4>(pos, n, c, tab, xp, key, val, counters, st);
     else
-      key_only_s<4>(pos, n, c, OneScene{}, xp, key, val, cnt, counters, st);
+      key_only_s<4>(pos, n, c, OneScene{}, xp, key, val, counters, st);
   }
 }
 
@@ -640,18 +657,20 @@ void launch_cell_starts(uint32_t* cnt, long long cells, uint32_t* tile_sum, uint
   k_scan_cells<<<ntiles, kBlock, 0, st>>>(cnt, cells, tile_sum, cs);
 }
 
-void launch_permute(int layout, const uint32_t* sval, const float* pos, const float4* xp,
-                    const uint32_t* iid, int n, float4* x0s, float4* xa, uint32_t* iid_s,
-                    hipStream_t st) {
+void launch_permute(int layout, const uint32_t* skey, const uint32_t* sval, const float* pos,
+                    const float4* xp, const uint32_t* iid, int n, float4* x0s, float4* xa,
+                    uint32_t* iid_s, uint32_t* cnt, hipStream_t st) {
   if (layout == 3)
-    k_permute<3><<<blocks_for(n), kBlock, 0, st>>>(sval, pos, xp, iid, n, x0s, xa, iid_s);
+    k_permute<3><<<blocks_for(n), kBlock, 0, st>>>(skey, sval, pos, xp, iid, n, x0s, xa, iid_s,
+                                                   cnt);
   else
-    k_permute<4><<<blocks_for(n), kBlock, 0, st>>>(sval, pos, xp, iid, n, x0s, xa, iid_s);
+    k_permute<4><<<blocks_for(n), kBlock, 0, st>>>(skey, sval, pos, xp, iid, n, x0s, xa, iid_s,
+                                                   cnt);
 }
 
-void launch_permute_pos(const uint32_t* sval, const float4* xp, int n, float4* xa,
-                        hipStream_t st) {
-  k_permute_pos<<<blocks_for(n), kBlock, 0, st>>>(sval, xp, n, xa);
+void launch_permute_pos(const uint32_t* skey, const uint32_t* sval, const float4* xp, int n,
+                        float4* xa, uint32_t* cnt, hipStream_t st) {
+  k_permute_pos<<<blocks_for(n), kBlock, 0, st>>>(skey, sval, xp, n, xa, cnt);
 }
 
 void launch_lambda(const float4* xa, const uint32_t* cs, int n, const StepConsts& c,

@@ -158,11 +158,14 @@ def timed(run, steps, stream):
     return e0.elapsed_time(e1) / steps
 
 
-def adjoint_ratio(pbf, p, dev, stream, reps=3):
+def adjoint_ratio(pbf, p, dev, stream, reps=3, store=True):
     """Backward / forward time of the 30-frame differentiable rollout of the
     16,384-particle scene (dam constants, 4 iterations): forward = the tape's
-    checkpointed 30 frames, backward = dL/d(pos0, vel0, pho0) through the
-    30 frames with each frame's iterations recomputed from its checkpoint."""
+    30 frames, backward = dL/d(pos0, vel0, pho0) through the 30 frames.
+    store=True: the forward run keeps every frame's iterations, sort and grid
+    on the device and the backward pass reads them; store=False: it keeps
+    only per-frame checkpoints and the backward pass recomputes each frame's
+    iterations."""
     import numpy as np
     import torch
 
@@ -172,7 +175,7 @@ def adjoint_ratio(pbf, p, dev, stream, reps=3):
     n, nf = sc.n, 30
     tgt = torch.from_numpy(scenes.target_cloud(sc)).to(dev)
     sim = pbf.Simulator(p, sc.ulim, sc.llim, capacity=n)
-    tape = pbf.Tape(sim, n, nf)
+    tape = pbf.Tape(sim, n, nf, store=store)
     pos, vel = sc.as_layout(4)
     pos0 = torch.from_numpy(pos.ravel()).to(dev)
     vel0 = torch.from_numpy(vel.ravel()).to(dev)
@@ -221,6 +224,7 @@ def adjoint_ratio(pbf, p, dev, stream, reps=3):
         t_p = time_it(plain)
     gv = gv0.view(n, 4)[:, :3]
     return {"value": t_b / t_f, "unit": "backward time / forward time",
+            "tape": "stored iterations" if store else "checkpoints, iterations recomputed",
             "forward_ms": t_f, "backward_ms": t_b, "plain_forward_ms": t_p,
             "backward_over_plain_forward": t_b / t_p,
             "gradients_finite": bool(torch.isfinite(gv).all().item()) and bool(np.isfinite(grho)),
@@ -292,8 +296,9 @@ def sweep_workload(pbf, dev, stream, frames=120, back=10):
     """The sweeping-wall workload: 4,194,304 particles, the +x wall moving as
     160h + 20h sin(2 pi f / 120), dam constants with 6 iterations.  Frames
     0 .. frames-back-1 run plainly (timed), the last `back` frames run on a
-    checkpointing tape, then dL/d(state at frame frames-back) is computed
-    through them (timed), L = mean squared distance to a seeded cloud."""
+    tape (timed), then dL/d(state at frame frames-back) is computed through
+    them (timed), L = mean squared distance to a seeded cloud; once with a
+    tape that stores the iterations and once with one that recomputes them."""
     import numpy as np
     import torch
 
@@ -312,48 +317,50 @@ def sweep_workload(pbf, dev, stream, frames=120, back=10):
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     plain = frames - back
+
+    def span(fn):
+        torch.cuda.synchronize()
+        e0.record(stream)
+        r = fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1), r
+
+    out = {"particles": n, "frames": frames, "niter": p.niter}
     with torch.cuda.stream(stream):
-        torch.cuda.synchronize()
-        e0.record(stream)
-        which = sim.run_frames(a[0], a[1], b[0], b[1], iid, n, 0, plain, sw, stream)
-        e1.record(stream)
-        torch.cuda.synchronize()
-        ms_plain = e0.elapsed_time(e1) / plain
+        ms, which = span(lambda: sim.run_frames(a[0], a[1], b[0], b[1], iid, n, 0, plain, sw,
+                                                stream))
+        out["plain_ms_per_frame"] = ms / plain
+        out["plain_rate"] = n * p.niter / (ms / plain * 1e-3)
         cur = b if which else a
-        free0 = torch.cuda.mem_get_info()[0]
-        tape = pbf.Tape(sim, n, back)
-        tape_bytes = free0 - torch.cuda.mem_get_info()[0]
-        torch.cuda.synchronize()
-        e0.record(stream)
-        tape.forward(cur[0], cur[1], iid, stream, first_frame=plain, sweep=sw)
-        e1.record(stream)
-        torch.cuda.synchronize()
-        ms_tape = e0.elapsed_time(e1)
-        xf = torch.empty(n * 4, device=dev)
-        iidf = torch.empty(n, dtype=torch.int32, device=dev)
-        tape.read(back, pos=xf, iid=iidf, stream=stream)
-        g = torch.zeros(n, 4, device=dev)
-        g[:, :3] = (2.0 * (xf.view(n, 4)[:, :3].double() - tgt[iidf.long()]) / n).float()
-        gp0 = torch.empty(n * 4, device=dev)
-        gv0 = torch.empty(n * 4, device=dev)
-        torch.cuda.synchronize()
-        e0.record(stream)
-        grho = tape.backward(g.view(-1), None, gp0, gv0, stream)
-        e1.record(stream)
-        torch.cuda.synchronize()
-        ms_back = e0.elapsed_time(e1)
-    gv = gv0.view(n, 4)[:, :3]
-    return {"particles": n, "frames": frames, "niter": p.niter,
-            "plain_ms_per_frame": ms_plain,
-            "plain_rate": n * p.niter / (ms_plain * 1e-3),
-            "tape_forward_ms": ms_tape, "backward_ms": ms_back,
-            "backward_forward_ratio": ms_back / ms_tape,
-            "tape_device_bytes": int(tape_bytes),
-            "gradients_finite": bool(torch.isfinite(gv).all().item()) and bool(np.isfinite(grho)),
-            "workload": "sweep_dam_4m: 256x64x256 jittered lattice in a 160h x 64h x 160h box, "
-                        "+x wall 160h + 20h sin(2 pi f/120), dam constants, 6 iterations; "
-                        f"frames 0..{plain - 1} plain, {plain}..{frames - 1} on a tape, "
-                        "backward through the last 10"}
+        for store in (True, False):
+            free0 = torch.cuda.mem_get_info()[0]
+            tape = pbf.Tape(sim, n, back, store=store)
+            tape_bytes = free0 - torch.cuda.mem_get_info()[0]
+            ms_f, _ = span(lambda: tape.forward(cur[0], cur[1], iid, stream, first_frame=plain,
+                                                sweep=sw))
+            xf = torch.empty(n * 4, device=dev)
+            iidf = torch.empty(n, dtype=torch.int32, device=dev)
+            tape.read(back, pos=xf, iid=iidf, stream=stream)
+            g = torch.zeros(n, 4, device=dev)
+            g[:, :3] = (2.0 * (xf.view(n, 4)[:, :3].double() - tgt[iidf.long()]) / n).float()
+            gp0 = torch.empty(n * 4, device=dev)
+            gv0 = torch.empty(n * 4, device=dev)
+            ms_b, grho = span(lambda: tape.backward(g.view(-1), None, gp0, gv0, stream))
+            gv = gv0.view(n, 4)[:, :3]
+            key = "stored" if store else "recompute"
+            out[key] = {"tape_forward_ms": ms_f, "backward_ms": ms_b,
+                        "backward_forward_ratio": ms_b / ms_f, "tape_device_bytes": int(tape_bytes),
+                        "gradients_finite": bool(torch.isfinite(gv).all().item())
+                        and bool(np.isfinite(grho))}
+            tape.close()
+            del tape, xf, iidf, g, gp0, gv0
+    out["backward_forward_ratio"] = out["stored"]["backward_forward_ratio"]
+    out["workload"] = ("sweep_dam_4m: 256x64x256 jittered lattice in a 160h x 64h x 160h box, "
+                       "+x wall 160h + 20h sin(2 pi f/120), dam constants, 6 iterations; "
+                       f"frames 0..{plain - 1} plain, {plain}..{frames - 1} on a tape, "
+                       f"backward through the last {back}")
+    return out
 
 
 def main():
@@ -468,16 +475,21 @@ def main():
 
     adjoint = None
     if not args.no_adjoint:
-        adjoint = adjoint_ratio(pbf, p, dev, stream)
+        adjoint = adjoint_ratio(pbf, p, dev, stream, store=True)
+        rec = adjoint_ratio(pbf, p, dev, stream, store=False)
+        adjoint["recompute"] = {k: rec[k] for k in ("value", "forward_ms", "backward_ms", "tape")}
         print(f"adjoint: forward {adjoint['forward_ms']:.2f} ms, backward "
-              f"{adjoint['backward_ms']:.2f} ms", file=sys.stderr, flush=True)
+              f"{adjoint['backward_ms']:.2f} ms (recomputing tape: {rec['forward_ms']:.2f} / "
+              f"{rec['backward_ms']:.2f} ms)", file=sys.stderr, flush=True)
 
     sweep = None
     if args.sweep:
         sweep = sweep_workload(pbf, dev, stream)
-        print(f"sweep: {sweep['plain_ms_per_frame']:.2f} ms/frame, backward "
-              f"{sweep['backward_ms']:.1f} ms / tape forward {sweep['tape_forward_ms']:.1f} ms",
-              file=sys.stderr, flush=True)
+        st_, rc_ = sweep["stored"], sweep["recompute"]
+        print(f"sweep: {sweep['plain_ms_per_frame']:.2f} ms/frame; stored tape backward "
+              f"{st_['backward_ms']:.1f} ms / forward {st_['tape_forward_ms']:.1f} ms; recomputing "
+              f"tape {rc_['backward_ms']:.1f} / {rc_['tape_forward_ms']:.1f} ms", file=sys.stderr,
+              flush=True)
 
     cpu = None
     if not args.no_cpu:

@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define PBF_ABI_VERSION 2
+#define PBF_ABI_VERSION 3
 
 /* status codes */
 #define PBF_OK 0
@@ -79,7 +79,11 @@ typedef struct pbf_solver pbf_solver;
  * times the initial box cells; here the caller states `capacity` (maximum
  * particles) and `max_cells` (maximum grid cells over the solver's life;
  * <= 0 means 4x the cells of the initial box, as the reference).
- * Allocates device scratch (hipMalloc), so it synchronises.
+ * Allocates device scratch (hipMalloc), so it synchronises: ~(500 B per
+ * particle of capacity + 8 B per cell), most of it the work list of crowded
+ * cells (8 chunk items of 48 B per particle, enough for piles of ~1/3 of a
+ * million particles in one cell; beyond the list a crowded target is summed
+ * by its own work-item, correct but slow).
  */
 int pbf_create(const pbf_params* params, pbf_vec3 ulim, pbf_vec3 llim,
                int capacity, long long max_cells, int layout,
@@ -262,6 +266,11 @@ int pbf_autotune(pbf_solver* s, const float* pos, const float* vel, const uint32
  * pbf_tape_create: tape for n particles and nframes frames under the
  *   solver's current niter (device memory ~ (36 nframes + 52 niter + 200) B
  *   per particle).  Allocates and synchronises.
+ * pbf_tape_create_ex: the same with flags.  PBF_TAPE_STORE: the forward run
+ *   keeps every frame's solver iterations, sort and cell-start table, and
+ *   the backward pass reads them instead of recomputing each frame (device
+ *   memory ~ nframes ((52 niter + 76) B per particle + 4 B per cell of the
+ *   solver's largest box) more; the gradients are the same bits).
  * pbf_tape_forward: runs the frames from (pos, vel, iid) (caller layout,
  *   not modified); the final state is pbf_tape_state(t, nframes).  With a
  *   sweep, frame k runs in the box pbf_sweep_box(sweep, first_frame + k)
@@ -278,7 +287,9 @@ int pbf_autotune(pbf_solver* s, const float* pos, const float* vel, const uint32
  *   Synchronises.  The parameters must be those of the forward run.
  */
 typedef struct pbf_tape pbf_tape;
+#define PBF_TAPE_STORE 1
 int pbf_tape_create(pbf_solver* s, int n, int nframes, pbf_tape** out);
+int pbf_tape_create_ex(pbf_solver* s, int n, int nframes, int flags, pbf_tape** out);
 void pbf_tape_destroy(pbf_tape* t);
 int pbf_tape_forward(pbf_solver* s, pbf_tape* t, const float* pos, const float* vel,
                      const uint32_t* iid, int first_frame, const pbf_sweep* sweep,

@@ -102,8 +102,9 @@ class BatchedSimulator : public Simulator {
 // Checkpointed multi-frame run of a single-scene Simulator and its adjoint.
 class Tape {
  public:
-  Tape(Simulator& sim, int n, int nframes) : sim_(sim) {
-    check(pbf_tape_create(sim.handle(), n, nframes, &t_), sim.handle());
+  // flags: PBF_TAPE_STORE keeps the iterations instead of recomputing them.
+  Tape(Simulator& sim, int n, int nframes, int flags = 0) : sim_(sim) {
+    check(pbf_tape_create_ex(sim.handle(), n, nframes, flags, &t_), sim.handle());
   }
   ~Tape() { pbf_tape_destroy(t_); }
   Tape(const Tape&) = delete;

@@ -53,16 +53,29 @@ __device__ __forceinline__ float3 poly6_grad(const float3& d, float r2, const St
   return make_float3(f * d.x, f * d.y, f * d.z);
 }
 
-// Hessian of k (h - r)^2 d / r (the spiky gradient with coefficient k)
-// applied to a: phi a + psi (d.a) d.
-__device__ __forceinline__ float3 spiky_hess(float k, const float3& d, float r, const float3& a,
-                                             const StepConsts& c) {
+// 1/x to about one ulp (v_rcp_f32 and one Newton step).  The adjoint only
+// has to agree with the FP64 adjoint to a relative 1e-4, so each pair
+// divides once and multiplies by the reciprocal where the forward formulas
+// divide.
+__device__ __forceinline__ float recip(float x) {
+  const float y = __builtin_amdgcn_rcpf(x);
+  return fmaf(fmaf(-x, y, 1.0f), y, y);
+}
+
+// Hessian of k (h - r)^2 d / r (the spiky gradient with coefficient k):
+// phi I + psi d d^T, phi = k u^2 / r, psi = -k u (h + r) / r^3, u = h - r.
+struct SpikyHess {
+  float phi, psi;
+};
+__device__ __forceinline__ SpikyHess spiky_hess(float k, float r, float inv_r,
+                                                const StepConsts& c) {
   const float u = c.h - r;
-  const float phi = k * u * u / r;
-  const float psi = -k * u * (c.h + r) / (r * r * r);
-  const float da = d.x * a.x + d.y * a.y + d.z * a.z;
-  return make_float3(phi * a.x + psi * da * d.x, phi * a.y + psi * da * d.y,
-                     phi * a.z + psi * da * d.z);
+  return SpikyHess{k * u * u * inv_r, -k * u * (c.h + r) * (inv_r * inv_r * inv_r)};
+}
+__device__ __forceinline__ float3 hess_apply(const SpikyHess& H, const float3& d,
+                                             const float3& a) {
+  const float pd = H.psi * (d.x * a.x + d.y * a.y + d.z * a.z);
+  return make_float3(H.phi * a.x + pd * d.x, H.phi * a.y + pd * d.y, H.phi * a.z + pd * d.z);
 }
 
 // d s_corr / d w for s_corr = coef_corr * w^n_corr.
@@ -89,6 +102,8 @@ __device__ __forceinline__ void atomic_add3(float4* p, const float3& v) {
 
 // ---- XSPH -----------------------------------------------------------------
 struct BwdXsphPass {
+  // hits reloaded per round of the sweep
+  static constexpr int kHitBatch = 4;
   struct Args {
     const float4* __restrict__ x;    // final positions
     const float4* __restrict__ vr;   // (v before XSPH, rho of the last lambda pass)
@@ -130,14 +145,16 @@ struct BwdXsphPass {
     const float3 d = make_float3(dx, dy, dz);
     const float W = poly6_of(r2, c);
     const float3 P = poly6_grad(d, r2, c);
-    const float den = t.vr.w + nb.vr.w;
-    const float f = 2.0f * c.c_xsph * W / den;
+    const float inv_den = recip(t.vr.w + nb.vr.w);  // 1 / (rho_i + rho_j)
+    const float k2 = 2.0f * c.c_xsph * inv_den;
+    const float f = k2 * W;
     const float3 dv = make_float3(nb.vr.x - t.vr.x, nb.vr.y - t.vr.y, nb.vr.z - t.vr.z);
+    const float kr = f * inv_den;  // 2 c W / (rho_i + rho_j)^2
     // own term of v'_i
     const float3 B = xyz(t.gv);
     const float u = dot3(B, dv);
-    const float fx = 2.0f * c.c_xsph * u / den;
-    const float fr = 2.0f * c.c_xsph * W * u / (den * den);
+    const float fx = k2 * u;
+    const float fr = kr * u;
     s.vx -= f * B.x;
     s.vy -= f * B.y;
     s.vz -= f * B.z;
@@ -149,14 +166,14 @@ struct BwdXsphPass {
       // j's term about i, gathered here
       const float3 Bj = xyz(nb.gv);
       const float uj = -dot3(Bj, dv);
-      const float gx = 2.0f * c.c_xsph * uj / den;
+      const float gx = k2 * uj;
       s.vx += f * Bj.x;
       s.vy += f * Bj.y;
       s.vz += f * Bj.z;
       s.xx += gx * P.x;
       s.xy += gx * P.y;
       s.xz += gx * P.z;
-      s.r -= 2.0f * c.c_xsph * W * uj / (den * den);
+      s.r -= kr * uj;
     } else if (j != t.i) {
       // j never visits i: deliver the own term's derivatives about j
       atomic_add3(&a.ex_v[j], make_float3(f * B.x, f * B.y, f * B.z));
@@ -182,6 +199,8 @@ struct BwdXsphPass {
 
 // ---- delta-p ----------------------------------------------------------------
 struct BwdDeltaPass {
+  // hits reloaded per round of the sweep
+  static constexpr int kHitBatch = 4;
   struct Args {
     const float4* __restrict__ xl;   // (x, lambda) of the iteration
     const float4* __restrict__ A;    // adjoint of the correction sum (before / pho0)
@@ -224,9 +243,8 @@ struct BwdDeltaPass {
     const float3 d = make_float3(dx, dy, dz);
     const float w = r2 < c.h2 ? poly6_of(r2, c) : 0.0f;
     const float q = (t.x.w + nb.x.w) + c.coef_corr * pow_corr(w, c);
-    const float u = c.h - r;
-    const float phi = c.spiky * u * u / r;
-    const float3 G = make_float3(phi * d.x, phi * d.y, phi * d.z);
+    const SpikyHess H = spiky_hess(c.spiky, r, recip(r), c);
+    const float3 G = make_float3(H.phi * d.x, H.phi * d.y, H.phi * d.z);
     float3 ds = make_float3(0.0f, 0.0f, 0.0f);
     if (r2 < c.h2) {
       const float k1 = dscorr_dw(w, c);
@@ -236,7 +254,7 @@ struct BwdDeltaPass {
     // own term: A_i . (q G(d))
     const float3 Ai = xyz(t.A);
     const float AG = dot3(Ai, G);
-    const float3 HA = spiky_hess(c.spiky, d, r, Ai, c);
+    const float3 HA = hess_apply(H, d, Ai);
     const float3 gi = make_float3(q * HA.x + AG * ds.x, q * HA.y + AG * ds.y, q * HA.z + AG * ds.z);
     s.x += gi.x;
     s.y += gi.y;
@@ -246,7 +264,7 @@ struct BwdDeltaPass {
       // j's term about i: A_j . (q G(x_j - x_i)) = -q A_j . G(d)
       const float3 Aj = xyz(nb.A);
       const float AGj = dot3(Aj, G);
-      const float3 HAj = spiky_hess(c.spiky, d, r, Aj, c);
+      const float3 HAj = hess_apply(H, d, Aj);
       s.x -= q * HAj.x + AGj * ds.x;
       s.y -= q * HAj.y + AGj * ds.y;
       s.z -= q * HAj.z + AGj * ds.z;
@@ -270,6 +288,8 @@ struct BwdDeltaPass {
 
 // ---- lambda (density and gradient norms) ------------------------------------
 struct BwdLambdaPass {
+  // hits reloaded per round of the sweep
+  static constexpr int kHitBatch = 4;
   struct Args {
     const float4* __restrict__ xr;   // (x, dL/drho)
     const float4* __restrict__ sd;   // (S, dL/dD)
@@ -298,7 +318,9 @@ struct BwdLambdaPass {
   static __device__ __forceinline__ bool hit(const Target& t, uint32_t, const float4& xj,
                                              const StepConsts& c) {
     float dx, dy, dz;
-    return dist2(t.x, xj, dx, dy, dz) < c.h2_outer;
+    // coincident pairs (and i itself) have d = 0: every term below vanishes
+    const float r2 = dist2(t.x, xj, dx, dy, dz);
+    return (r2 < c.h2_outer) & (r2 > 0.0f);
   }
   static __device__ __forceinline__ void add(Acc& s, const Target& t, uint32_t j, const Nb& nb,
                                              const StepConsts& c, const Args& a) {
@@ -306,7 +328,7 @@ struct BwdLambdaPass {
     const float r2 = dist2(t.x, nb.x, dx, dy, dz);
     if (!(r2 < c.h2_outer)) return;
     const float3 d = make_float3(dx, dy, dz);
-    const bool recip = is_candidate_of(t.cell, nb.x, c);
+    const bool mutual = is_candidate_of(t.cell, nb.x, c);
     float3 gi = make_float3(0.0f, 0.0f, 0.0f);  // own terms' d/dx_i
     float3 gr = make_float3(0.0f, 0.0f, 0.0f);  // j's terms' d/dx_i
     if (r2 < c.h2) {
@@ -316,22 +338,21 @@ struct BwdLambdaPass {
     }
     const float r = sqrtf(r2);
     if (r >= c.keps && r < c.h && j != t.i) {
-      const float u = c.h - r;
-      const float phig = c.spiky_rho0 * u * u / r;
-      const float3 g = make_float3(phig * d.x, phig * d.y, phig * d.z);
+      const SpikyHess H = spiky_hess(c.spiky_rho0, r, recip(r), c);
+      const float3 g = make_float3(H.phi * d.x, H.phi * d.y, H.phi * d.z);
       const float3 Vi = make_float3(g.x + t.sd.x, g.y + t.sd.y, g.z + t.sd.z);
-      const float3 Hi = spiky_hess(c.spiky_rho0, d, r, Vi, c);
+      const float3 Hi = hess_apply(H, d, Vi);
       const float ki = 2.0f * t.sd.w;
       gi = make_float3(gi.x + ki * Hi.x, gi.y + ki * Hi.y, gi.z + ki * Hi.z);
       const float3 Vj = make_float3(g.x - nb.sd.x, g.y - nb.sd.y, g.z - nb.sd.z);
-      const float3 Hj = spiky_hess(c.spiky_rho0, d, r, Vj, c);
+      const float3 Hj = hess_apply(H, d, Vj);
       const float kj = 2.0f * nb.sd.w;
       gr = make_float3(gr.x + kj * Hj.x, gr.y + kj * Hj.y, gr.z + kj * Hj.z);
     }
     s.x += gi.x;
     s.y += gi.y;
     s.z += gi.z;
-    if (recip) {
+    if (mutual) {
       s.x += gr.x;
       s.y += gr.y;
       s.z += gr.z;
@@ -492,19 +513,40 @@ __global__ __launch_bounds__(kBlock) void k_to_f4(const float* __restrict__ a, i
   out[i] = load_vec<L>(a, (uint32_t)i);
 }
 
-// Sum of n doubles by one workgroup in a fixed order.
-__global__ __launch_bounds__(kBlock) void k_sum_f64(const double* __restrict__ a, int n,
-                                                    double* __restrict__ out) {
+// Sum of n doubles in a fixed order (independent of the hardware's
+// scheduling): kSumParts contiguous segments, each summed by one workgroup
+// (strided per work-item, then a fixed tree), then the segment sums by one
+// workgroup.  part holds kSumParts doubles.
+constexpr int kSumParts = 1024;
+
+__device__ __forceinline__ double block_sum(double v) {
   __shared__ double sh[kBlock];
-  double s = 0.0;
-  for (int i = threadIdx.x; i < n; i += kBlock) s += a[i];
-  sh[threadIdx.x] = s;
+  sh[threadIdx.x] = v;
   __syncthreads();
   for (int off = kBlock / 2; off > 0; off >>= 1) {
     if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[0] = sh[0];
+  return sh[0];
+}
+
+__global__ __launch_bounds__(kBlock) void k_sum_parts(const double* __restrict__ a, int n,
+                                                      double* __restrict__ part) {
+  const long long seg = ((long long)n + kSumParts - 1) / kSumParts;
+  const long long lo = (long long)blockIdx.x * seg;
+  const long long hi = lo + seg < n ? lo + seg : n;
+  double v = 0.0;
+  for (long long i = lo + threadIdx.x; i < hi; i += kBlock) v += a[i];
+  const double t = block_sum(v);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(kBlock) void k_sum_total(const double* __restrict__ part,
+                                                      double* __restrict__ out) {
+  double v = 0.0;
+  for (int i = threadIdx.x; i < kSumParts; i += kBlock) v += part[i];
+  const double t = block_sum(v);
+  if (threadIdx.x == 0) out[0] = t;
 }
 
 int fill_blocks(int n) {
@@ -559,8 +601,9 @@ void launch_to_f4(int layout, const float* a, int n, float4* out, hipStream_t st
     k_to_f4<4><<<blocks_for(n), kBlock, 0, st>>>(a, n, out);
 }
 
-void launch_sum_f64(const double* a, int n, double* out, hipStream_t st) {
-  k_sum_f64<<<1, kBlock, 0, st>>>(a, n, out);
+void launch_sum_f64(const double* a, int n, double* part, double* out, hipStream_t st) {
+  k_sum_parts<<<kSumParts, kBlock, 0, st>>>(a, n, part);
+  k_sum_total<<<1, kBlock, 0, st>>>(part, out);
 }
 
 }  // namespace pbf

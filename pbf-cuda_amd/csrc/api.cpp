@@ -156,9 +156,15 @@ struct pbf_tape {
          *gxo = nullptr, *gvo = nullptr;
   float* grho_x = nullptr;
   float *gp[2] = {nullptr, nullptr}, *gv[2] = {nullptr, nullptr};  // inter-frame adjoints
-  double *r0 = nullptr, *r0_sum = nullptr;
+  double *r0 = nullptr, *r0_sum = nullptr;  // r0_sum: total, then 1024 partial sums
   pbf_params prm{};                 // parameters of the forward run
   std::vector<pbf_vec3> ul, ll;     // box of each frame of the forward run
+  // PBF_TAPE_STORE: what the backward pass would otherwise recompute, per
+  // frame (iteration arrays indexed frame * niter + iteration)
+  bool store = false;
+  std::vector<float4*> f_xl, f_sd, f_dp, f_xa, f_vr;
+  std::vector<float*> f_rho;
+  std::vector<uint32_t*> f_key, f_val, f_cs;
 };
 
 namespace {
@@ -567,9 +573,12 @@ int create_impl(const pbf_params& params, int capacity, long long cells, long lo
   const size_t N = (size_t)capacity;
   const size_t tiles = (size_t)pbf::radix_tiles(capacity);
   const size_t dblk = (size_t)pbf::density_partials(capacity);
-  // chunk items of crowded targets: 2 per particle covers piles holding a
-  // tenth of the particles; beyond that the targets are summed in place
-  const size_t ICAP = 2 * N + 1024;
+  // chunk items of crowded targets (48 B each with their partial sums): 8
+  // per particle.  A pile of P particles in one cell needs ~P^2/64 items (the
+  // 4M-particle sweeping-wall run piles 17,000 into a corner cell and needs
+  // ~18M); targets beyond the list are summed by their own work-item, which
+  // is correct but serial
+  const size_t ICAP = std::min<size_t>(8 * N + 4096, 0xF0000000u);
   size_t off = 0;
   auto carve = [&](size_t bytes) {
     const size_t o = off;
@@ -1140,6 +1149,10 @@ int pbf_get_timing(pbf_solver* s, double ms_total[PBF_NUM_PHASES],
 
 
 int pbf_tape_create(pbf_solver* s, int n, int nframes, pbf_tape** out) {
+  return pbf_tape_create_ex(s, n, nframes, 0, out);
+}
+
+int pbf_tape_create_ex(pbf_solver* s, int n, int nframes, int flags, pbf_tape** out) {
   if (!out) return fail(s, PBF_ERR_INVALID_ARGUMENT, "out is NULL");
   *out = nullptr;
   if (!s) return fail(nullptr, PBF_ERR_INVALID_ARGUMENT, "solver is NULL");
@@ -1149,6 +1162,7 @@ int pbf_tape_create(pbf_solver* s, int n, int nframes, pbf_tape** out) {
                 n);
   if (nframes < 1 || nframes > 100000)
     return fail(s, PBF_ERR_INVALID_ARGUMENT, "nframes must be in [1, 100000], got %d", nframes);
+  if (flags & ~PBF_TAPE_STORE) return fail(s, PBF_ERR_INVALID_ARGUMENT, "unknown tape flags %d", flags);
   DeviceGuard g(s->device);
   if (!g.ok) return fail(s, PBF_ERR_HIP, "hipSetDevice(%d) failed", s->device);
   pbf_tape* t = new (std::nothrow) pbf_tape();
@@ -1158,6 +1172,7 @@ int pbf_tape_create(pbf_solver* s, int n, int nframes, pbf_tape** out) {
   t->nframes = nframes;
   t->niter = s->prm.niter;
   t->layout = s->layout;
+  t->store = (flags & PBF_TAPE_STORE) != 0;
   const size_t N = (size_t)n, L = (size_t)s->layout, NI = (size_t)t->niter;
   size_t off = 0;
   auto carve = [&](size_t bytes) {
@@ -1184,7 +1199,24 @@ int pbf_tape_create(pbf_solver* s, int n, int nframes, pbf_tape** out) {
   for (auto& o : o_f4) o = carve(16 * N);
   const size_t o_grx = carve(4 * N), o_gp0 = carve(4 * L * N), o_gp1 = carve(4 * L * N),
                o_gv0 = carve(4 * L * N), o_gv1 = carve(4 * L * N), o_r0 = carve(8 * N),
-               o_r0s = carve(8);
+               o_r0s = carve(8 * (1 + 1024));
+  std::vector<size_t> o_fxl, o_fsd, o_fdp, o_frho, o_fxa, o_fvr, o_fkey, o_fval, o_fcs;
+  if (t->store) {
+    const size_t cs_words = (size_t)s->max_cells + 1;
+    for (int f = 0; f < nframes; ++f) {
+      for (size_t k = 0; k < NI; ++k) {
+        o_fxl.push_back(carve(16 * N));
+        o_fsd.push_back(carve(16 * N));
+        o_fdp.push_back(carve(16 * N));
+        o_frho.push_back(carve(4 * N));
+      }
+      o_fxa.push_back(carve(16 * N));
+      o_fvr.push_back(carve(16 * N));
+      o_fkey.push_back(carve(4 * N));
+      o_fval.push_back(carve(4 * N));
+      o_fcs.push_back(carve(4 * cs_words));
+    }
+  }
   hipError_t e = hipMalloc(&t->arena, off);
   if (e != hipSuccess) {
     delete t;
@@ -1217,6 +1249,19 @@ int pbf_tape_create(pbf_solver* s, int n, int nframes, pbf_tape** out) {
   t->gv[1] = (float*)(b + o_gv1);
   t->r0 = (double*)(b + o_r0);
   t->r0_sum = (double*)(b + o_r0s);
+  for (size_t q = 0; q < o_fxl.size(); ++q) {
+    t->f_xl.push_back((float4*)(b + o_fxl[q]));
+    t->f_sd.push_back((float4*)(b + o_fsd[q]));
+    t->f_dp.push_back((float4*)(b + o_fdp[q]));
+    t->f_rho.push_back((float*)(b + o_frho[q]));
+  }
+  for (size_t q = 0; q < o_fxa.size(); ++q) {
+    t->f_xa.push_back((float4*)(b + o_fxa[q]));
+    t->f_vr.push_back((float4*)(b + o_fvr[q]));
+    t->f_key.push_back((uint32_t*)(b + o_fkey[q]));
+    t->f_val.push_back((uint32_t*)(b + o_fval[q]));
+    t->f_cs.push_back((uint32_t*)(b + o_fcs[q]));
+  }
   // the scatter arrays start (and stay between kernels) all zero
   pbf::launch_fill_f4(t->ex_x, n, nullptr);
   pbf::launch_fill_f4(t->ex_v, n, nullptr);
@@ -1304,9 +1349,38 @@ int pbf_tape_forward(pbf_solver* s, pbf_tape* t, const float* pos, const float* 
         (e = hipMemcpyAsync(t->ciid[f + 1], t->ciid[f], ib, hipMemcpyDeviceToDevice, st)) !=
             hipSuccess)
       return fail_hip(s, e, "tape checkpoint copy");
-    if ((rc = pbf_step(s, t->wpos, t->cpos[f + 1], t->wvel, t->cvel[f + 1], t->ciid[f + 1], t->n,
-                       stream)))
-      return rc;
+    if (!t->store) {
+      if ((rc = pbf_step(s, t->wpos, t->cpos[f + 1], t->wvel, t->cvel[f + 1], t->ciid[f + 1],
+                         t->n, stream)))
+        return rc;
+      continue;
+    }
+    // stored tape: the step keeps its iterations in frame f's arrays, then
+    // the grid, the sort and the pre-XSPH state the adjoint reads are kept
+    pbf::StepConsts c;
+    if ((rc = step_consts(s, c))) return rc;
+    const size_t q0 = (size_t)f * (size_t)t->niter;
+    Record rec;
+    rec.xl = &t->f_xl[q0];
+    rec.sd = &t->f_sd[q0];
+    rec.dp = &t->f_dp[q0];
+    rec.rho = &t->f_rho[q0];
+    if (s->cnt_dirty) pbf::launch_fill_u32(s->cnt, s->max_cells, 0u, st);
+    s->cnt_dirty = true;
+    const StepArrays arr{t->wpos, t->cpos[f + 1], t->wvel, t->cvel[f + 1], t->ciid[f + 1], t->n};
+    enqueue_step(s, arr, c, st, false, &rec);
+    const size_t fb = (size_t)f, v4 = (size_t)t->n * 16;
+    if ((e = hipMemcpyAsync(t->f_key[fb], rec.skey, ib, hipMemcpyDeviceToDevice, st)) !=
+            hipSuccess ||
+        (e = hipMemcpyAsync(t->f_val[fb], rec.sval, ib, hipMemcpyDeviceToDevice, st)) !=
+            hipSuccess ||
+        (e = hipMemcpyAsync(t->f_cs[fb], s->cs, ((size_t)c.cells + 1) * 4,
+                            hipMemcpyDeviceToDevice, st)) != hipSuccess ||
+        (e = hipMemcpyAsync(t->f_xa[fb], s->xa, v4, hipMemcpyDeviceToDevice, st)) != hipSuccess ||
+        (e = hipMemcpyAsync(t->f_vr[fb], s->xp, v4, hipMemcpyDeviceToDevice, st)) != hipSuccess)
+      return fail_hip(s, e, "tape frame copy");
+    if ((e = hipGetLastError()) != hipSuccess) return fail_hip(s, e, "kernel launch");
+    s->cnt_dirty = false;
   }
   t->prm = s->prm;
   t->forward_done = true;
@@ -1355,6 +1429,10 @@ int pbf_tape_backward(pbf_solver* s, pbf_tape* t, const float* gpos, const float
   rec.dp = dp.data();
   rec.rho = rho.data();
   int cur = 0;
+  // Neighbour passes of the adjoint kernels: after a recompute they follow
+  // its passes 0 .. 2 niter; a stored tape numbers them on from the forward
+  // run's last pass (any step leaves the odd parity's counters cleared).
+  int pass = 2 * NI + 1;
   for (int f = t->nframes - 1; f >= 0; --f) {
     // frame f under its own box
     s->ulim = t->ul[(size_t)f];
@@ -1365,22 +1443,45 @@ int pbf_tape_backward(pbf_solver* s, pbf_tape* t, const float* gpos, const float
       s->llim = ll_saved;
       return rc;
     }
-    // recompute frame f from its checkpoint, keeping its iterations
-    if ((e = hipMemcpyAsync(t->wpos, t->cpos[f], vb, hipMemcpyDeviceToDevice, st)) != hipSuccess ||
-        (e = hipMemcpyAsync(t->wvel, t->cvel[f], vb, hipMemcpyDeviceToDevice, st)) != hipSuccess ||
-        (e = hipMemcpyAsync(t->wiid, t->ciid[f], ib, hipMemcpyDeviceToDevice, st)) != hipSuccess)
-      return fail_hip(s, e, "tape checkpoint copy");
-    if (s->cnt_dirty) pbf::launch_fill_u32(s->cnt, s->max_cells, 0u, st);
-    s->cnt_dirty = true;
-    const StepArrays arr{t->wpos, t->wnpos, t->wvel, t->wnvel, t->wiid, n};
-    enqueue_step(s, arr, c, st, false, &rec);
+    const uint32_t* cs = s->cs;
+    const float4 *x_final = s->xa, *vr = s->xp;
+    const uint32_t *skey = nullptr, *sval = nullptr;
+    std::vector<pbf::BwdIter> iters((size_t)NI);
+    if (t->store) {
+      const size_t q0 = (size_t)f * (size_t)NI, fb = (size_t)f;
+      for (size_t k = 0; k < (size_t)NI; ++k)
+        iters[k] = pbf::BwdIter{t->f_xl[q0 + k], t->f_sd[q0 + k], t->f_rho[q0 + k],
+                                t->f_dp[q0 + k]};
+      cs = t->f_cs[fb];
+      x_final = t->f_xa[fb];
+      vr = t->f_vr[fb];
+      skey = t->f_key[fb];
+      sval = t->f_val[fb];
+    } else {
+      // recompute frame f from its checkpoint, keeping its iterations
+      if ((e = hipMemcpyAsync(t->wpos, t->cpos[f], vb, hipMemcpyDeviceToDevice, st)) !=
+              hipSuccess ||
+          (e = hipMemcpyAsync(t->wvel, t->cvel[f], vb, hipMemcpyDeviceToDevice, st)) !=
+              hipSuccess ||
+          (e = hipMemcpyAsync(t->wiid, t->ciid[f], ib, hipMemcpyDeviceToDevice, st)) != hipSuccess)
+        return fail_hip(s, e, "tape checkpoint copy");
+      if (s->cnt_dirty) pbf::launch_fill_u32(s->cnt, s->max_cells, 0u, st);
+      s->cnt_dirty = true;
+      const StepArrays arr{t->wpos, t->wnpos, t->wvel, t->wnvel, t->wiid, n};
+      enqueue_step(s, arr, c, st, false, &rec);
+      for (size_t k = 0; k < (size_t)NI; ++k)
+        iters[k] = pbf::BwdIter{t->xl[k], t->sd[k], t->rho[k], t->dp[k]};
+      skey = rec.skey;
+      sval = rec.sval;
+      pass = 2 * NI + 1;
+    }
     // adjoint of the frame
     pbf::launch_to_f4(L, t->gp[cur], n, t->gxo, st);
     pbf::launch_to_f4(L, t->gv[cur], n, t->gvo, st);
     pbf::BwdFrame bf{};
-    bf.x_final = s->xa;
-    bf.vr = s->xp;
-    bf.key = rec.skey;
+    bf.x_final = x_final;
+    bf.vr = vr;
+    bf.key = skey;
     bf.gx_out = t->gxo;
     bf.gv_out = t->gvo;
     bf.out_x = t->out_x;
@@ -1395,24 +1496,22 @@ int pbf_tape_backward(pbf_solver* s, pbf_tape* t, const float* gpos, const float
     bf.sdb = t->sdb;
     bf.grho_x = t->grho_x;
     bf.r0 = t->r0;
-    int pass = 2 * NI + 1;  // the recompute used neighbour passes 0 .. 2 niter
-    pbf::launch_bwd_xsph(bf, n, c, s->cs, s->hl, pass++, s->shapes, st);
+    pbf::launch_bwd_xsph(bf, n, c, cs, s->hl, pass++, s->shapes, st);
     for (int k = NI - 1; k >= 0; --k) {
-      const pbf::BwdIter it{t->xl[(size_t)k], t->sd[(size_t)k], t->rho[(size_t)k],
-                            t->dp[(size_t)k]};
-      pbf::launch_bwd_iteration(bf, it, k == NI - 1, n, c, s->cs, s->hl, pass, s->shapes, st);
+      pbf::launch_bwd_iteration(bf, iters[(size_t)k], k == NI - 1, n, c, cs, s->hl, pass,
+                                s->shapes, st);
       pass += 2;
     }
     float* op = f == 0 ? gpos0 : t->gp[cur ^ 1];
     float* ov = f == 0 ? gvel0 : t->gv[cur ^ 1];
-    pbf::launch_bwd_predict(L, bf, rec.sval, n, c, op, ov, st);
+    pbf::launch_bwd_predict(L, bf, sval, n, c, op, ov, st);
     cur ^= 1;
     if ((e = hipGetLastError()) != hipSuccess) return fail_hip(s, e, "kernel launch");
     s->cnt_dirty = false;
   }
   s->ulim = ul_saved;
   s->llim = ll_saved;
-  pbf::launch_sum_f64(t->r0, n, t->r0_sum, st);
+  pbf::launch_sum_f64(t->r0, n, t->r0_sum + 1, t->r0_sum, st);
   double host = 0.0;
   if ((e = hipMemcpyAsync(&host, t->r0_sum, sizeof(double), hipMemcpyDeviceToHost, st)) != hipSuccess)
     return fail_hip(s, e, "hipMemcpyAsync");

@@ -59,6 +59,7 @@ void launch_fill_f64(double* a, int n, hipStream_t st);
 // Caller-layout vector array -> float4.
 void launch_to_f4(int layout, const float* a, int n, float4* out, hipStream_t st);
 // out[0] = sum of a[0..n) in a fixed order (one workgroup).
-void launch_sum_f64(const double* a, int n, double* out, hipStream_t st);
+// Fixed-order sum of n doubles into out[0]; part holds 1024 doubles.
+void launch_sum_f64(const double* a, int n, double* part, double* out, hipStream_t st);
 
 }  // namespace pbf

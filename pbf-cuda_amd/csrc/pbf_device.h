@@ -254,14 +254,15 @@ struct SeqCursor1 {
 // calling add() only for candidates passing the cheap test hit().  Blocks of
 // 64 positions are screened eight loads at a time (across column
 // boundaries, so a sparse neighbourhood costs a few memory round trips, not
-// one or two per column), then the hits are reloaded four at a time and
-// accumulated in ascending order.  Candidates failing hit() contribute
+// one or two per column), then the hits are reloaded G at a time (a pass
+// policy's kHitBatch) and accumulated in ascending order.  Candidates
+// failing hit() contribute
 // exactly nothing in add(), so the sums equal the plain in-order sum.
 // The screening is branch-free after the loads (all eight tests are
 // evaluated and masked), so the compiler keeps the eight loads in flight
 // together instead of sinking some of them into conditional blocks that
 // wait for each load in turn; hit() must be branch-free as well.
-template <class LoadS, class LoadF, class Hit, class Add>
+template <int G, class LoadS, class LoadF, class Hit, class Add>
 __device__ __forceinline__ void sweep_seq(const SeqTable& st, uint32_t m, uint32_t u0,
                                           uint32_t u1, LoadS&& load_s, LoadF&& load_f, Hit&& hit,
                                           Add&& add) {
@@ -290,10 +291,10 @@ __device__ __forceinline__ void sweep_seq(const SeqTable& st, uint32_t m, uint32
     uint32_t last = 0;
 #pragma unroll 1
     while (mask) {
-      uint32_t j[4];
-      bool h[4];
+      uint32_t j[G];
+      bool h[G];
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
+      for (int t = 0; t < G; ++t) {
         h[t] = mask != 0u;
         // past the last hit, repeat it (the cursor only moves forward)
         const uint32_t bit = h[t] ? (uint32_t)__builtin_ctzll(mask) : last;
@@ -301,11 +302,11 @@ __device__ __forceinline__ void sweep_seq(const SeqTable& st, uint32_t m, uint32
         mask &= mask - 1u;
         j[t] = c2.slot(bb + bit);
       }
-      decltype(load_f(0u)) d[4];
+      decltype(load_f(0u)) d[G];
 #pragma unroll
-      for (int t = 0; t < 4; ++t) d[t] = load_f(j[t]);
+      for (int t = 0; t < G; ++t) d[t] = load_f(j[t]);
 #pragma unroll
-      for (int t = 0; t < 4; ++t)
+      for (int t = 0; t < G; ++t)
         if (h[t]) add(j[t], d[t]);
     }
   }

@@ -146,7 +146,8 @@ __global__ __launch_bounds__(kBlock) void k_light(typename P::Args a, int n, Ste
   auto hit = [&](uint32_t j, const auto& d) { return P::hit(t, j, d, c); };
   typename P::Acc s;
   if (ms <= kLightMax) {
-    sweep_seq(sts, ms, 0u, ms, ls, lf, hit, [&](uint32_t j, const auto& d) { P::add(s, t, j, d, c, a); });
+    sweep_seq<P::kHitBatch>(sts, ms, 0u, ms, ls, lf, hit,
+                            [&](uint32_t j, const auto& d) { P::add(s, t, j, d, c, a); });
   } else {
     // chunk list full: this lane sums the target's chunks in order
     const uint32_t nc = (ms + kChunk - 1) / kChunk;
@@ -154,7 +155,7 @@ __global__ __launch_bounds__(kBlock) void k_light(typename P::Args a, int n, Ste
       typename P::Acc part;
       const uint32_t s0 = ch * kChunk;
       const uint32_t s1 = s0 + kChunk < ms ? s0 + kChunk : ms;
-      sweep_seq(sts, ms, s0, s1, ls, lf, hit,
+      sweep_seq<P::kHitBatch>(sts, ms, s0, s1, ls, lf, hit,
                 [&](uint32_t j, const auto& d) { P::add(part, t, j, d, c, a); });
       P::accumulate(s, part);
     }
@@ -183,7 +184,7 @@ __global__ __launch_bounds__(kBlock) void k_chunks(typename P::Args a, StepConst
     typename P::Acc part;
     const uint32_t s0 = it.y * kChunk;
     const uint32_t s1 = s0 + kChunk < m ? s0 + kChunk : m;
-    sweep_seq(
+    sweep_seq<P::kHitBatch>(
         st, m, s0, s1, [&](uint32_t j) { return P::load_s(a, j); },
         [&](uint32_t j) { return P::load_f(a, j); },
         [&](uint32_t j, const auto& d) { return P::hit(t, j, d, c); },

@@ -33,6 +33,8 @@ namespace {
 // into grad_i and, for j != i, its squared norm into gradj_l2; then the wall
 // term and lambda = -(rho/pho0 - 1) / (gradj_l2 + |grad_i|^2 + eps).
 struct LambdaPass {
+  // hits reloaded per round of the sweep
+  static constexpr int kHitBatch = 4;
   struct Args {
     const float4* __restrict__ x;
     float4* __restrict__ xl;
@@ -111,6 +113,8 @@ struct LambdaPass {
 // (x, lambda) from xl, writes the new iterate to xa (Jacobi; the ping-pong
 // replaces thrust::copy_n at fluids/Simulator.cu:247-248).
 struct DeltaPass {
+  // hits reloaded per round of the sweep
+  static constexpr int kHitBatch = 4;
   struct Args {
     const float4* __restrict__ xl;
     float4* __restrict__ xa;
@@ -182,6 +186,8 @@ struct DeltaPass {
 // positions into the caller's npos.
 template <int L>
 struct XsphPass {
+  // hits reloaded per round of the sweep
+  static constexpr int kHitBatch = 4;
   struct Args {
     const float4* __restrict__ xa;
     const float4* __restrict__ vr;  // (v, rho)
@@ -243,6 +249,8 @@ struct XsphPass {
 // rho on a fresh grid (the lambda pass's density term), for the density
 // error statistic: |rho/pho0 - 1| per particle in FP64.
 struct DensityPass {
+  // hits reloaded per round of the sweep
+  static constexpr int kHitBatch = 4;
   struct Args {
     const float4* __restrict__ x;
     double* __restrict__ err;

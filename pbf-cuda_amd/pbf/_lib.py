@@ -104,6 +104,8 @@ _SIGNATURES = [
      [_P, _P, _P, _P, ctypes.c_int, ctypes.c_int, _P, ctypes.POINTER(PbfShapes),
       ctypes.POINTER(ctypes.c_double)]),
     ("pbf_tape_create", ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_P)]),
+    ("pbf_tape_create_ex", ctypes.c_int,
+     [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_P)]),
     ("pbf_tape_destroy", None, [_P]),
     ("pbf_tape_forward", ctypes.c_int,
      [_P, _P, _P, _P, _P, ctypes.c_int, ctypes.POINTER(PbfSweep), _P]),

@@ -234,18 +234,27 @@ class Simulator:
         return (int(best.light_block), int(best.chunk_blocks)), float(ms.value)
 
 
+TAPE_STORE = 1
+
+
 class Tape:
     """Checkpointed run of `nframes` frames of a single-scene Simulator and
-    its backward pass (pbf_tape_*)."""
+    its backward pass (pbf_tape_*).  store=True keeps every frame's solver
+    iterations, sort and grid on the device (pbf_tape_create_ex with
+    PBF_TAPE_STORE), so the backward pass recomputes nothing; the gradients
+    are the same bits either way."""
 
-    def __init__(self, sim: "Simulator", nparticle: int, nframes: int):
+    def __init__(self, sim: "Simulator", nparticle: int, nframes: int, store: bool = False):
         self._lib = _lib.load()
         self.sim = sim
         self.n = int(nparticle)
         self.nframes = int(nframes)
+        self.store = bool(store)
         self._t = ctypes.c_void_p()
         h = sim._handle()
-        _lib.check(self._lib.pbf_tape_create(h, self.n, self.nframes, ctypes.byref(self._t)), h)
+        _lib.check(self._lib.pbf_tape_create_ex(h, self.n, self.nframes,
+                                                TAPE_STORE if self.store else 0,
+                                                ctypes.byref(self._t)), h)
 
     def close(self) -> None:
         if self._t:

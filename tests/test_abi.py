@@ -25,7 +25,7 @@ def test_library_exports_every_declared_symbol(pbf_mod):
     for name in declared:
         assert hasattr(lib, name), name
     assert set(declared) == set(_lib.SYMBOLS), "ctypes prototypes and header disagree"
-    assert lib.pbf_abi_version() == 2
+    assert lib.pbf_abi_version() == 3
 
 
 def test_status_strings(pbf_mod):

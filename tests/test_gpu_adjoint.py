@@ -31,15 +31,15 @@ def _target(sc, seed=1):
     return sc.pos.astype(np.float64) + rng.normal(0, 0.05, (sc.n, 3))
 
 
-def gpu_grad(pbf_mod, sc, p, nframes, target, layout=4):
+def gpu_grad(pbf_mod, sc, p, nframes, target, layout=4, store=False, sweep=None):
     """Loss gradients from the C ABI: (g_pos0, g_vel0 as N x 3 in input order,
     g_rho0, final positions by identity)."""
     n = sc.n
     sim = pbf_mod.Simulator(p, sc.ulim, sc.llim, capacity=n, layout=layout)
-    tape = pbf_mod.Tape(sim, n, nframes)
+    tape = pbf_mod.Tape(sim, n, nframes, store=store)
     pos, vel = sc.as_layout(layout)
     dev = "cuda:0"
-    tape.forward(to_dev(pos, dev), to_dev(vel, dev), to_dev(sc.iid, dev))
+    tape.forward(to_dev(pos, dev), to_dev(vel, dev), to_dev(sc.iid, dev), sweep=sweep)
     xf = torch.zeros(n * layout, device=dev)
     iidf = torch.zeros(n, dtype=torch.int32, device=dev)
     tape.read(nframes, pos=xf, iid=iidf)
@@ -212,3 +212,67 @@ def test_adjoint_with_a_moving_wall_matches_oracle(pbf_mod, orc, device):
     x_by_iid[I] = X
     np.testing.assert_array_equal(x_by_iid.astype(np.float32), o["x_final"].astype(np.float32))
     assert rel(gv, o["g_vel0"]) < 1e-4, rel(gv, o["g_vel0"])
+
+
+@pytest.mark.parametrize("pname,layout,moving", [("reference", 4, False), ("dam", 3, False),
+                                                 ("reference", 4, True)])
+def test_stored_tape_gives_the_same_gradients(pbf_mod, device, pname, layout, moving):
+    """A tape that stores its iterations (PBF_TAPE_STORE) gives the gradients
+    of one that recomputes them, including under a moving wall (per-frame
+    grids of different sizes).  The forward states are the same bits; the
+    gradients agree to the run-to-run spread of the adjoint's atomic scatter
+    sums (pairs seen from one side only), whose order of addition varies."""
+    p = P.dam_params() if pname == "dam" else P.reference_defaults()
+    sc = scenes.block((8, 8, 8), (1.0, 1.0, 1.0), seed=9, vel_sigma=3.0)
+    tgt = _target(sc)
+    sw = (pbf_mod.Sweep(sc.ulim, sc.llim, a_ulim=(-0.75, 0.0, 0.0), w=0.5) if moving else None)
+    print(f"stored tape {pname} layout {layout} moving {moving}", flush=True)
+    a = gpu_grad(pbf_mod, sc, p, 4, tgt, layout, store=False, sweep=sw)
+    b = gpu_grad(pbf_mod, sc, p, 4, tgt, layout, store=True, sweep=sw)
+    np.testing.assert_array_equal(a[3], b[3])  # final positions
+    assert rel(b[0], a[0]) < 1e-4, rel(b[0], a[0])
+    assert rel(b[1], a[1]) < 1e-4, rel(b[1], a[1])
+    assert abs(b[2] - a[2]) <= 1e-4 * abs(a[2]), (a[2], b[2])
+
+
+def test_stored_tape_backward_twice_and_after_steps(pbf_mod, device):
+    """A stored tape's backward pass can run again, also after unrelated
+    steps of the solver, with the same result."""
+    p = P.reference_defaults()
+    sc = _scene()
+    tgt = _target(sc)
+    n = sc.n
+    sim = pbf_mod.Simulator(p, sc.ulim, sc.llim, capacity=n)
+    tape = pbf_mod.Tape(sim, n, 3, store=True)
+    pos, vel = sc.as_layout(4)
+    tape.forward(to_dev(pos, device), to_dev(vel, device), to_dev(sc.iid, device))
+    g = torch.from_numpy(np.tile(np.float32([0.1, -0.2, 0.3, 0.0]), n)).to(device)
+    outs = []
+    for rep in range(2):
+        gp0 = torch.zeros(n * 4, device=device)
+        gv0 = torch.zeros(n * 4, device=device)
+        gr = tape.backward(g, None, gp0, gv0)
+        outs.append((from_dev(gp0), from_dev(gv0), gr))
+        # unrelated steps in between
+        x = [to_dev(pos, device), to_dev(vel, device)]
+        y = [torch.zeros(n * 4, device=device), torch.zeros(n * 4, device=device)]
+        ids = to_dev(sc.iid, device)
+        sim.step(x[0], y[0], x[1], y[1], ids, n)
+        torch.cuda.synchronize()
+    assert rel(outs[1][0], outs[0][0]) < 1e-5
+    assert rel(outs[1][1], outs[0][1]) < 1e-5
+    assert abs(outs[1][2] - outs[0][2]) <= 1e-5 * abs(outs[0][2])
+
+
+def test_tape_rejects_unknown_flags(pbf_mod, device):
+    import ctypes
+
+    from pbf import _lib
+
+    p = P.reference_defaults()
+    sc = _scene()
+    sim = pbf_mod.Simulator(p, sc.ulim, sc.llim, capacity=sc.n)
+    lib = _lib.load()
+    t = ctypes.c_void_p()
+    assert lib.pbf_tape_create_ex(sim._handle(), sc.n, 2, 4, ctypes.byref(t)) == 1
+    assert b"flags" in lib.pbf_last_error(sim._handle())

@@ -231,8 +231,7 @@ def adjoint_ratio(pbf, p, dev, stream, reps=3, store=True):
             "workload": "differentiable_16k: 16,384 particles (32x16x32 jittered lattice, "
                         "v0 ~ N(0, 0.5^2)), box 40h x 20h x 40h, 30 frames, 4 iterations, dam "
                         "constants; loss = mean squared distance to a seeded target cloud; "
-                        "gradients w.r.t. pos0, vel0, pho0; per-frame checkpoints, iterations "
-                        "recomputed"}
+                        "gradients w.r.t. pos0, vel0, pho0"}
 
 
 def slab_share(pbf, p, pos, vel, iid, ulim, llim, shapes, dev, stream, parts=8, warmup=3,

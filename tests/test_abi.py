@@ -187,6 +187,9 @@ def test_new_entry_points_reject_null_solver(pbf_mod):
     k = ctypes.c_int(0)
     assert lib.pbf_get_scenes(None, ctypes.byref(k), None) == bad
     assert lib.pbf_autotune(None, None, None, None, 10, 3, None, None, None) == bad
+    t = ctypes.c_void_p()
+    assert lib.pbf_tape_create_ex(None, 10, 2, 1, ctypes.byref(t)) == bad
+    assert lib.pbf_tape_create_ex(None, 10, 2, 0, None) == bad
 
 
 def test_batch_scene_generator(pbf_mod):

@@ -390,10 +390,13 @@ __global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t* __restrict__ a, l
 // Replaces cudaMemset + computeGridRange (fluids/Simulator.cu:201-204,
 // fluids/Simulator_kernel.cuh:21-50): cs[c] = number of particles in cells
 // < c, which is the first sorted slot of cell c (empty cells: the next
-// non-empty cell's start).  Three kernels: tile sums, scan of the tile sums,
-// tile-local scan + offset (which also clears the counts for the next step).
+// non-empty cell's start).  Kernels: tile sums, scan of the tile sums (large
+// grids only), tile-local scan + offset (which also clears the counts for the
+// next step).
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kBlock * kScanItems;
+// Grids of up to this many tiles (4.2M cells) scan in two kernels.
+constexpr int kFusedScanTiles = 2048;
 
 __device__ __forceinline__ uint32_t block_sum(uint32_t v, uint32_t* sh) {
   // wave sums by shuffles, then the four wave sums
@@ -455,31 +458,63 @@ __global__ __launch_bounds__(kBlock) void k_scan_tile_sums(uint32_t* __restrict_
   if (threadIdx.x == 0) cs[cells] = carry;
 }
 
+// Tile-local exclusive scan plus the tile's offset; also clears the counts
+// for the next step.  tile_off holds the tile offsets (kFused = false), or the
+// tile sums themselves (kFused = true: each workgroup adds up the sums of the
+// tiles before it, and the last one writes cs[cells]).  Thread t owns cells
+// base + 8t .. base + 8t + 7, read and written as two 16-byte vectors.
+template <bool kFused>
 __global__ __launch_bounds__(kBlock) void k_scan_cells(uint32_t* __restrict__ cnt, long long cells,
                                                        const uint32_t* __restrict__ tile_off,
                                                        uint32_t* __restrict__ cs) {
+  static_assert(kScanItems == 8, "two uint4 per thread");
   __shared__ uint32_t sh[kBlock / 64];
-  // thread t owns cells base + t*kScanItems .. + kScanItems-1 (consecutive)
+  uint32_t off;
+  if (kFused) {
+    uint32_t v = 0;
+    for (int t = threadIdx.x; t < (int)blockIdx.x; t += kBlock) v += tile_off[t];
+    off = block_sum(v, sh);
+  } else {
+    off = tile_off[blockIdx.x];
+  }
   const long long base = (long long)blockIdx.x * kScanTile + (long long)threadIdx.x * kScanItems;
+  const bool whole = base + kScanItems <= cells;
   uint32_t v[kScanItems];
+  if (whole) {
+    const uint4 a0 = *reinterpret_cast<const uint4*>(cnt + base);
+    const uint4 a1 = *reinterpret_cast<const uint4*>(cnt + base + 4);
+    v[0] = a0.x, v[1] = a0.y, v[2] = a0.z, v[3] = a0.w;
+    v[4] = a1.x, v[5] = a1.y, v[6] = a1.z, v[7] = a1.w;
+  } else {
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) v[k] = base + k < cells ? cnt[base + k] : 0u;
+  }
   uint32_t sum = 0;
 #pragma unroll
-  for (int k = 0; k < kScanItems; ++k) {
-    const long long c = base + k;
-    v[k] = c < cells ? cnt[c] : 0u;
-    sum += v[k];
-  }
+  for (int k = 0; k < kScanItems; ++k) sum += v[k];
   uint32_t total;
-  uint32_t run = tile_off[blockIdx.x] + block_exclusive(sum, sh, &total);
+  const uint32_t first = off + block_exclusive(sum, sh, &total);
+  uint32_t o[kScanItems];
+  uint32_t run = first;
 #pragma unroll
   for (int k = 0; k < kScanItems; ++k) {
-    const long long c = base + k;
-    if (c < cells) {
-      cs[c] = run;
-      cnt[c] = 0u;
-    }
+    o[k] = run;
     run += v[k];
   }
+  if (whole) {
+    *reinterpret_cast<uint4*>(cs + base) = make_uint4(o[0], o[1], o[2], o[3]);
+    *reinterpret_cast<uint4*>(cs + base + 4) = make_uint4(o[4], o[5], o[6], o[7]);
+    *reinterpret_cast<uint4*>(cnt + base) = make_uint4(0u, 0u, 0u, 0u);
+    *reinterpret_cast<uint4*>(cnt + base + 4) = make_uint4(0u, 0u, 0u, 0u);
+  } else {
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k)
+      if (base + k < cells) {
+        cs[base + k] = o[k];
+        cnt[base + k] = 0u;
+      }
+  }
+  if (kFused && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) cs[cells] = off + total;
 }
 
 // Gather of the sorted payload (the zip permutation of
@@ -661,8 +696,13 @@ void launch_cell_starts(uint32_t* cnt, long long cells, uint32_t* tile_sum, uint
                         hipStream_t st) {
   const int ntiles = (int)scan_tiles(cells);
   k_scan_tiles<<<ntiles, kBlock, 0, st>>>(cnt, cells, tile_sum);
+  if (ntiles <= kFusedScanTiles) {
+    // each workgroup adds up the tile sums before it (<= 2048 words)
+    k_scan_cells<true><<<ntiles, kBlock, 0, st>>>(cnt, cells, tile_sum, cs);
+    return;
+  }
   k_scan_tile_sums<<<1, kBlock, 0, st>>>(tile_sum, ntiles, cells, cs);
-  k_scan_cells<<<ntiles, kBlock, 0, st>>>(cnt, cells, tile_sum, cs);
+  k_scan_cells<false><<<ntiles, kBlock, 0, st>>>(cnt, cells, tile_sum, cs);
 }
 
 void launch_permute(int layout, const uint32_t* skey, const uint32_t* sval, const float* pos,

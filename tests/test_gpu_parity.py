@@ -283,3 +283,21 @@ def test_run_frames_sweeping_wall_bitexact(pbf_mod, orc, device):
     np.testing.assert_array_equal(from_dev(run.iid, dtype=np.uint32), iid)
     np.testing.assert_array_equal(g_pos.view(np.uint32), pos.view(np.uint32))
     np.testing.assert_array_equal(g_vel.view(np.uint32), vel.view(np.uint32))
+
+
+@pytest.mark.parametrize("box", [(30.0, 20.0, 15.0), (6.4, 6.4, 3.2)])
+def test_large_and_small_grids_bitexact(pbf_mod, orc, device, box):
+    """A sparse 300x200x150-cell box (9M cells: the cell-start scan runs its
+    three-kernel form) and a 64x64x32 one (two-kernel form), two frames each,
+    bit-exact against the FP32 mirror."""
+    p = P.dam_params()
+    sc = scenes.block((16, 16, 16), box, seed=21, vel_sigma=1.0)
+    print(f"grid box {box} n {sc.n}", flush=True)
+    run = Run(pbf_mod, sc, p, 4)
+    pos, vel = sc.as_layout(4)
+    iid = sc.iid
+    for f in range(2):
+        g = run.step()
+        m = mirror(orc, p, sc, pos, vel, iid)
+        assert_bitexact(g, m, f"box {box} frame {f}")
+        pos, vel, iid = m["npos"], m["nvel"], m["iid"]

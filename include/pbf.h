@@ -177,7 +177,8 @@ int pbf_density_error(pbf_solver* s, const float* pos, int n, void* stream,
 #define PBF_PHASE_GRID 2    /* cell clear + permute + cell ranges */
 #define PBF_PHASE_LAMBDA 3  /* one lambda launch */
 #define PBF_PHASE_DELTA 4   /* one delta-p launch */
-#define PBF_PHASE_VEL 5     /* velocity update */
+#define PBF_PHASE_VEL 5     /* velocity update: fused into the last delta-p launch, so
+                               it records nothing of its own */
 #define PBF_PHASE_XSPH 6    /* XSPH viscosity */
 #define PBF_PHASE_STEP 7    /* whole step */
 #define PBF_NUM_PHASES 8

@@ -126,6 +126,7 @@ struct pbf_solver {
   uint32_t *key0 = nullptr, *val0 = nullptr, *key1 = nullptr, *val1 = nullptr;
   uint32_t *iid_s = nullptr, *hist = nullptr, *totals = nullptr;
   float4 *xp = nullptr, *x0s = nullptr, *xa = nullptr;
+  float4* vr = nullptr;  // (velocity before XSPH, rho of the last lambda pass)
   float* rho = nullptr;
   uint32_t* counters = nullptr;  // heavy-target work list counters
   pbf::HeavyLists hl{};
@@ -476,20 +477,18 @@ void enqueue_step(pbf_solver* s, const StepArrays& a, const pbf::StepConsts& c, 
     pbf::launch_lambda(s->xa, s->cs, n, c, scn, xl, rho, rec ? rec->sd[it] : nullptr, s->hl,
                        pass++, sh, st);
     te(PBF_PHASE_LAMBDA);
+    // the last iteration also does updateVelocity (Simulator.cpp:72,
+    // Simulator.cu:267-274) as each particle's final position is known
+    const pbf::VelocityOut vo{L,         s->x0s, rec ? rec->rho[NI - 1] : s->rho, s->iid_s,
+                              s->vr,     a.pos,  a.vel,                            a.iid};
     tb(PBF_PHASE_DELTA);
-    pbf::launch_delta(xl, s->cs, n, c, scn, s->xa, rec ? rec->dp[it] : nullptr, s->hl, pass++, sh,
-                      st);
+    pbf::launch_delta(xl, s->cs, n, c, scn, s->xa, rec ? rec->dp[it] : nullptr,
+                      it + 1 == NI ? &vo : nullptr, s->hl, pass++, sh, st);
     te(PBF_PHASE_DELTA);
   }
-  // updateVelocity (Simulator.cpp:72, Simulator.cu:267-274); vr aliases xp.
-  float4* vr = s->xp;
-  tb(PBF_PHASE_VEL);
-  pbf::launch_velocity(L, s->xa, s->x0s, s->iid_s, rec ? rec->rho[NI - 1] : s->rho, n, c, vr,
-                       a.pos, a.vel, a.iid, st);
-  te(PBF_PHASE_VEL);
   // correctVelocity (Simulator.cpp:77, Simulator.cu:251-265)
   tb(PBF_PHASE_XSPH);
-  pbf::launch_xsph(L, s->xa, vr, s->cs, n, c, scn, a.npos, a.nvel, s->hl, pass, sh, st);
+  pbf::launch_xsph(L, s->xa, s->vr, s->cs, n, c, scn, a.npos, a.nvel, s->hl, pass, sh, st);
   te(PBF_PHASE_XSPH);
   te(PBF_PHASE_STEP);
 }
@@ -588,7 +587,8 @@ int create_impl(const pbf_params& params, int capacity, long long cells, long lo
   const size_t o_key0 = carve(4 * N), o_val0 = carve(4 * N), o_key1 = carve(4 * N),
                o_val1 = carve(4 * N), o_iid = carve(4 * N), o_hist = carve(4 * 256 * tiles),
                o_tot = carve(4 * 256), o_xp = carve(16 * N), o_x0s = carve(16 * N),
-               o_xa = carve(16 * N), o_rho = carve(4 * N), o_recs = carve(16 * N),
+               o_xa = carve(16 * N), o_vr = carve(16 * N), o_rho = carve(4 * N),
+               o_recs = carve(16 * N),
                o_items = carve(16 * ICAP), o_part = carve(4 * pbf::kPartialWords * ICAP),
                o_cnt = carve(4 * 8), o_de = carve(8 * N), o_dp = carve(16 * dblk),
                o_do = carve(16), o_cnt2 = carve(4 * (size_t)max_cells),
@@ -613,6 +613,7 @@ int create_impl(const pbf_params& params, int capacity, long long cells, long lo
   s->xp = (float4*)(b + o_xp);
   s->x0s = (float4*)(b + o_x0s);
   s->xa = (float4*)(b + o_xa);
+  s->vr = (float4*)(b + o_vr);
   s->rho = (float*)(b + o_rho);
   s->counters = (uint32_t*)(b + o_cnt);
   s->hl.counters = s->counters;
@@ -1377,7 +1378,7 @@ int pbf_tape_forward(pbf_solver* s, pbf_tape* t, const float* pos, const float* 
         (e = hipMemcpyAsync(t->f_cs[fb], s->cs, ((size_t)c.cells + 1) * 4,
                             hipMemcpyDeviceToDevice, st)) != hipSuccess ||
         (e = hipMemcpyAsync(t->f_xa[fb], s->xa, v4, hipMemcpyDeviceToDevice, st)) != hipSuccess ||
-        (e = hipMemcpyAsync(t->f_vr[fb], s->xp, v4, hipMemcpyDeviceToDevice, st)) != hipSuccess)
+        (e = hipMemcpyAsync(t->f_vr[fb], s->vr, v4, hipMemcpyDeviceToDevice, st)) != hipSuccess)
       return fail_hip(s, e, "tape frame copy");
     if ((e = hipGetLastError()) != hipSuccess) return fail_hip(s, e, "kernel launch");
     s->cnt_dirty = false;
@@ -1444,7 +1445,7 @@ int pbf_tape_backward(pbf_solver* s, pbf_tape* t, const float* gpos, const float
       return rc;
     }
     const uint32_t* cs = s->cs;
-    const float4 *x_final = s->xa, *vr = s->xp;
+    const float4 *x_final = s->xa, *vr = s->vr;
     const uint32_t *skey = nullptr, *sval = nullptr;
     std::vector<pbf::BwdIter> iters((size_t)NI);
     if (t->store) {

@@ -62,12 +62,23 @@ void launch_permute_pos(const uint32_t* skey, const uint32_t* sval, const float4
 void launch_lambda(const float4* xa, const uint32_t* cs, int n, const StepConsts& c,
                    const Scenes& scn, float4* xl, float* rho, float4* sd, const HeavyLists& hl,
                    int pass, const Shapes& sh, hipStream_t st);
+// The end-of-step outputs of updateVelocity, fused into the last delta-p
+// launch: vr = (v, rho) for XSPH, the caller's pos (sorted start positions),
+// vel (velocities before XSPH) and iid (sorted identities).
+struct VelocityOut {
+  int layout;
+  const float4* x0s;
+  const float* rho;
+  const uint32_t* iid_s;
+  float4* vr;
+  float* pos;
+  float* vel;
+  uint32_t* iid;
+};
+// vo: null except on the step's last iteration.
 void launch_delta(const float4* xl, const uint32_t* cs, int n, const StepConsts& c,
-                  const Scenes& scn, float4* xa, float4* dp, const HeavyLists& hl, int pass,
-                  const Shapes& sh, hipStream_t st);
-void launch_velocity(int layout, const float4* xa, const float4* x0s, const uint32_t* iid_s,
-                     const float* rho, int n, const StepConsts& c, float4* vr, float* pos,
-                     float* vel, uint32_t* iid, hipStream_t st);
+                  const Scenes& scn, float4* xa, float4* dp, const VelocityOut* vo,
+                  const HeavyLists& hl, int pass, const Shapes& sh, hipStream_t st);
 void launch_xsph(int layout, const float4* xa, const float4* vr, const uint32_t* cs, int n,
                  const StepConsts& c, const Scenes& scn, float* npos, float* nvel,
                  const HeavyLists& hl, int pass, const Shapes& sh, hipStream_t st);

@@ -165,8 +165,9 @@ struct DeltaPass {
     tot.py = tot.py + p.py;
     tot.pz = tot.pz + p.pz;
   }
-  static __device__ __forceinline__ void finish(const Args& a, const Target& t, const Acc& s,
-                                                const StepConsts& c, const SceneGeo& g) {
+  // The new iterate (stored into xa); also records dp on a tape.
+  static __device__ __forceinline__ float3 update(const Args& a, const Target& t, const Acc& s,
+                                                  const StepConsts& c, const SceneGeo& g) {
     const float ux = s.px / c.pho0, uy = s.py / c.pho0, uz = s.pz / c.pho0;
     if (a.dp) a.dp[t.i] = make_float4(ux, uy, uz, 0.0f);
     const float px = fminf(fmaxf(ux, -c.max_dp), c.max_dp);
@@ -176,6 +177,41 @@ struct DeltaPass {
     const float y = fmaxf(fminf(t.x.y + py, g.hiy), g.loy);
     const float z = fmaxf(fminf(t.x.z + pz, g.hiz), g.loz);
     a.xa[t.i] = make_float4(x, y, z, 0.0f);
+    return make_float3(x, y, z);
+  }
+  static __device__ __forceinline__ void finish(const Args& a, const Target& t, const Acc& s,
+                                                const StepConsts& c, const SceneGeo& g) {
+    update(a, t, s, c, g);
+  }
+};
+
+// The last delta-p pass of a step, fused with updateVelocity
+// (fluids/Simulator.cu:127-137, :267-274): v = (x_final - x_start) * (1/dt),
+// packed with the last lambda pass's rho for XSPH, and the caller's pos / vel
+// / iid receive their end-of-step contents (sorted start positions, pre-XSPH
+// velocities, sorted identities).
+template <int L>
+struct DeltaLastPass : DeltaPass {
+  struct Args : DeltaPass::Args {
+    const float4* __restrict__ x0s;
+    const float* __restrict__ rho;
+    const uint32_t* __restrict__ iid_s;
+    float4* __restrict__ vr;
+    float* __restrict__ pos;
+    float* __restrict__ vel;
+    uint32_t* __restrict__ iid;
+  };
+  static __device__ __forceinline__ void finish(const Args& a, const Target& t, const Acc& s,
+                                                const StepConsts& c, const SceneGeo& g) {
+    const float3 x = update(a, t, s, c, g);
+    const float4 x0 = a.x0s[t.i];
+    const float vx = (x.x - x0.x) * c.inv_dt;
+    const float vy = (x.y - x0.y) * c.inv_dt;
+    const float vz = (x.z - x0.z) * c.inv_dt;
+    a.vr[t.i] = make_float4(vx, vy, vz, a.rho[t.i]);
+    store_vec<L>(a.vel, t.i, vx, vy, vz);
+    store_vec<L>(a.pos, t.i, x0.x, x0.y, x0.z);
+    a.iid[t.i] = a.iid_s[t.i];
   }
 };
 
@@ -550,29 +586,6 @@ __global__ __launch_bounds__(kBlock) void k_permute_pos(const uint32_t* __restri
   xa[s] = xp[sval[s] < (uint32_t)n ? sval[s] : 0u];
 }
 
-// h_updateVelocity (fluids/Simulator.cu:127-137, :267-274):
-// v = (x_final - x_start) * (1/dt).  Also writes the caller's pos / vel /
-// iid arrays with their end-of-step contents (sorted start positions,
-// pre-XSPH velocities, sorted identities) and packs (v, rho) for XSPH.
-template <int L>
-__global__ __launch_bounds__(kBlock) void k_velocity(
-    const float4* __restrict__ xa, const float4* __restrict__ x0s,
-    const uint32_t* __restrict__ iid_s, const float* __restrict__ rho, int n, StepConsts c,
-    float4* __restrict__ vr, float* __restrict__ pos, float* __restrict__ vel,
-    uint32_t* __restrict__ iid) {
-  const int s = blockIdx.x * kBlock + threadIdx.x;
-  if (s >= n) return;
-  const float4 x = xa[s];
-  const float4 x0 = x0s[s];
-  const float vx = (x.x - x0.x) * c.inv_dt;
-  const float vy = (x.y - x0.y) * c.inv_dt;
-  const float vz = (x.z - x0.z) * c.inv_dt;
-  vr[s] = make_float4(vx, vy, vz, rho[s]);
-  store_vec<L>(vel, s, vx, vy, vz);
-  store_vec<L>(pos, s, x0.x, x0.y, x0.z);
-  iid[s] = iid_s[s];
-}
-
 constexpr int kRedBlock = 256;
 constexpr int kRedItems = 16;  // particles per work-item in the first reduction
 
@@ -728,18 +741,20 @@ void launch_lambda(const float4* xa, const uint32_t* cs, int n, const StepConsts
 }
 
 void launch_delta(const float4* xl, const uint32_t* cs, int n, const StepConsts& c,
-                  const Scenes& scn, float4* xa, float4* dp, const HeavyLists& hl, int pass,
-                  const Shapes& sh, hipStream_t st) {
-  launch_pass<DeltaPass>(DeltaPass::Args{xl, xa, dp}, n, c, scn, cs, hl, pass, sh, st);
-}
-
-void launch_velocity(int layout, const float4* xa, const float4* x0s, const uint32_t* iid_s,
-                     const float* rho, int n, const StepConsts& c, float4* vr, float* pos,
-                     float* vel, uint32_t* iid, hipStream_t st) {
-  if (layout == 3)
-    k_velocity<3><<<blocks_for(n), kBlock, 0, st>>>(xa, x0s, iid_s, rho, n, c, vr, pos, vel, iid);
-  else
-    k_velocity<4><<<blocks_for(n), kBlock, 0, st>>>(xa, x0s, iid_s, rho, n, c, vr, pos, vel, iid);
+                  const Scenes& scn, float4* xa, float4* dp, const VelocityOut* vo,
+                  const HeavyLists& hl, int pass, const Shapes& sh, hipStream_t st) {
+  const DeltaPass::Args base{xl, xa, dp};
+  if (!vo) {
+    launch_pass<DeltaPass>(base, n, c, scn, cs, hl, pass, sh, st);
+  } else if (vo->layout == 3) {
+    const DeltaLastPass<3>::Args a{base, vo->x0s, vo->rho, vo->iid_s, vo->vr, vo->pos, vo->vel,
+                                   vo->iid};
+    launch_pass<DeltaLastPass<3>>(a, n, c, scn, cs, hl, pass, sh, st);
+  } else {
+    const DeltaLastPass<4>::Args a{base, vo->x0s, vo->rho, vo->iid_s, vo->vr, vo->pos, vo->vel,
+                                   vo->iid};
+    launch_pass<DeltaLastPass<4>>(a, n, c, scn, cs, hl, pass, sh, st);
+  }
 }
 
 void launch_xsph(int layout, const float4* xa, const float4* vr, const uint32_t* cs, int n,

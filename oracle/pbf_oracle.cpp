@@ -191,12 +191,10 @@ inline T dist2(const T* xi, const T* xj, T d[3]) {
 
 // Canonical summation order (shared with the GPU kernels, which must
 // reproduce it bit for bit): a target's candidates, in the reference's walk
-// order, are summed sequentially when there are at most kLightMax of them;
-// otherwise in consecutive chunks of kChunk candidates, each chunk summed
-// sequentially from zero and the chunk sums added in chunk order.  The
-// chunked form lets the GPU spread a crowded target's candidates over a
-// wavefront instead of one lane.
-constexpr size_t kLightMax = 256;
+// order, are summed in consecutive chunks of kChunk candidates, each chunk
+// summed sequentially from zero and the chunk sums added in chunk order (one
+// chunk: the plain sequential sum).  The chunked form lets the GPU spread a
+// crowded target's candidates over many work-items.
 constexpr size_t kChunk = 64;
 
 template <class T, int K>
@@ -208,10 +206,6 @@ template <class T, int K, class F>
 Acc<T, K> canonical_sum(const std::vector<uint32_t>& cand, F&& add) {
   Acc<T, K> total;
   const size_t m = cand.size();
-  if (m <= kLightMax) {
-    for (uint32_t j : cand) add(total, j);
-    return total;
-  }
   for (size_t c0 = 0; c0 < m; c0 += kChunk) {
     Acc<T, K> part;
     const size_t c1 = std::min(m, c0 + kChunk);

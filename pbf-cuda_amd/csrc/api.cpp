@@ -590,7 +590,7 @@ int create_impl(const pbf_params& params, int capacity, long long cells, long lo
                o_xa = carve(16 * N), o_vr = carve(16 * N), o_rho = carve(4 * N),
                o_recs = carve(16 * N),
                o_items = carve(16 * ICAP), o_part = carve(4 * pbf::kPartialWords * ICAP),
-               o_cnt = carve(4 * 8), o_de = carve(8 * N), o_dp = carve(16 * dblk),
+               o_cnt = carve(4 * pbf::kCounterWords), o_de = carve(8 * N), o_dp = carve(16 * dblk),
                o_do = carve(16), o_cnt2 = carve(4 * (size_t)max_cells),
                o_cs = carve(4 * ((size_t)max_cells + 1)),
                o_ts = carve(4 * ((size_t)pbf::scan_tiles(max_cells) + 1)),
@@ -629,8 +629,10 @@ int create_impl(const pbf_params& params, int capacity, long long cells, long lo
   s->tile_sum = (uint32_t*)(b + o_ts);
   s->geo_d = (pbf::SceneGeo*)(b + o_geo);
   s->sid_d = (uint16_t*)(b + o_sid);
-  // the per-cell counts start (and stay, between steps) all zero
+  // the per-cell counts start (and stay, between steps) all zero; the heavy
+  // threshold word starts at 0 (kLightMax)
   pbf::launch_fill_u32(s->cnt, max_cells, 0u, nullptr);
+  pbf::launch_fill_u32(s->counters, pbf::kCounterWords, 0u, nullptr);
   if ((e = hipDeviceSynchronize()) != hipSuccess) {
     (void)hipFree(arena);
     delete s;

@@ -97,15 +97,27 @@ __device__ __forceinline__ uint32_t cell_id(const Cell& q, const SceneGeo& g) {
 }
 
 // Canonical summation order, shared with the host oracle's FP32 mirror: a
-// target's candidate sequence is summed sequentially when it has at most
-// kLightMax candidates (the "light" kernels, one work-item per target);
-// otherwise in consecutive chunks of kChunk candidates, each summed
-// sequentially from zero, chunk sums added in chunk order (the "heavy"
-// kernels, one wavefront per target, one chunk per lane).  Crowded cells
-// (particles piled into a corner) then cost one wavefront a few chunk
-// lengths instead of one lane thousands of dependent gathers.
-constexpr uint32_t kLightMax = 256;
+// target's candidate sequence is summed in consecutive chunks of kChunk
+// candidates, each summed sequentially from zero, the chunk sums added in
+// chunk order (one chunk: the plain sequential sum).  The order does not
+// depend on who computes the sum, so the kernels may split the work freely:
+// a light target's work-item walks all its chunks; a heavy target's chunks
+// go to one work-item each (the chunk kernel) and are added up by the
+// combine kernel.  Crowded cells (particles piled into a corner) then cost a
+// few chunk lengths instead of one lane thousands of dependent gathers.
 constexpr uint32_t kChunk = 64;
+// A target is heavy when it has more than kLightMax candidates, or more than
+// kHeavyMin when the previous neighbour pass made fewer than kSparseItems
+// chunk items per kSparseParticles targets.  In sparse water the few long
+// lanes of mid-sized targets set the light kernel's tail and are better
+// spread over the chunk kernel; in dense water they are common, balance each
+// other within the workgroups, and would only lengthen the chunk lists.
+// Which targets are heavy changes no result (the canonical order above),
+// only the timing.
+constexpr uint32_t kLightMax = 256;
+constexpr uint32_t kHeavyMin = 128;
+constexpr uint32_t kSparseItems = 2;
+constexpr uint32_t kSparseParticles = 5;
 
 // A work-item's candidate sequence as a compact column table in LDS: entry
 // k < nk is (first slot of the k-th non-empty column, sequence position of
@@ -262,10 +274,10 @@ struct SeqCursor1 {
 // evaluated and masked), so the compiler keeps the eight loads in flight
 // together instead of sinking some of them into conditional blocks that
 // wait for each load in turn; hit() must be branch-free as well.
-template <int G, class LoadS, class LoadF, class Hit, class Add>
+template <int G, class LoadS, class LoadF, class Hit, class Add, class BlockEnd>
 __device__ __forceinline__ void sweep_seq(const SeqTable& st, uint32_t m, uint32_t u0,
                                           uint32_t u1, LoadS&& load_s, LoadF&& load_f, Hit&& hit,
-                                          Add&& add) {
+                                          Add&& add, BlockEnd&& block_end) {
   if (u0 >= u1) return;
   SeqCursor c1;
   c1.init(st, u0, m);
@@ -309,6 +321,7 @@ __device__ __forceinline__ void sweep_seq(const SeqTable& st, uint32_t m, uint32
       for (int t = 0; t < G; ++t)
         if (h[t]) add(j[t], d[t]);
     }
+    block_end();
   }
 }
 
@@ -328,13 +341,19 @@ __device__ __forceinline__ uint32_t wave_exclusive_sum(uint32_t v, uint32_t* tot
 
 // Work lists of the heavy targets of one neighbour pass.
 struct HeavyLists {
-  uint32_t* counters;  // [2][4]: per pass parity {records, chunk items, -, -}
+  // [2][4]: per pass parity {records, chunk items, -, -}; [kThresholdWord]:
+  // the heavy threshold for the next pass (0: kLightMax), set by each combine
+  // kernel, on a cache line of its own (every light wavefront reads it while
+  // others add to the counters)
+  uint32_t* counters;
   uint4* recs;         // per heavy target: (i, first partial slot, chunks, 0)
   uint4* items;        // per chunk: (i, chunk index, partial slot, 0); i = ~0u: unused
   float* partials;     // kPartialWords floats per partial slot
   uint32_t item_cap;   // capacity of items / partials
 };
 constexpr int kPartialWords = 8;
+constexpr int kThresholdWord = 64;
+constexpr int kCounterWords = 2 * kThresholdWord;
 
 // x^n for the s_corr exponent: repeated multiplication for small integer n
 // (the reference default n=4), powf otherwise.

@@ -14,8 +14,8 @@
 // Each neighbour pass is three kernels:
 //   light   one work-item per target over the cell-sorted order; workgroups
 //           remapped so each XCD's L2 serves one contiguous particle range; a
-//           target with more than kLightMax candidates is appended to the
-//           heavy list instead;
+//           heavy target (pbf_device.h: kLightMax, kHeavyMin, kHeavyFactor)
+//           is appended to the heavy list instead;
 //   chunks  one work-item per kChunk-candidate chunk of the heavy targets;
 //   combine one work-item per heavy target, adding its chunk sums in order.
 // Both follow the canonical summation order of pbf_device.h.
@@ -37,9 +37,9 @@ inline int blocks_for(long long n) { return (int)((n + kBlock - 1) / kBlock); }
 // ---------------------------------------------------------------------------
 
 // Light: one work-item per target; the workgroup size is a launch shape
-// (64, 128 or 256), the column tables live in dynamic LDS.  A target with
-// more than kLightMax candidates is heavy: it gets one chunk item per kChunk
-// candidates and a record, appended with one atomic per wavefront each.
+// (64, 128 or 256), the column tables live in dynamic LDS.  A heavy target
+// gets one chunk item per kChunk candidates and a record, appended with one
+// atomic per wavefront each.
 // counters[pass & 1] belong to this pass; the kernel clears the other
 // parity's counters for the next pass (their previous users finished earlier
 // on the stream).  If the chunk list is full the lane sums its target itself,
@@ -54,6 +54,7 @@ __global__ __launch_bounds__(kBlock) void k_light(typename P::Args a, int n, Ste
   __shared__ uint32_t bucket_sh[kRankBuckets];
   uint32_t* cnt = hl.counters + 4 * (pass & 1);
   if (blockIdx.x == 0 && threadIdx.x < 4) hl.counters[4 * ((pass + 1) & 1) + threadIdx.x] = 0u;
+  const uint32_t thr = hl.counters[kThresholdWord] ? hl.counters[kThresholdWord] : kLightMax;
   const int tid = threadIdx.x;
   const int nb = (int)blockDim.x;
   const int block_base = xcd_block_index() * nb;
@@ -67,7 +68,7 @@ __global__ __launch_bounds__(kBlock) void k_light(typename P::Args a, int n, Ste
     const SceneGeo g = sc.at((uint32_t)i, c);
     m = build_seq(st, cell_of(t.x.x, t.x.y, t.x.z, g, c.h), cs, g);
   }
-  const bool is_heavy = active && m > kLightMax;
+  const bool is_heavy = active && m > thr;
   const uint32_t nch = is_heavy ? (m + kChunk - 1) / kChunk : 0u;
   bool in_lane = false;  // chunk list full: this lane sums its chunks itself
   if (__ballot(is_heavy) != 0ull) {
@@ -144,22 +145,16 @@ __global__ __launch_bounds__(kBlock) void k_light(typename P::Args a, int n, Ste
   auto ls = [&](uint32_t j) { return P::load_s(a, j); };
   auto lf = [&](uint32_t j) { return P::load_f(a, j); };
   auto hit = [&](uint32_t j, const auto& d) { return P::hit(t, j, d, c); };
-  typename P::Acc s;
-  if (ms <= kLightMax) {
-    sweep_seq<P::kHitBatch>(sts, ms, 0u, ms, ls, lf, hit,
-                            [&](uint32_t j, const auto& d) { P::add(s, t, j, d, c, a); });
-  } else {
-    // chunk list full: this lane sums the target's chunks in order
-    const uint32_t nc = (ms + kChunk - 1) / kChunk;
-    for (uint32_t ch = 0; ch < nc; ++ch) {
-      typename P::Acc part;
-      const uint32_t s0 = ch * kChunk;
-      const uint32_t s1 = s0 + kChunk < ms ? s0 + kChunk : ms;
-      sweep_seq<P::kHitBatch>(sts, ms, s0, s1, ls, lf, hit,
-                [&](uint32_t j, const auto& d) { P::add(part, t, j, d, c, a); });
-      P::accumulate(s, part);
-    }
-  }
+  // the sweep's blocks are the canonical chunks: each is summed from zero
+  // and added to the total in order (also for a heavy target whose chunk
+  // list was full)
+  typename P::Acc s, part;
+  sweep_seq<P::kHitBatch>(
+      sts, ms, 0u, ms, ls, lf, hit, [&](uint32_t j, const auto& d) { P::add(part, t, j, d, c, a); },
+      [&] {
+        P::accumulate(s, part);
+        part = typename P::Acc{};
+      });
   P::finish(a, t, s, c, sc.at(ti, c));
 }
 
@@ -188,7 +183,7 @@ __global__ __launch_bounds__(kBlock) void k_chunks(typename P::Args a, StepConst
         st, m, s0, s1, [&](uint32_t j) { return P::load_s(a, j); },
         [&](uint32_t j) { return P::load_f(a, j); },
         [&](uint32_t j, const auto& d) { return P::hit(t, j, d, c); },
-        [&](uint32_t j, const auto& d) { P::add(part, t, j, d, c, a); });
+        [&](uint32_t j, const auto& d) { P::add(part, t, j, d, c, a); }, [] {});
     static_assert(sizeof(part) <= kPartialWords * sizeof(float), "partial too large");
     *reinterpret_cast<typename P::Acc*>(hl.partials + (size_t)it.z * kPartialWords) = part;
   }
@@ -197,9 +192,14 @@ __global__ __launch_bounds__(kBlock) void k_chunks(typename P::Args a, StepConst
 // One work-item per heavy target: chunk partials added in chunk order.
 template <class P, class S>
 __global__ __launch_bounds__(kBlock) void k_combine(typename P::Args a, StepConsts c, S sc,
-                                                    HeavyLists hl, int pass) {
+                                                    HeavyLists hl, int pass, int n) {
   const uint32_t* cnt = hl.counters + 4 * (pass & 1);
   const uint32_t nrec = cnt[0];
+  if (blockIdx.x == 0 && threadIdx.x == 0)  // the next pass's heavy threshold
+    hl.counters[kThresholdWord] = (unsigned long long)cnt[1] * kSparseParticles <
+                             (unsigned long long)n * kSparseItems
+                         ? kHeavyMin
+                         : kLightMax;
   const uint32_t stride = gridDim.x * kBlock;
   for (uint32_t r = blockIdx.x * kBlock + threadIdx.x; r < nrec; r += stride) {
     const uint4 rec = hl.recs[r];
@@ -237,7 +237,7 @@ void launch_pass_s(const typename P::Args& a, int n, const StepConsts& c, const 
   k_light<P, S><<<(n + lb - 1) / lb, lb, 10 * lb * sizeof(uint2), st>>>(a, n, c, sc, cs, hl, pass);
   k_chunks<P, S><<<grid_for_items(hl.item_cap, sh.chunk_blocks), kBlock, 0, st>>>(a, c, sc, cs,
                                                                                     hl, pass);
-  k_combine<P, S><<<grid_for_items(n, 512), kBlock, 0, st>>>(a, c, sc, hl, pass);
+  k_combine<P, S><<<grid_for_items(n, 512), kBlock, 0, st>>>(a, c, sc, hl, pass, n);
 }
 
 template <class P>

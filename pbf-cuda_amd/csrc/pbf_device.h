@@ -341,10 +341,12 @@ __device__ __forceinline__ uint32_t wave_exclusive_sum(uint32_t v, uint32_t* tot
 
 // Work lists of the heavy targets of one neighbour pass.
 struct HeavyLists {
-  // [2][4]: per pass parity {records, chunk items, -, -}; [kThresholdWord]:
-  // the heavy threshold for the next pass (0: kLightMax), set by each combine
-  // kernel, on a cache line of its own (every light wavefront reads it while
-  // others add to the counters)
+  // per pass parity: the record and chunk-item counters (rec_counter,
+  // item_counter); counters[kThresholdWord]: the heavy threshold for the next
+  // pass (0: kLightMax), set by each combine kernel.  Each word has a
+  // 256-byte line of its own: wavefronts add to the counters with atomics
+  // while every light wavefront reads the threshold, and atomics on one line
+  // serialise
   uint32_t* counters;
   uint4* recs;         // per heavy target: (i, first partial slot, chunks, 0)
   uint4* items;        // per chunk: (i, chunk index, partial slot, 0); i = ~0u: unused
@@ -352,8 +354,15 @@ struct HeavyLists {
   uint32_t item_cap;   // capacity of items / partials
 };
 constexpr int kPartialWords = 8;
-constexpr int kThresholdWord = 64;
-constexpr int kCounterWords = 2 * kThresholdWord;
+constexpr int kCounterLine = 64;  // words
+constexpr int kThresholdWord = 4 * kCounterLine;
+constexpr int kCounterWords = 5 * kCounterLine;
+__host__ __device__ __forceinline__ int rec_counter(int parity) {
+  return 2 * parity * kCounterLine;
+}
+__host__ __device__ __forceinline__ int item_counter(int parity) {
+  return (2 * parity + 1) * kCounterLine;
+}
 
 // x^n for the s_corr exponent: repeated multiplication for small integer n
 // (the reference default n=4), powf otherwise.

@@ -40,8 +40,8 @@ inline int blocks_for(long long n) { return (int)((n + kBlock - 1) / kBlock); }
 // (64, 128 or 256), the column tables live in dynamic LDS.  A heavy target
 // gets one chunk item per kChunk candidates and a record, appended with one
 // atomic per wavefront each.
-// counters[pass & 1] belong to this pass; the kernel clears the other
-// parity's counters for the next pass (their previous users finished earlier
+// The counters of parity pass & 1 belong to this pass; the kernel clears the
+// other parity's counters for the next pass (their previous users finished earlier
 // on the stream).  If the chunk list is full the lane sums its target itself,
 // still chunk by chunk, so the canonical order holds either way.
 template <class P, class S>
@@ -52,8 +52,12 @@ __global__ __launch_bounds__(kBlock) void k_light(typename P::Args a, int n, Ste
   __shared__ uint32_t key_sh[kBlock];
   __shared__ uint32_t order_sh[kBlock];
   __shared__ uint32_t bucket_sh[kRankBuckets];
-  uint32_t* cnt = hl.counters + 4 * (pass & 1);
-  if (blockIdx.x == 0 && threadIdx.x < 4) hl.counters[4 * ((pass + 1) & 1) + threadIdx.x] = 0u;
+  uint32_t* rec_cnt = hl.counters + rec_counter(pass & 1);
+  uint32_t* item_cnt = hl.counters + item_counter(pass & 1);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    hl.counters[rec_counter((pass + 1) & 1)] = 0u;
+    hl.counters[item_counter((pass + 1) & 1)] = 0u;
+  }
   const uint32_t thr = hl.counters[kThresholdWord] ? hl.counters[kThresholdWord] : kLightMax;
   const int tid = threadIdx.x;
   const int nb = (int)blockDim.x;
@@ -81,7 +85,7 @@ __global__ __launch_bounds__(kBlock) void k_light(typename P::Args a, int n, Ste
     uint32_t item_total;
     const uint32_t item_off = wave_exclusive_sum(nch, &item_total);
     uint32_t item_base = 0, rec_base = 0;
-    if ((threadIdx.x & 63) == 0) item_base = atomicAdd(&cnt[1], item_total);
+    if ((threadIdx.x & 63) == 0) item_base = atomicAdd(item_cnt, item_total);
     item_base = __shfl(item_base, 0);
     const uint32_t first = item_base + item_off;
     const bool wave_fits = item_base + item_total <= hl.item_cap;
@@ -104,7 +108,7 @@ __global__ __launch_bounds__(kBlock) void k_light(typename P::Args a, int n, Ste
     }
     const uint64_t hm = __ballot(is_heavy);
     if (wave_fits) {
-      if ((threadIdx.x & 63) == 0) rec_base = atomicAdd(&cnt[0], (uint32_t)__popcll(hm));
+      if ((threadIdx.x & 63) == 0) rec_base = atomicAdd(rec_cnt, (uint32_t)__popcll(hm));
       rec_base = __shfl(rec_base, 0);
       if (is_heavy)
         hl.recs[rec_base + (uint32_t)__popcll(hm & lt)] = make_uint4((uint32_t)i, first, nch, 0u);
@@ -166,8 +170,8 @@ __global__ __launch_bounds__(kBlock) void k_chunks(typename P::Args a, StepConst
                                                    const uint32_t* __restrict__ cs, HeavyLists hl,
                                                    int pass) {
   __shared__ uint2 seq[10][kBlock];
-  const uint32_t* cnt = hl.counters + 4 * (pass & 1);
-  const uint32_t total = cnt[1] < hl.item_cap ? cnt[1] : hl.item_cap;
+  const uint32_t items = hl.counters[item_counter(pass & 1)];
+  const uint32_t total = items < hl.item_cap ? items : hl.item_cap;
   const uint32_t stride = gridDim.x * kBlock;
   const SeqTable st{&seq[0][threadIdx.x], kBlock};
   for (uint32_t u = (uint32_t)xcd_block_index() * kBlock + threadIdx.x; u < total; u += stride) {
@@ -193,10 +197,10 @@ __global__ __launch_bounds__(kBlock) void k_chunks(typename P::Args a, StepConst
 template <class P, class S>
 __global__ __launch_bounds__(kBlock) void k_combine(typename P::Args a, StepConsts c, S sc,
                                                     HeavyLists hl, int pass, int n) {
-  const uint32_t* cnt = hl.counters + 4 * (pass & 1);
-  const uint32_t nrec = cnt[0];
+  const uint32_t nrec = hl.counters[rec_counter(pass & 1)];
+  const uint32_t items = hl.counters[item_counter(pass & 1)];
   if (blockIdx.x == 0 && threadIdx.x == 0)  // the next pass's heavy threshold
-    hl.counters[kThresholdWord] = (unsigned long long)cnt[1] * kSparseParticles <
+    hl.counters[kThresholdWord] = (unsigned long long)items * kSparseParticles <
                              (unsigned long long)n * kSparseItems
                          ? kHeavyMin
                          : kLightMax;

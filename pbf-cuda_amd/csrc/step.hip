@@ -375,7 +375,10 @@ __global__ __launch_bounds__(kBlock) void k_predict_key(const float* __restrict_
                                                         uint32_t* __restrict__ val,
                                                         uint32_t* __restrict__ counters) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i < 4) counters[i] = 0u;
+  if (i == 0) {
+    counters[rec_counter(0)] = 0u;
+    counters[item_counter(0)] = 0u;
+  }
   if (blockIdx.x * kBlock >= n) return;  // whole workgroup past the end
   const int ii = i < n ? i : n - 1;      // idle lanes repeat the last particle
   const float4 p = load_vec<L>(pos, ii);
@@ -402,7 +405,10 @@ __global__ __launch_bounds__(kBlock) void k_key_only(const float* __restrict__ p
                                                      uint32_t* __restrict__ val,
                                                      uint32_t* __restrict__ counters) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i < 4) counters[i] = 0u;
+  if (i == 0) {
+    counters[rec_counter(0)] = 0u;
+    counters[item_counter(0)] = 0u;
+  }
   if (blockIdx.x * kBlock >= n) return;
   const int ii = i < n ? i : n - 1;
   const float4 p = load_vec<L>(pos, ii);

@@ -71,6 +71,17 @@ __device__ __forceinline__ int xcd_block_index() {
   return x * per + (x < rem ? x : rem) + (p >> 3);
 }
 
+// The same remap over the first `active` blocks of the grid only (blocks
+// from `active` on get -1).  For a grid-stride kernel whose work fills only a
+// prefix of the grid: remapping over the whole grid would hand that prefix to
+// the first XCDs and leave the others idle.
+__device__ __forceinline__ int xcd_block_index_of(int active) {
+  const int p = (int)blockIdx.x;
+  if (p >= active) return -1;
+  const int per = active >> 3, rem = active & 7, x = p & 7;
+  return x * per + (x < rem ? x : rem) + (p >> 3);
+}
+
 // One axis of the cell coordinate, (int)((p - llim)/h) clamped to
 // [0, dim-1] in FP32 (fluids/Simulator.cu:30-36).  Clamping the quotient
 // before truncating gives the same integer for every finite quotient and

@@ -172,9 +172,15 @@ __global__ __launch_bounds__(kBlock) void k_chunks(typename P::Args a, StepConst
   __shared__ uint2 seq[10][kBlock];
   const uint32_t items = hl.counters[item_counter(pass & 1)];
   const uint32_t total = items < hl.item_cap ? items : hl.item_cap;
-  const uint32_t stride = gridDim.x * kBlock;
+  // only as many workgroups as there are items to go round, spread over all
+  // XCDs, each XCD on one contiguous range of the (chunk-major) item list
+  const uint32_t need = (total + kBlock - 1) / kBlock;
+  const int active = need < gridDim.x ? (int)need : (int)gridDim.x;
+  const int vb = xcd_block_index_of(active);
+  if (vb < 0) return;
+  const uint32_t stride = (uint32_t)active * kBlock;
   const SeqTable st{&seq[0][threadIdx.x], kBlock};
-  for (uint32_t u = (uint32_t)xcd_block_index() * kBlock + threadIdx.x; u < total; u += stride) {
+  for (uint32_t u = (uint32_t)vb * kBlock + threadIdx.x; u < total; u += stride) {
     const uint4 it = hl.items[u];
     if (it.x == ~0u) continue;
     const typename P::Target t = P::target(a, it.x, c);

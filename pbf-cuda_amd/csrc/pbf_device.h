@@ -118,17 +118,16 @@ __device__ __forceinline__ uint32_t cell_id(const Cell& q, const SceneGeo& g) {
 // few chunk lengths instead of one lane thousands of dependent gathers.
 constexpr uint32_t kChunk = 64;
 // A target is heavy when it has more than kLightMax candidates, or more than
-// kHeavyMin when the previous neighbour pass made fewer than kSparseItems
-// chunk items per kSparseParticles targets.  In sparse water the few long
-// lanes of mid-sized targets set the light kernel's tail and are better
-// spread over the chunk kernel; in dense water they are common, balance each
-// other within the workgroups, and would only lengthen the chunk lists.
-// Which targets are heavy changes no result (the canonical order above),
-// only the timing.
+// kHeavyMin when, in the previous neighbour pass, targets with kHeavyMin < m
+// <= kLightMax held less than 1/kMidShare of all candidates.  In sparse water
+// the few long lanes of such mid-sized targets set the light kernel's tail
+// and are better spread over the chunk kernel; in dense water they are
+// common, balance each other within the workgroups, and would only lengthen
+// the chunk lists.  Which targets are heavy changes no result (the canonical
+// order above), only the timing.
 constexpr uint32_t kLightMax = 256;
 constexpr uint32_t kHeavyMin = 128;
-constexpr uint32_t kSparseItems = 2;
-constexpr uint32_t kSparseParticles = 5;
+constexpr unsigned long long kMidShare = 6;
 
 // A work-item's candidate sequence as a compact column table in LDS: entry
 // k < nk is (first slot of the k-th non-empty column, sequence position of
@@ -354,10 +353,11 @@ __device__ __forceinline__ uint32_t wave_exclusive_sum(uint32_t v, uint32_t* tot
 struct HeavyLists {
   // per pass parity: the record and chunk-item counters (rec_counter,
   // item_counter); counters[kThresholdWord]: the heavy threshold for the next
-  // pass (0: kLightMax), set by each combine kernel.  Each word has a
-  // 256-byte line of its own: wavefronts add to the counters with atomics
-  // while every light wavefront reads the threshold, and atomics on one line
-  // serialise
+  // pass (0: kLightMax), set by each combine kernel from the kStatSlots
+  // candidate statistics {mid-sized, all} (stat_slot) the light kernel's
+  // wavefronts add to.  Each word or slot has a 256-byte line of its own:
+  // atomics on one line serialise, and every light wavefront reads the
+  // threshold
   uint32_t* counters;
   uint4* recs;         // per heavy target: (i, first partial slot, chunks, 0)
   uint4* items;        // per chunk: (i, chunk index, partial slot, 0); i = ~0u: unused
@@ -367,7 +367,10 @@ struct HeavyLists {
 constexpr int kPartialWords = 8;
 constexpr int kCounterLine = 64;  // words
 constexpr int kThresholdWord = 4 * kCounterLine;
-constexpr int kCounterWords = 5 * kCounterLine;
+constexpr int kStatSlots = 64;
+constexpr int kCounterWords = (5 + kStatSlots) * kCounterLine;
+// The {mid-sized candidates, all candidates} pair of statistics slot k.
+__host__ __device__ __forceinline__ int stat_slot(int k) { return (5 + k) * kCounterLine; }
 __host__ __device__ __forceinline__ int rec_counter(int parity) {
   return 2 * parity * kCounterLine;
 }

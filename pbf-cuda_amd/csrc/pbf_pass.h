@@ -72,6 +72,23 @@ __global__ __launch_bounds__(kBlock) void k_light(typename P::Args a, int n, Ste
     const SceneGeo g = sc.at((uint32_t)i, c);
     m = build_seq(st, cell_of(t.x.x, t.x.y, t.x.z, g, c.h), cs, g);
   }
+  {
+    // this pass's statistics for the next pass's threshold
+    unsigned long long mid = m > kHeavyMin && m <= kLightMax ? m : 0u;
+    unsigned long long all = m;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      mid += __shfl_xor(mid, off);
+      all += __shfl_xor(all, off);
+    }
+    if ((threadIdx.x & 63) == 0 && all) {
+      const int wave = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+      unsigned long long* slot =
+          reinterpret_cast<unsigned long long*>(hl.counters + stat_slot(wave % kStatSlots));
+      atomicAdd(&slot[0], mid);
+      atomicAdd(&slot[1], all);
+    }
+  }
   const bool is_heavy = active && m > thr;
   const uint32_t nch = is_heavy ? (m + kChunk - 1) / kChunk : 0u;
   bool in_lane = false;  // chunk list full: this lane sums its chunks itself
@@ -202,14 +219,23 @@ __global__ __launch_bounds__(kBlock) void k_chunks(typename P::Args a, StepConst
 // One work-item per heavy target: chunk partials added in chunk order.
 template <class P, class S>
 __global__ __launch_bounds__(kBlock) void k_combine(typename P::Args a, StepConsts c, S sc,
-                                                    HeavyLists hl, int pass, int n) {
+                                                    HeavyLists hl, int pass) {
   const uint32_t nrec = hl.counters[rec_counter(pass & 1)];
-  const uint32_t items = hl.counters[item_counter(pass & 1)];
-  if (blockIdx.x == 0 && threadIdx.x == 0)  // the next pass's heavy threshold
-    hl.counters[kThresholdWord] = (unsigned long long)items * kSparseParticles <
-                             (unsigned long long)n * kSparseItems
-                         ? kHeavyMin
-                         : kLightMax;
+  if (blockIdx.x == 0 && threadIdx.x < 64) {
+    // the next pass's heavy threshold from this pass's statistics, which are
+    // cleared for the next pass
+    unsigned long long* slot =
+        reinterpret_cast<unsigned long long*>(hl.counters + stat_slot((int)threadIdx.x));
+    unsigned long long mid = slot[0], all = slot[1];
+    slot[0] = 0ull;
+    slot[1] = 0ull;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      mid += __shfl_xor(mid, off);
+      all += __shfl_xor(all, off);
+    }
+    if (threadIdx.x == 0) hl.counters[kThresholdWord] = mid * kMidShare < all ? kHeavyMin : kLightMax;
+  }
   const uint32_t stride = gridDim.x * kBlock;
   for (uint32_t r = blockIdx.x * kBlock + threadIdx.x; r < nrec; r += stride) {
     const uint4 rec = hl.recs[r];
@@ -247,7 +273,7 @@ void launch_pass_s(const typename P::Args& a, int n, const StepConsts& c, const 
   k_light<P, S><<<(n + lb - 1) / lb, lb, 10 * lb * sizeof(uint2), st>>>(a, n, c, sc, cs, hl, pass);
   k_chunks<P, S><<<grid_for_items(hl.item_cap, sh.chunk_blocks), kBlock, 0, st>>>(a, c, sc, cs,
                                                                                     hl, pass);
-  k_combine<P, S><<<grid_for_items(n, 512), kBlock, 0, st>>>(a, c, sc, hl, pass, n);
+  k_combine<P, S><<<grid_for_items(n, 512), kBlock, 0, st>>>(a, c, sc, hl, pass);
 }
 
 template <class P>

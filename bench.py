@@ -446,13 +446,35 @@ def main():
                 br.sim.set_graph(True)
             ms_b = timed(br, args.steps, stream)
         rate = br.n * p.niter / (ms_b * 1e-3)
+        del br
+        # the same 64 scenes as 64 single-scene solvers stepped one after the
+        # other on the stream (same frames, same shapes, graph mode): what
+        # batching saves
+        runs = [SingleRun(pbf, p, sc_k, dev) for sc_k in scs]
+        for r in runs:
+            r.sim.set_shapes(*bshapes)
+            r.sim.set_graph(not args.no_graph)
+
+        class _All:
+            def step(self, st):
+                for r in runs:
+                    r.step(st)
+
+        with torch.cuda.stream(stream):
+            for _ in range(args.warmup):
+                _All().step(stream)
+            ms_seq = timed(_All(), args.steps, stream)
+        del runs
         batched = {"value": rate, "unit": "particle-solver-iters/s", "ms_per_step": ms_b,
-                   "scenes": len(scs), "particles": br.n, "frac_of_single": rate / value,
+                   "scenes": len(scs), "particles": int(sum(x.n for x in scs)),
+                   "frac_of_single": rate / value,
+                   "one_solver_per_scene_ms_per_step": ms_seq,
+                   "speedup_over_one_solver_per_scene": ms_seq / ms_b,
                    "shapes": list(bshapes), "graph": not args.no_graph,
                    "workload": "64 box scenes, counts log-uniform in [4096, 262144] (seeded), "
                                "fill 0.2-0.5, one jittered lattice block each; same constants"}
-        print(f"batched: {ms_b:.4f} ms/step, {rate:.4g} /s", file=sys.stderr, flush=True)
-        del br
+        print(f"batched: {ms_b:.4f} ms/step, {rate:.4g} /s; one solver per scene "
+              f"{ms_seq:.3f} ms/step", file=sys.stderr, flush=True)
 
     # one GPU's share of an 8-way x-slab split of the single dam (its state
     # after the timed frames) and of the 4M sweep dam (its initial state)

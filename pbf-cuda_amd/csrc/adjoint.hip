@@ -94,14 +94,44 @@ __device__ __forceinline__ float dot3(const float3& a, const float3& b) {
 }
 __device__ __forceinline__ float3 xyz(const float4& v) { return make_float3(v.x, v.y, v.z); }
 
-__device__ __forceinline__ void atomic_add3(float4* p, const float3& v) {
-  atomicAdd(&p->x, v.x);
-  atomicAdd(&p->y, v.y);
-  atomicAdd(&p->z, v.z);
+// The scattered pair sums: kScatterCopies copies of n float4 entries, stored
+// component-major ([component][copy][n] floats) so the four atomic adds of
+// one scatter land on four cache lines.  A workgroup adds to copy
+// blockIdx.x % kScatterCopies.
+__device__ __forceinline__ size_t scatter_copy(uint32_t n) {
+  return (size_t)(blockIdx.x % kScatterCopies) * n;
+}
+// Adds (x, y, z[, w]) at copy offset e (scatter_copy(n) + j).
+__device__ __forceinline__ void scatter_add(float4* ex, uint32_t n, size_t e, float x, float y,
+                                            float z, float w, bool with_w) {
+  float* f = reinterpret_cast<float*>(ex);
+  const size_t cs = (size_t)kScatterCopies * n;
+  atomicAdd(&f[e], x);
+  atomicAdd(&f[cs + e], y);
+  atomicAdd(&f[2 * cs + e], z);
+  if (with_w) atomicAdd(&f[3 * cs + e], w);
+}
+// Sum of the copies of entry i (in copy order), which are cleared.
+__device__ __forceinline__ float4 take_scattered(float4* __restrict__ ex, int i, int n) {
+  float* f = reinterpret_cast<float*>(ex);
+  const size_t cs = (size_t)kScatterCopies * n;
+  float v[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int k = 0; k < kScatterCopies; ++k) {
+      const size_t idx = q * cs + (size_t)k * n + i;
+      v[q] = v[q] + f[idx];
+      f[idx] = 0.0f;
+    }
+  return make_float4(v[0], v[1], v[2], v[3]);
 }
 
 // ---- XSPH -----------------------------------------------------------------
 struct BwdXsphPass {
+  // workgroups in launch order, round-robin over the XCDs: a crowded cell's
+  // targets then scatter from all eight XCDs, each into its own copy
+  static constexpr bool kXcdRanges = false;
   // hits reloaded per round of the sweep
   static constexpr int kHitBatch = 4;
   struct Args {
@@ -111,8 +141,9 @@ struct BwdXsphPass {
     const uint32_t* __restrict__ key;  // sorted keys
     float4* __restrict__ out_x;      // gathered (dL/dx, dL/drho)
     float4* __restrict__ out_v;      // gathered dL/dv (pair terms)
-    float4* __restrict__ ex_x;       // scattered (dL/dx, dL/drho)
-    float4* __restrict__ ex_v;       // scattered dL/dv
+    float4* __restrict__ ex_x;       // scattered (dL/dx, dL/drho), kScatterCopies x ex_n
+    float4* __restrict__ ex_v;       // scattered dL/dv, kScatterCopies x ex_n
+    uint32_t ex_n;
   };
   struct Target {
     float4 x, vr, gv;
@@ -176,9 +207,9 @@ struct BwdXsphPass {
       s.r -= kr * uj;
     } else if (j != t.i) {
       // j never visits i: deliver the own term's derivatives about j
-      atomic_add3(&a.ex_v[j], make_float3(f * B.x, f * B.y, f * B.z));
-      atomic_add3(&a.ex_x[j], make_float3(-fx * P.x, -fx * P.y, -fx * P.z));
-      atomicAdd(&a.ex_x[j].w, -fr);
+      const size_t e = scatter_copy(a.ex_n) + j;
+      scatter_add(a.ex_v, a.ex_n, e, f * B.x, f * B.y, f * B.z, 0.0f, false);
+      scatter_add(a.ex_x, a.ex_n, e, -fx * P.x, -fx * P.y, -fx * P.z, -fr, true);
     }
   }
   static __device__ __forceinline__ void accumulate(Acc& tot, const Acc& p) {
@@ -199,6 +230,9 @@ struct BwdXsphPass {
 
 // ---- delta-p ----------------------------------------------------------------
 struct BwdDeltaPass {
+  // workgroups in launch order, round-robin over the XCDs: a crowded cell's
+  // targets then scatter from all eight XCDs, each into its own copy
+  static constexpr bool kXcdRanges = false;
   // hits reloaded per round of the sweep
   static constexpr int kHitBatch = 4;
   struct Args {
@@ -206,7 +240,8 @@ struct BwdDeltaPass {
     const float4* __restrict__ A;    // adjoint of the correction sum (before / pho0)
     const uint32_t* __restrict__ key;
     float4* __restrict__ out;        // gathered (dL/dx, dL/dlambda)
-    float4* __restrict__ ex;         // scattered (dL/dx, dL/dlambda)
+    float4* __restrict__ ex;         // scattered (dL/dx, dL/dlambda), kScatterCopies x ex_n
+    uint32_t ex_n;
   };
   struct Target {
     float4 x, A;
@@ -270,8 +305,8 @@ struct BwdDeltaPass {
       s.z -= q * HAj.z + AGj * ds.z;
       s.l -= AGj;
     } else {
-      atomic_add3(&a.ex[j], make_float3(-gi.x, -gi.y, -gi.z));
-      atomicAdd(&a.ex[j].w, AG);
+      const size_t e = scatter_copy(a.ex_n) + j;
+      scatter_add(a.ex, a.ex_n, e, -gi.x, -gi.y, -gi.z, AG, true);
     }
   }
   static __device__ __forceinline__ void accumulate(Acc& tot, const Acc& p) {
@@ -288,6 +323,9 @@ struct BwdDeltaPass {
 
 // ---- lambda (density and gradient norms) ------------------------------------
 struct BwdLambdaPass {
+  // workgroups in launch order, round-robin over the XCDs: a crowded cell's
+  // targets then scatter from all eight XCDs, each into its own copy
+  static constexpr bool kXcdRanges = false;
   // hits reloaded per round of the sweep
   static constexpr int kHitBatch = 4;
   struct Args {
@@ -295,7 +333,8 @@ struct BwdLambdaPass {
     const float4* __restrict__ sd;   // (S, dL/dD)
     const uint32_t* __restrict__ key;
     float4* __restrict__ gx;         // adjoint of x: += gathered
-    float4* __restrict__ ex;         // scattered dL/dx
+    float4* __restrict__ ex;         // scattered dL/dx, kScatterCopies x ex_n
+    uint32_t ex_n;
   };
   struct Target {
     float4 x, sd;
@@ -357,7 +396,7 @@ struct BwdLambdaPass {
       s.y += gr.y;
       s.z += gr.z;
     } else if (j != t.i) {
-      atomic_add3(&a.ex[j], make_float3(-gi.x, -gi.y, -gi.z));
+      scatter_add(a.ex, a.ex_n, scatter_copy(a.ex_n) + j, -gi.x, -gi.y, -gi.z, 0.0f, false);
     }
   }
   static __device__ __forceinline__ void accumulate(Acc& tot, const Acc& p) {
@@ -383,15 +422,13 @@ __global__ __launch_bounds__(kBlock) void k_bwd_velocity(
     float* __restrict__ grho) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
-  const float4 ox = out_x[i], ov = out_v[i], ex = ex_x[i], ev = ex_v[i], go = gx_out[i],
-               gvo = gv_out[i];
+  const float4 ox = out_x[i], ov = out_v[i], ex = take_scattered(ex_x, i, n),
+               ev = take_scattered(ex_v, i, n), go = gx_out[i], gvo = gv_out[i];
   const float vx = gvo.x + ov.x + ev.x, vy = gvo.y + ov.y + ev.y, vz = gvo.z + ov.z + ev.z;
   gx[i] = make_float4(go.x + ox.x + ex.x + vx * c.inv_dt, go.y + ox.y + ex.y + vy * c.inv_dt,
                       go.z + ox.z + ex.z + vz * c.inv_dt, 0.0f);
   gx0[i] = make_float4(-vx * c.inv_dt, -vy * c.inv_dt, -vz * c.inv_dt, 0.0f);
   grho[i] = ox.w + ex.w;
-  ex_x[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  ex_v[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
 // Position update of iteration k (fluids/Simulator_kernel.cuh:184-192):
@@ -439,7 +476,7 @@ __global__ __launch_bounds__(kBlock) void k_bwd_lambda_pre(
     float4* __restrict__ sdb, double* __restrict__ r0) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
-  const float4 o = out[i], e = ex[i], x = xl[i], s = sd[i];
+  const float4 o = out[i], e = take_scattered(ex, i, n), x = xl[i], s = sd[i];
   const float gl = o.w + e.w;
   const float D = s.w, rh = rho[i];
   const float C = rh / c.pho0 - 1.0f;
@@ -462,7 +499,6 @@ __global__ __launch_bounds__(kBlock) void k_bwd_lambda_pre(
   gxn[i] = g;
   xr[i] = make_float4(x.x, x.y, x.z, gr);
   sdb[i] = make_float4(s.x, s.y, s.z, gD);
-  ex[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
 // gx = gxn + scattered extras; clears the extras.
@@ -471,9 +507,8 @@ __global__ __launch_bounds__(kBlock) void k_bwd_fold(int n, const float4* __rest
                                                      float4* __restrict__ gx) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
-  const float4 g = gxn[i], e = ex[i];
+  const float4 g = gxn[i], e = take_scattered(ex, i, n);
   gx[i] = make_float4(g.x + e.x, g.y + e.y, g.z + e.z, 0.0f);
-  ex[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
 // Prediction x[0] = x + dt (v + dt g) (fluids/Simulator_kernel.cuh:14-15),
@@ -559,7 +594,7 @@ int fill_blocks(int n) {
 void launch_bwd_xsph(const BwdFrame& f, int n, const StepConsts& c, const uint32_t* cs,
                      const HeavyLists& hl, int pass, const Shapes& sh, hipStream_t st) {
   launch_pass<BwdXsphPass>(BwdXsphPass::Args{f.x_final, f.vr, f.gv_out, f.key, f.out_x, f.out_v,
-                                             f.ex_x, f.ex_v},
+                                             f.ex_x, f.ex_v, (uint32_t)n},
                            n, c, Scenes{}, cs, hl, pass, sh, st);
   k_bwd_velocity<<<blocks_for(n), kBlock, 0, st>>>(n, c, f.gx_out, f.gv_out, f.out_x, f.out_v,
                                                    f.ex_x, f.ex_v, f.gx, f.gx0, f.grho_x);
@@ -569,11 +604,11 @@ void launch_bwd_iteration(const BwdFrame& f, const BwdIter& it, int last, int n,
                           const StepConsts& c, const uint32_t* cs, const HeavyLists& hl, int pass,
                           const Shapes& sh, hipStream_t st) {
   k_bwd_clamp<<<blocks_for(n), kBlock, 0, st>>>(n, c, f.gx, it.xl, it.dp, f.gxn, f.A, f.r0);
-  launch_pass<BwdDeltaPass>(BwdDeltaPass::Args{it.xl, f.A, f.key, f.out_x, f.ex_x}, n, c,
+  launch_pass<BwdDeltaPass>(BwdDeltaPass::Args{it.xl, f.A, f.key, f.out_x, f.ex_x, (uint32_t)n}, n, c,
                             Scenes{}, cs, hl, pass, sh, st);
   k_bwd_lambda_pre<<<blocks_for(n), kBlock, 0, st>>>(n, c, last, f.out_x, f.ex_x, it.xl, it.sd,
                                                      it.rho, f.grho_x, f.gxn, f.xr, f.sdb, f.r0);
-  launch_pass<BwdLambdaPass>(BwdLambdaPass::Args{f.xr, f.sdb, f.key, f.gxn, f.ex_x}, n, c,
+  launch_pass<BwdLambdaPass>(BwdLambdaPass::Args{f.xr, f.sdb, f.key, f.gxn, f.ex_x, (uint32_t)n}, n, c,
                              Scenes{}, cs, hl, pass + 1, sh, st);
   k_bwd_fold<<<blocks_for(n), kBlock, 0, st>>>(n, f.gxn, f.ex_x, f.gx);
 }

@@ -1199,7 +1199,8 @@ int pbf_tape_create_ex(pbf_solver* s, int n, int nframes, int flags, pbf_tape** 
     o_rho.push_back(carve(4 * N));
   }
   size_t o_f4[12];
-  for (auto& o : o_f4) o = carve(16 * N);
+  for (int q = 0; q < 12; ++q)  // ex_x and ex_v (2, 3) hold kScatterCopies copies
+    o_f4[q] = carve(16 * N * (q == 2 || q == 3 ? (size_t)pbf::kScatterCopies : 1));
   const size_t o_grx = carve(4 * N), o_gp0 = carve(4 * L * N), o_gp1 = carve(4 * L * N),
                o_gv0 = carve(4 * L * N), o_gv1 = carve(4 * L * N), o_r0 = carve(8 * N),
                o_r0s = carve(8 * (1 + 1024));
@@ -1266,8 +1267,8 @@ int pbf_tape_create_ex(pbf_solver* s, int n, int nframes, int flags, pbf_tape** 
     t->f_cs.push_back((uint32_t*)(b + o_fcs[q]));
   }
   // the scatter arrays start (and stay between kernels) all zero
-  pbf::launch_fill_f4(t->ex_x, n, nullptr);
-  pbf::launch_fill_f4(t->ex_v, n, nullptr);
+  pbf::launch_fill_f4(t->ex_x, n * pbf::kScatterCopies, nullptr);
+  pbf::launch_fill_f4(t->ex_v, n * pbf::kScatterCopies, nullptr);
   if ((e = hipDeviceSynchronize()) != hipSuccess) {
     (void)hipFree(t->arena);
     delete t;

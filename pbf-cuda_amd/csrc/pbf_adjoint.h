@@ -12,6 +12,13 @@
 namespace pbf {
 
 // One frame's backward state.
+// Copies of the scattered pair sums: workgroup b adds to copy b % 8 (the XCD
+// it runs on, as workgroups are dispatched round-robin), which spreads the
+// atomic adds a crowded cell sends to one particle over eight addresses
+// (same-address atomics serialise); the next pointwise kernel adds the
+// copies.
+constexpr int kScatterCopies = 8;
+
 struct BwdFrame {
   // recorded forward (read)
   const float4* x_final;  // positions after the last iteration
@@ -22,7 +29,8 @@ struct BwdFrame {
   const float4* gv_out;   // dL/d(velocities after XSPH)
   // working arrays
   float4 *out_x, *out_v;  // gathered pair sums
-  float4 *ex_x, *ex_v;    // scattered pair sums (all zero between kernels)
+  float4 *ex_x, *ex_v;    // scattered pair sums: kScatterCopies copies of n entries,
+                          // component-major (all zero between kernels)
   float4* gx;             // dL/d(positions) of the current iteration
   float4* gxn;            // the same for the iteration before
   float4* gx0;            // dL/d(start positions) from v = (x - x0)/dt

@@ -61,7 +61,7 @@ __global__ __launch_bounds__(kBlock) void k_light(typename P::Args a, int n, Ste
   const uint32_t thr = hl.counters[kThresholdWord] ? hl.counters[kThresholdWord] : kLightMax;
   const int tid = threadIdx.x;
   const int nb = (int)blockDim.x;
-  const int block_base = xcd_block_index() * nb;
+  const int block_base = (P::kXcdRanges ? xcd_block_index() : (int)blockIdx.x) * nb;
   const int i = block_base + tid;
   const bool active = i < n;
   const SeqTable st{&seq_dyn[tid], nb};
@@ -193,7 +193,8 @@ __global__ __launch_bounds__(kBlock) void k_chunks(typename P::Args a, StepConst
   // XCDs, each XCD on one contiguous range of the (chunk-major) item list
   const uint32_t need = (total + kBlock - 1) / kBlock;
   const int active = need < gridDim.x ? (int)need : (int)gridDim.x;
-  const int vb = xcd_block_index_of(active);
+  const int vb = P::kXcdRanges ? xcd_block_index_of(active)
+                               : ((int)blockIdx.x < active ? (int)blockIdx.x : -1);
   if (vb < 0) return;
   const uint32_t stride = (uint32_t)active * kBlock;
   const SeqTable st{&seq[0][threadIdx.x], kBlock};

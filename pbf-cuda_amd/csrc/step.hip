@@ -33,6 +33,7 @@ namespace {
 // into grad_i and, for j != i, its squared norm into gradj_l2; then the wall
 // term and lambda = -(rho/pho0 - 1) / (gradj_l2 + |grad_i|^2 + eps).
 struct LambdaPass {
+  static constexpr bool kXcdRanges = true;  // one contiguous range per XCD
   // hits reloaded per round of the sweep
   static constexpr int kHitBatch = 4;
   struct Args {
@@ -113,6 +114,7 @@ struct LambdaPass {
 // (x, lambda) from xl, writes the new iterate to xa (Jacobi; the ping-pong
 // replaces thrust::copy_n at fluids/Simulator.cu:247-248).
 struct DeltaPass {
+  static constexpr bool kXcdRanges = true;  // one contiguous range per XCD
   // hits reloaded per round of the sweep
   static constexpr int kHitBatch = 4;
   struct Args {
@@ -222,6 +224,7 @@ struct DeltaLastPass : DeltaPass {
 // positions into the caller's npos.
 template <int L>
 struct XsphPass {
+  static constexpr bool kXcdRanges = true;  // one contiguous range per XCD
   // hits reloaded per round of the sweep
   static constexpr int kHitBatch = 4;
   struct Args {
@@ -285,6 +288,7 @@ struct XsphPass {
 // rho on a fresh grid (the lambda pass's density term), for the density
 // error statistic: |rho/pho0 - 1| per particle in FP64.
 struct DensityPass {
+  static constexpr bool kXcdRanges = true;  // one contiguous range per XCD
   // hits reloaded per round of the sweep
   static constexpr int kHitBatch = 4;
   struct Args {

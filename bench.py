@@ -365,8 +365,9 @@ def sweep_workload(pbf, dev, stream, frames=120, back=10):
 def main():
     args = parse_args()
     if args.gpus != 1:
-        print(f"bench.py: --gpus {args.gpus}: this build measures one GPU only "
-              "(the step has no multi-GPU path; see DESIGN.md on sharding)", file=sys.stderr)
+        print(f"bench.py: --gpus {args.gpus}: the benchmark measures one GPU only "
+              "(the step has no multi-GPU path; the single scene would shard as x-slabs, whose "
+              "one-GPU share the slab_share figure times)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

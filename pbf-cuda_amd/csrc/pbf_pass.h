@@ -38,12 +38,13 @@ inline int blocks_for(long long n) { return (int)((n + kBlock - 1) / kBlock); }
 
 // Light: one work-item per target; the workgroup size is a launch shape
 // (64, 128 or 256), the column tables live in dynamic LDS.  A heavy target
-// gets one chunk item per kChunk candidates and a record, appended with one
-// atomic per wavefront each.
-// The counters of parity pass & 1 belong to this pass; the kernel clears the
-// other parity's counters for the next pass (their previous users finished earlier
-// on the stream).  If the chunk list is full the lane sums its target itself,
-// still chunk by chunk, so the canonical order holds either way.
+// (more candidates than the threshold word, which the previous pass's
+// combine kernel set) gets one chunk item per kChunk candidates and a
+// record, appended with one atomic per wavefront each.  The counters of
+// parity pass & 1 belong to this pass; the kernel clears the other parity's
+// counters for the next pass (their previous users finished earlier on the
+// stream).  If the chunk list is full the lane sums its target itself, still
+// chunk by chunk, so the canonical order holds either way.
 template <class P, class S>
 __global__ __launch_bounds__(kBlock) void k_light(typename P::Args a, int n, StepConsts c, S sc,
                                                   const uint32_t* __restrict__ cs, HeavyLists hl,

@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define PBF_ABI_VERSION 3
+#define PBF_ABI_VERSION 4
 
 /* status codes */
 #define PBF_OK 0
@@ -242,6 +242,17 @@ typedef struct pbf_shapes {
 } pbf_shapes;
 int pbf_set_shapes(pbf_solver* s, const pbf_shapes* shapes);
 int pbf_get_shapes(const pbf_solver* s, pbf_shapes* out);
+
+/*
+ * Crowded targets (no counterpart in the reference): a target with more
+ * candidate neighbours than the threshold has its 64-candidate chunks spread
+ * over the chunk kernel instead of one lane.  0 (the default) = adaptive:
+ * 256, or 128 when the previous neighbour pass found few targets between 128
+ * and 256 candidates; otherwise a fixed count >= 64.  Like the launch
+ * shapes, it changes no result, only the speed.
+ */
+int pbf_set_heavy_threshold(pbf_solver* s, int threshold);
+int pbf_get_heavy_threshold(const pbf_solver* s, int* out);
 
 /*
  * Autotuner: times one step from the given state for each candidate shape

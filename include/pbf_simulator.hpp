@@ -69,6 +69,8 @@ class Simulator {
     const pbf_shapes sh{light_block, chunk_blocks};
     check(pbf_set_shapes(s_, &sh), s_);
   }
+  // 0 = adaptive (default); results never depend on it.
+  void setHeavyThreshold(int threshold) { check(pbf_set_heavy_threshold(s_, threshold), s_); }
   pbf_shapes autotune(const float* pos, const float* vel, const uint32_t* iid, int n, int reps = 3,
                       void* stream = nullptr, double* best_ms = nullptr) {
     pbf_shapes best;

@@ -918,6 +918,25 @@ int pbf_set_shapes(pbf_solver* s, const pbf_shapes* sh) {
   return PBF_OK;
 }
 
+int pbf_set_heavy_threshold(pbf_solver* s, int threshold) {
+  if (!s) return fail(nullptr, PBF_ERR_INVALID_ARGUMENT, "solver is NULL");
+  if (threshold != 0 && (threshold < 64 || threshold > (1 << 30)))
+    return fail(s, PBF_ERR_INVALID_ARGUMENT, "heavy threshold must be 0 or in [64, 2^30], got %d",
+                threshold);
+  if (threshold != s->shapes.heavy) {
+    DeviceGuard g(s->device);
+    drop_graphs(s);
+  }
+  s->shapes.heavy = threshold;
+  return PBF_OK;
+}
+
+int pbf_get_heavy_threshold(const pbf_solver* s, int* out) {
+  if (!s || !out) return fail(nullptr, PBF_ERR_INVALID_ARGUMENT, "NULL argument");
+  *out = s->shapes.heavy;
+  return PBF_OK;
+}
+
 int pbf_get_shapes(const pbf_solver* s, pbf_shapes* out) {
   if (!s || !out) return fail(nullptr, PBF_ERR_INVALID_ARGUMENT, "NULL argument");
   out->light_block = s->shapes.light_block;

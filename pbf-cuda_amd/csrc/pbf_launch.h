@@ -23,6 +23,7 @@ struct Scenes {
 struct Shapes {
   int light_block = 256;   // workgroup size of the light kernels: 64, 128 or 256
   int chunk_blocks = 2048;  // grid cap of the chunk kernels (grid-stride)
+  int heavy = 0;            // heavy-target threshold; 0: adaptive
 };
 
 // sort.hip

@@ -38,8 +38,8 @@ inline int blocks_for(long long n) { return (int)((n + kBlock - 1) / kBlock); }
 
 // Light: one work-item per target; the workgroup size is a launch shape
 // (64, 128 or 256), the column tables live in dynamic LDS.  A heavy target
-// (more candidates than the threshold word, which the previous pass's
-// combine kernel set) gets one chunk item per kChunk candidates and a
+// (more candidates than fixed_threshold, or when that is 0 than the
+// threshold word the previous pass's combine kernel set) gets one chunk item per kChunk candidates and a
 // record, appended with one atomic per wavefront each.  The counters of
 // parity pass & 1 belong to this pass; the kernel clears the other parity's
 // counters for the next pass (their previous users finished earlier on the
@@ -48,7 +48,7 @@ inline int blocks_for(long long n) { return (int)((n + kBlock - 1) / kBlock); }
 template <class P, class S>
 __global__ __launch_bounds__(kBlock) void k_light(typename P::Args a, int n, StepConsts c, S sc,
                                                   const uint32_t* __restrict__ cs, HeavyLists hl,
-                                                  int pass) {
+                                                  int pass, uint32_t fixed_threshold) {
   extern __shared__ uint2 seq_dyn[];
   __shared__ uint32_t key_sh[kBlock];
   __shared__ uint32_t order_sh[kBlock];
@@ -59,7 +59,8 @@ __global__ __launch_bounds__(kBlock) void k_light(typename P::Args a, int n, Ste
     hl.counters[rec_counter((pass + 1) & 1)] = 0u;
     hl.counters[item_counter((pass + 1) & 1)] = 0u;
   }
-  const uint32_t thr = hl.counters[kThresholdWord] ? hl.counters[kThresholdWord] : kLightMax;
+  const uint32_t word = hl.counters[kThresholdWord];
+  const uint32_t thr = fixed_threshold ? fixed_threshold : (word ? word : kLightMax);
   const int tid = threadIdx.x;
   const int nb = (int)blockDim.x;
   const int block_base = (P::kXcdRanges ? xcd_block_index() : (int)blockIdx.x) * nb;
@@ -272,7 +273,8 @@ void launch_pass_s(const typename P::Args& a, int n, const StepConsts& c, const 
                    const uint32_t* cs, const HeavyLists& hl, int pass, const Shapes& sh,
                    hipStream_t st) {
   const int lb = sh.light_block;
-  k_light<P, S><<<(n + lb - 1) / lb, lb, 10 * lb * sizeof(uint2), st>>>(a, n, c, sc, cs, hl, pass);
+  k_light<P, S><<<(n + lb - 1) / lb, lb, 10 * lb * sizeof(uint2), st>>>(a, n, c, sc, cs, hl, pass,
+                                                                       (uint32_t)sh.heavy);
   k_chunks<P, S><<<grid_for_items(hl.item_cap, sh.chunk_blocks), kBlock, 0, st>>>(a, c, sc, cs,
                                                                                     hl, pass);
   k_combine<P, S><<<grid_for_items(n, 512), kBlock, 0, st>>>(a, c, sc, hl, pass);

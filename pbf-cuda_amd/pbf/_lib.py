@@ -100,6 +100,8 @@ _SIGNATURES = [
     ("pbf_set_graph", ctypes.c_int, [_P, ctypes.c_int]),
     ("pbf_set_shapes", ctypes.c_int, [_P, ctypes.POINTER(PbfShapes)]),
     ("pbf_get_shapes", ctypes.c_int, [_P, ctypes.POINTER(PbfShapes)]),
+    ("pbf_set_heavy_threshold", ctypes.c_int, [_P, ctypes.c_int]),
+    ("pbf_get_heavy_threshold", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int)]),
     ("pbf_autotune", ctypes.c_int,
      [_P, _P, _P, _P, ctypes.c_int, ctypes.c_int, _P, ctypes.POINTER(PbfShapes),
       ctypes.POINTER(ctypes.c_double)]),

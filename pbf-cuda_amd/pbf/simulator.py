@@ -216,6 +216,18 @@ class Simulator:
         _lib.check(self._lib.pbf_get_shapes(h, ctypes.byref(sh)), h)
         return int(sh.light_block), int(sh.chunk_blocks)
 
+    def set_heavy_threshold(self, threshold: int) -> None:
+        """Candidate count above which a target's chunks go to the chunk
+        kernel; 0 = adaptive (the default).  Results never depend on it."""
+        h = self._handle()
+        _lib.check(self._lib.pbf_set_heavy_threshold(h, int(threshold)), h)
+
+    def get_heavy_threshold(self) -> int:
+        v = ctypes.c_int(0)
+        h = self._handle()
+        _lib.check(self._lib.pbf_get_heavy_threshold(h, ctypes.byref(v)), h)
+        return int(v.value)
+
     def autotune(self, pos, vel, iid, nparticle: int, reps: int = 3, stream=None):
         """Times one step from (pos, vel, iid) per candidate launch shape on
         private copies, checks every shape gives the same bits, installs the

@@ -25,7 +25,7 @@ def test_library_exports_every_declared_symbol(pbf_mod):
     for name in declared:
         assert hasattr(lib, name), name
     assert set(declared) == set(_lib.SYMBOLS), "ctypes prototypes and header disagree"
-    assert lib.pbf_abi_version() == 3
+    assert lib.pbf_abi_version() == 4
 
 
 def test_status_strings(pbf_mod):
@@ -187,6 +187,8 @@ def test_new_entry_points_reject_null_solver(pbf_mod):
     k = ctypes.c_int(0)
     assert lib.pbf_get_scenes(None, ctypes.byref(k), None) == bad
     assert lib.pbf_autotune(None, None, None, None, 10, 3, None, None, None) == bad
+    assert lib.pbf_set_heavy_threshold(None, 128) == bad
+    assert lib.pbf_get_heavy_threshold(None, None) == bad
     t = ctypes.c_void_p()
     assert lib.pbf_tape_create_ex(None, 10, 2, 1, ctypes.byref(t)) == bad
     assert lib.pbf_tape_create_ex(None, 10, 2, 0, None) == bad

@@ -190,3 +190,32 @@ def test_autotune_leaves_inputs_and_installs_a_candidate(pbf_mod, device):
     for _ in range(6):
         ref.step()
     assert_bitexact(r.step(), ref.step(), "after autotune")
+
+
+@pytest.mark.parametrize("batched", [False, True])
+def test_heavy_threshold_changes_no_bit(pbf_mod, device, batched):
+    """Which targets are summed by one lane and which are spread over the
+    chunk kernel is a scheduling choice only: fixed thresholds of 64, 128,
+    256 and 2^30 (no target spread unless over the list's capacity) and the
+    adaptive default give the same bits, frame after frame, in a scene with
+    piles (crowded corners)."""
+    p = P.dam_params()
+    sc = scenes.block((24, 24, 24), (0.8, 0.8, 0.8), seed=33, vel_sigma=4.0)
+    print(f"heavy threshold batched={batched} n={sc.n}", flush=True)
+    results = {}
+    for thr in (0, 64, 128, 256, 1 << 30):
+        if batched:
+            run = BatchRun(pbf_mod, [sc, sc], p)
+        else:
+            run = Run(pbf_mod, sc, p, 4)
+        run.sim.set_heavy_threshold(thr)
+        assert run.sim.get_heavy_threshold() == thr
+        frames = [run.step() for _ in range(4)]
+        results[thr] = frames
+    base = results[0]
+    for thr, frames in results.items():
+        for f, (g, b) in enumerate(zip(frames, base)):
+            for k in ("npos", "nvel", "iid"):
+                np.testing.assert_array_equal(g[k], b[k], err_msg=f"threshold {thr} frame {f} {k}")
+    with pytest.raises(pbf_mod.PBFError):
+        Run(pbf_mod, sc, p, 4).sim.set_heavy_threshold(10)

@@ -158,6 +158,12 @@ def timed(run, steps, stream):
     return e0.elapsed_time(e1) / steps
 
 
+GRAD_NOTE = ("with the dam constants the exact derivative of the rollout grows by ~1e3-1e4 per "
+             "frame (Delta-p clamped at MAX_DP makes the map piecewise smooth); the FP64 "
+             "oracle's gradient of the 30-frame 16K rollout is ~1e110, beyond the FP32 range, so "
+             "non-finite gradients here are expected; gradient parity is tested at short horizons")
+
+
 def adjoint_ratio(pbf, p, dev, stream, reps=3, store=True):
     """Backward / forward time of the 30-frame differentiable rollout of the
     16,384-particle scene (dam constants, 4 iterations): forward = the tape's
@@ -228,6 +234,7 @@ def adjoint_ratio(pbf, p, dev, stream, reps=3, store=True):
             "forward_ms": t_f, "backward_ms": t_b, "plain_forward_ms": t_p,
             "backward_over_plain_forward": t_b / t_p,
             "gradients_finite": bool(torch.isfinite(gv).all().item()) and bool(np.isfinite(grho)),
+            "gradients_note": GRAD_NOTE,
             "workload": "differentiable_16k: 16,384 particles (32x16x32 jittered lattice, "
                         "v0 ~ N(0, 0.5^2)), box 40h x 20h x 40h, 30 frames, 4 iterations, dam "
                         "constants; loss = mean squared distance to a seeded target cloud; "
@@ -351,7 +358,8 @@ def sweep_workload(pbf, dev, stream, frames=120, back=10):
             out[key] = {"tape_forward_ms": ms_f, "backward_ms": ms_b,
                         "backward_forward_ratio": ms_b / ms_f, "tape_device_bytes": int(tape_bytes),
                         "gradients_finite": bool(torch.isfinite(gv).all().item())
-                        and bool(np.isfinite(grho))}
+                        and bool(np.isfinite(grho)),
+                        "gradients_note": GRAD_NOTE}
             tape.close()
             del tape, xf, iidf, g, gp0, gv0
     out["backward_forward_ratio"] = out["stored"]["backward_forward_ratio"]

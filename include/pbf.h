@@ -247,8 +247,9 @@ int pbf_get_shapes(const pbf_solver* s, pbf_shapes* out);
  * Crowded targets (no counterpart in the reference): a target with more
  * candidate neighbours than the threshold has its 64-candidate chunks spread
  * over the chunk kernel instead of one lane.  0 (the default) = adaptive:
- * 256, or 128 when the previous neighbour pass found few targets between 128
- * and 256 candidates; otherwise a fixed count >= 64.  Like the launch
+ * the smallest of 64, 80, 96, 112, 128, 160, 192 such that the previous
+ * neighbour pass found at most 1 in 25 targets with more candidates than it
+ * and at most 256, else 256; otherwise a fixed count >= 64.  Like the launch
  * shapes, it changes no result, only the speed.
  */
 int pbf_set_heavy_threshold(pbf_solver* s, int threshold);

@@ -118,16 +118,27 @@ __device__ __forceinline__ uint32_t cell_id(const Cell& q, const SceneGeo& g) {
 // few chunk lengths instead of one lane thousands of dependent gathers.
 constexpr uint32_t kChunk = 64;
 // A target is heavy when it has more than kLightMax candidates, or more than
-// kHeavyMin when, in the previous neighbour pass, targets with kHeavyMin < m
-// <= kLightMax held less than 1/kMidShare of all candidates.  In sparse water
-// the few long lanes of such mid-sized targets set the light kernel's tail
-// and are better spread over the chunk kernel; in dense water they are
-// common, balance each other within the workgroups, and would only lengthen
-// the chunk lists.  Which targets are heavy changes no result (the canonical
-// order above), only the timing.
+// the threshold the previous neighbour pass chose: the smallest level L of
+// heavy_level() such that at most n / kLevelShare targets had L < m <=
+// kLightMax (else kLightMax).  A light wavefront lasts as long as its longest
+// lane, so in sparse water the few long lanes set the light kernel's tail and
+// are better spread over the chunk kernel (threshold 64); in dense water long
+// lanes are common, balance each other within the wavefronts, and spreading
+// them would only lengthen the chunk lists (threshold up to kLightMax).
+// Which targets are heavy changes no result (the canonical order above),
+// only the timing.
 constexpr uint32_t kLightMax = 256;
-constexpr uint32_t kHeavyMin = 128;
-constexpr unsigned long long kMidShare = 6;
+constexpr int kLevels = 7;
+__host__ __device__ constexpr uint32_t heavy_level(int k) {  // 64, 80, .., 128, 160, 192
+  return k < 4 ? 64u + 16u * (uint32_t)k : 128u + 32u * (uint32_t)(k - 4);
+}
+constexpr uint32_t kLevelShare = 25;
+// The per-level counts travel packed, three 21-bit fields to a 64-bit word
+// (exact while a statistics slot counts fewer than 2^21 targets per level,
+// i.e. below 2^27 targets; beyond that a count may spill into the next field
+// and misjudge the threshold, which costs time only).
+constexpr int kLevelBits = 21;
+constexpr int kLevelWords = (kLevels + 2) / 3;
 
 // A work-item's candidate sequence as a compact column table in LDS: entry
 // k < nk is (first slot of the k-th non-empty column, sequence position of
@@ -354,8 +365,8 @@ struct HeavyLists {
   // per pass parity: the record and chunk-item counters (rec_counter,
   // item_counter); counters[kThresholdWord]: the heavy threshold for the next
   // pass (0: kLightMax), set by each combine kernel from the kStatSlots
-  // candidate statistics {mid-sized, all} (stat_slot) the light kernel's
-  // wavefronts add to.  Each word or slot has a 256-byte line of its own:
+  // per-level target counts (stat_slot) the light kernel's wavefronts add
+  // to.  Each word or slot has a 256-byte line of its own:
   // atomics on one line serialise, and every light wavefront reads the
   // threshold
   uint32_t* counters;
@@ -369,7 +380,7 @@ constexpr int kCounterLine = 64;  // words
 constexpr int kThresholdWord = 4 * kCounterLine;
 constexpr int kStatSlots = 64;
 constexpr int kCounterWords = (5 + kStatSlots) * kCounterLine;
-// The {mid-sized candidates, all candidates} pair of statistics slot k.
+// Statistics slot k: kLevelWords words of packed per-level target counts.
 __host__ __device__ __forceinline__ int stat_slot(int k) { return (5 + k) * kCounterLine; }
 __host__ __device__ __forceinline__ int rec_counter(int parity) {
   return 2 * parity * kCounterLine;

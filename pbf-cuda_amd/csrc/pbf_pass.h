@@ -14,7 +14,7 @@
 // Each neighbour pass is three kernels:
 //   light   one work-item per target over the cell-sorted order; workgroups
 //           remapped so each XCD's L2 serves one contiguous particle range; a
-//           heavy target (pbf_device.h: kLightMax, kHeavyMin, kHeavyFactor)
+//           heavy target (pbf_device.h: kLightMax, heavy_level, kLevelShare)
 //           is appended to the heavy list instead;
 //   chunks  one work-item per kChunk-candidate chunk of the heavy targets;
 //   combine one work-item per heavy target, adding its chunk sums in order.
@@ -75,20 +75,21 @@ __global__ __launch_bounds__(kBlock) void k_light(typename P::Args a, int n, Ste
     m = build_seq(st, cell_of(t.x.x, t.x.y, t.x.z, g, c.h), cs, g);
   }
   {
-    // this pass's statistics for the next pass's threshold
-    unsigned long long mid = m > kHeavyMin && m <= kLightMax ? m : 0u;
-    unsigned long long all = m;
+    // this pass's statistics for the next pass's threshold: per level, how
+    // many of the wavefront's targets have level < m <= kLightMax
+    unsigned long long w[kLevelWords] = {};
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      mid += __shfl_xor(mid, off);
-      all += __shfl_xor(all, off);
+    for (int k = 0; k < kLevels; ++k) {
+      const uint64_t b = __ballot(m > heavy_level(k) && m <= kLightMax);
+      w[k / 3] += (unsigned long long)__popcll(b) << (kLevelBits * (k % 3));
     }
-    if ((threadIdx.x & 63) == 0 && all) {
+    if ((threadIdx.x & 63) == 0) {
       const int wave = (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
       unsigned long long* slot =
           reinterpret_cast<unsigned long long*>(hl.counters + stat_slot(wave % kStatSlots));
-      atomicAdd(&slot[0], mid);
-      atomicAdd(&slot[1], all);
+#pragma unroll
+      for (int j = 0; j < kLevelWords; ++j)
+        if (w[j]) atomicAdd(&slot[j], w[j]);
     }
   }
   const bool is_heavy = active && m > thr;
@@ -221,7 +222,7 @@ __global__ __launch_bounds__(kBlock) void k_chunks(typename P::Args a, StepConst
 
 // One work-item per heavy target: chunk partials added in chunk order.
 template <class P, class S>
-__global__ __launch_bounds__(kBlock) void k_combine(typename P::Args a, StepConsts c, S sc,
+__global__ __launch_bounds__(kBlock) void k_combine(typename P::Args a, int n, StepConsts c, S sc,
                                                     HeavyLists hl, int pass) {
   const uint32_t nrec = hl.counters[rec_counter(pass & 1)];
   if (blockIdx.x == 0 && threadIdx.x < 64) {
@@ -229,15 +230,26 @@ __global__ __launch_bounds__(kBlock) void k_combine(typename P::Args a, StepCons
     // cleared for the next pass
     unsigned long long* slot =
         reinterpret_cast<unsigned long long*>(hl.counters + stat_slot((int)threadIdx.x));
-    unsigned long long mid = slot[0], all = slot[1];
-    slot[0] = 0ull;
-    slot[1] = 0ull;
+    unsigned long long cnt[kLevels];
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      mid += __shfl_xor(mid, off);
-      all += __shfl_xor(all, off);
+    for (int j = 0; j < kLevelWords; ++j) {
+      const unsigned long long w = slot[j];
+      slot[j] = 0ull;
+#pragma unroll
+      for (int f = 0; f < 3 && 3 * j + f < kLevels; ++f)
+        cnt[3 * j + f] = (w >> (kLevelBits * f)) & ((1ull << kLevelBits) - 1ull);
     }
-    if (threadIdx.x == 0) hl.counters[kThresholdWord] = mid * kMidShare < all ? kHeavyMin : kLightMax;
+#pragma unroll
+    for (int k = 0; k < kLevels; ++k)
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) cnt[k] += __shfl_xor(cnt[k], off);
+    if (threadIdx.x == 0) {
+      uint32_t thr = kLightMax;
+#pragma unroll
+      for (int k = kLevels - 1; k >= 0; --k)
+        if (cnt[k] * kLevelShare <= (unsigned long long)n) thr = heavy_level(k);
+      hl.counters[kThresholdWord] = thr;
+    }
   }
   const uint32_t stride = gridDim.x * kBlock;
   for (uint32_t r = blockIdx.x * kBlock + threadIdx.x; r < nrec; r += stride) {
@@ -277,7 +289,7 @@ void launch_pass_s(const typename P::Args& a, int n, const StepConsts& c, const 
                                                                        (uint32_t)sh.heavy);
   k_chunks<P, S><<<grid_for_items(hl.item_cap, sh.chunk_blocks), kBlock, 0, st>>>(a, c, sc, cs,
                                                                                     hl, pass);
-  k_combine<P, S><<<grid_for_items(n, 512), kBlock, 0, st>>>(a, c, sc, hl, pass);
+  k_combine<P, S><<<grid_for_items(n, 512), kBlock, 0, st>>>(a, n, c, sc, hl, pass);
 }
 
 template <class P>

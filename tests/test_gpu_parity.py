@@ -285,6 +285,35 @@ def test_run_frames_sweeping_wall_bitexact(pbf_mod, orc, device):
     np.testing.assert_array_equal(g_vel.view(np.uint32), vel.view(np.uint32))
 
 
+def test_sweep_dam_4m_piled_frame_bitexact(pbf_mod, orc, device):
+    """The 4,194,304-particle sweeping-wall workload (6 iterations) at full
+    size: 60 frames on the GPU drive ~10,000 particles into one corner cell;
+    frame 60, stepped by the GPU from that state, equals the FP32 mirror's
+    step from the same state bit for bit."""
+    p = P.dam_params(niter=6)
+    sc = scenes.sweep_dam()
+    sw = pbf_mod.Sweep(*scenes.sweep_of_sweep_dam())
+    print(f"sweep dam 4M: {sc.n} particles, 60 frames then one compared", flush=True)
+    run = Run(pbf_mod, sc, p)
+    which = run.sim.run_frames(run.a[0], run.a[1], run.b[0], run.b[1], run.iid, sc.n, 0, 60, sw)
+    cur, nxt = (run.b, run.a) if which else (run.a, run.b)
+    torch.cuda.synchronize()
+    pos = from_dev(cur[0]).reshape(-1, 4)
+    vel = from_dev(cur[1]).reshape(-1, 4)
+    iid = from_dev(run.iid, dtype=np.uint32)
+    which = run.sim.run_frames(cur[0], cur[1], nxt[0], nxt[1], run.iid, sc.n, 60, 1, sw)
+    assert which == 1
+    torch.cuda.synchronize()
+    ul, ll = sw.box(60)
+    m = orc.step(p, ul, ll, pos, vel, iid, fp32=True)
+    assert int(np.bincount(m["keys"]).max()) > 5000  # the corner pile is there
+    np.testing.assert_array_equal(from_dev(run.iid, dtype=np.uint32), m["iid"])
+    np.testing.assert_array_equal(from_dev(nxt[0]).reshape(-1, 4).view(np.uint32),
+                                  m["npos"].view(np.uint32))
+    np.testing.assert_array_equal(from_dev(nxt[1]).reshape(-1, 4).view(np.uint32),
+                                  m["nvel"].view(np.uint32))
+
+
 @pytest.mark.parametrize("box", [(30.0, 20.0, 15.0), (6.4, 6.4, 3.2)])
 def test_large_and_small_grids_bitexact(pbf_mod, orc, device, box):
     """A sparse 300x200x150-cell box (9M cells: the cell-start scan runs its

@@ -33,6 +33,7 @@ HBM_PEAK_GBS = 8000.0
 # lambda (4 B), write x (12 B).
 DELTA_BYTES_PER_PARTICLE = 28.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_delta_single_dam_1m_traffic.md")
+SQ_FILE = os.path.join(ROOT, "profiles", "neighbour_kernels_single_dam_1m_sq_counters.md")
 
 
 def parse_args():
@@ -65,6 +66,38 @@ def measured_traffic():
         return None
     m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", text)
     return float(m.group(1)) if m else None
+
+
+def measured_valu():
+    """VALU issue figures of the delta-p light kernel from the committed
+    rocprofv3 SQ counter summary, or None: wave-instructions per launch, the
+    time they take at 256 CUs x 128 FP32 lanes x 2.4 GHz, and the traced
+    launch duration."""
+    try:
+        text = open(SQ_FILE).read()
+    except OSError:
+        return None
+    got = {}
+    for key in ("valu_wave_insts_per_launch", "valu_issue_us_per_launch", "traced_launch_us"):
+        m = re.search(key + r":\s*([0-9.eE+]+)", text)
+        if not m:
+            return None
+        got[key] = float(m.group(1))
+    return {"kernel": "k_light<DeltaPass> (rocprofv3 SQ_INSTS_VALU, timed frames)",
+            "wave_insts_per_launch": got["valu_wave_insts_per_launch"],
+            "issue_us": got["valu_issue_us_per_launch"], "launch_us": got["traced_launch_us"],
+            "frac": got["valu_issue_us_per_launch"] / got["traced_launch_us"],
+            "peak": "256 CUs x 128 FP32 lanes x 2.4 GHz"}
+
+
+def step_bytes(n, niter, cells):
+    """Algorithmic HBM bytes of one whole step: one compulsory read or write of
+    each logical array per phase, neighbour re-reads counted as cache hits;
+    per particle predict 48, cell key 16, sort (key, index) 16, payload
+    permute 80, cell ranges 4, per iteration 48 (lambda: x 12 -> lambda, rho
+    8; delta-p: x 12, lambda 4 -> x 12), velocity 36, XSPH 40; plus 8 per
+    cell for the cell table."""
+    return n * (240 + 48 * niter) + 8 * cells
 
 
 def cpu_baseline(p, sc, pos_np, vel_np, iid_np, steps):
@@ -529,6 +562,7 @@ def main():
         I = eager.iid.cpu().numpy().astype(np.uint32)
         cpu = cpu_baseline(p, sc, P4, V4, I, args.cpu_steps)
 
+    sb = step_bytes(n, p.niter, 256 * 128 * 128)
     out = {
         "metric": METRIC,
         "value": value,
@@ -553,6 +587,10 @@ def main():
         "eager_ms_per_step": ms_eager,
         "graph_ms_per_step": ms_graph,
         "roofline": roofline,
+        "step_roofline": {"bound": "hbm", "algorithmic_bytes": sb,
+                          "achieved": round(sb / (ms * 1e-3) / 1e9, 3), "peak": HBM_PEAK_GBS,
+                          "unit": "GB/s", "frac": sb / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "valu": measured_valu(),
         "cpu_baseline": cpu,
         "phase_ms": phase_ms,
         "batched_rate": batched,

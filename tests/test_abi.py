@@ -250,3 +250,24 @@ def test_cpp_mirror_header_compiles_and_links(tmp_path):
                         str(out), "-L", libdir, "-lpbf", "-L/opt/rocm/lib", "-lamdhip64",
                         f"-Wl,-rpath,{libdir}"], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr[-2000:]
+
+
+def test_stepsim_example_checks_its_input(tmp_path):
+    """The C++ host program pbf-cuda_amd/examples/stepsim is built and rejects
+    a missing argument, a missing file and a short header before any device
+    call (exit status 2)."""
+    import subprocess
+
+    exe = os.path.join(ROOT, "pbf-cuda_amd", "examples", "stepsim")
+    if not os.path.exists(exe):
+        pytest.skip("examples/stepsim not built")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "usage" in r.stderr
+    r = subprocess.run([exe, str(tmp_path / "none.bin"), str(tmp_path / "out.bin")],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "cannot open" in r.stderr
+    short = tmp_path / "short.bin"
+    short.write_bytes(b"\x01\x00\x00\x00")
+    r = subprocess.run([exe, str(short), str(tmp_path / "out.bin")], capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode == 2 and "bad header" in r.stderr

@@ -314,6 +314,46 @@ def test_sweep_dam_4m_piled_frame_bitexact(pbf_mod, orc, device):
                                   m["nvel"].view(np.uint32))
 
 
+def test_cpp_host_program_equals_run_frames(pbf_mod, device, tmp_path):
+    """pbf-cuda_amd/examples/stepsim, FluidSystem::stepSimulate written as a
+    C++ host program over include/pbf_simulator.hpp (setLim + step per frame,
+    tic-toc buffers), run as a child process, equals the library's
+    run_frames over the same 15 frames of a sweeping wall bit for bit."""
+    import os
+    import subprocess
+
+    from conftest import ROOT
+
+    exe = os.path.join(ROOT, "pbf-cuda_amd", "examples", "stepsim")
+    if not os.path.exists(exe):
+        pytest.fail("pbf-cuda_amd/examples/stepsim is not built (make -C pbf-cuda_amd)")
+    p = P.dam_params()
+    sc = scenes.block((32, 16, 32), (3.0, 1.6, 2.0), seed=41)
+    sw = pbf_mod.Sweep(sc.ulim, sc.llim, a_ulim=(0.6, 0.0, 0.0), w=2 * np.pi / 20, start_frame=2)
+    nf = 15
+    pos, vel = sc.as_layout(4)
+    src, dst = tmp_path / "in.bin", tmp_path / "out.bin"
+    src.write_bytes(np.array([sc.n, nf], np.int32).tobytes() + bytes(p.to_c()) + bytes(sw.c)
+                    + pos.tobytes() + vel.tobytes() + sc.iid.astype(np.uint32).tobytes())
+    print(f"stepsim: {sc.n} particles, {nf} frames", flush=True)
+    r = subprocess.run([exe, str(src), str(dst)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    raw = np.frombuffer(dst.read_bytes(), np.uint8)
+    nb = sc.n * 16
+    c_pos = raw[:nb].view(np.float32).reshape(-1, 4)
+    c_vel = raw[nb:2 * nb].view(np.float32).reshape(-1, 4)
+    c_iid = raw[2 * nb:].view(np.uint32)
+    run = Run(pbf_mod, sc, p)
+    which = run.sim.run_frames(run.a[0], run.a[1], run.b[0], run.b[1], run.iid, sc.n, 0, nf, sw)
+    torch.cuda.synchronize()
+    out = run.b if which else run.a
+    np.testing.assert_array_equal(c_iid, from_dev(run.iid, dtype=np.uint32))
+    np.testing.assert_array_equal(c_pos.view(np.uint32),
+                                  from_dev(out[0]).reshape(-1, 4).view(np.uint32))
+    np.testing.assert_array_equal(c_vel.view(np.uint32),
+                                  from_dev(out[1]).reshape(-1, 4).view(np.uint32))
+
+
 @pytest.mark.parametrize("box", [(30.0, 20.0, 15.0), (6.4, 6.4, 3.2)])
 def test_large_and_small_grids_bitexact(pbf_mod, orc, device, box):
     """A sparse 300x200x150-cell box (9M cells: the cell-start scan runs its

@@ -595,7 +595,8 @@ int create_impl(const pbf_params& params, int capacity, long long cells, long lo
                o_cs = carve(4 * ((size_t)max_cells + 1)),
                o_ts = carve(4 * ((size_t)pbf::scan_tiles(max_cells) + 1)),
                o_geo = carve(sizeof(pbf::SceneGeo) * (size_t)(nscenes > 0 ? nscenes : 1)),
-               o_sid = carve(nscenes > 0 ? 2 * N : 2);
+               o_sid = carve(nscenes > 0 ? 2 * N : 2),
+               o_mask = carve(8 * pbf::kMaskBlocks * N);
   void* arena = nullptr;
   if ((e = hipMalloc(&arena, off)) != hipSuccess) {
     delete s;
@@ -621,6 +622,8 @@ int create_impl(const pbf_params& params, int capacity, long long cells, long lo
   s->hl.items = (uint4*)(b + o_items);
   s->hl.partials = (float*)(b + o_part);
   s->hl.item_cap = (uint32_t)ICAP;
+  s->hl.masks = (uint64_t*)(b + o_mask);
+  s->hl.mask_stride = (uint32_t)N;
   s->dens_err = (double*)(b + o_de);
   s->dens_partial = (double*)(b + o_dp);
   s->dens_out = (double*)(b + o_do);

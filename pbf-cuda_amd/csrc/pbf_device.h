@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace pbf {
 
 // Geometry of one scene's box and grid.  A batch of scenes shares one global
@@ -139,6 +141,26 @@ constexpr uint32_t kLevelShare = 25;
 // and misjudge the threshold, which costs time only).
 constexpr int kLevelBits = 21;
 constexpr int kLevelWords = (kLevels + 2) / 3;
+
+// Hit masks shared by the two passes of one solver iteration.  lambda_k and
+// delta-p_k read the same positions x_k and walk the same candidate sequence,
+// and every pair delta-p adds (KERNAL_EPS <= r < h, j != i) passes lambda's
+// screening test (r^2 < h^2 padded): a pass policy with kMask = kMaskStore
+// (lambda) records the 64-bit hit mask of each 64-candidate block of targets
+// with at most kMaskBlocks blocks, and one with kMask = kMaskLoad (delta-p)
+// uses those masks instead of screening.  Its add() still tests every pair it
+// is handed, so the extra bits (coincident pairs, i itself) add nothing and
+// the sums are those of its own screening.
+constexpr int kMaskNone = 0, kMaskStore = 1, kMaskLoad = 2;
+constexpr uint32_t kMaskBlocks = 4;
+template <class P, class = void>
+struct MaskMode {
+  static constexpr int value = kMaskNone;
+};
+template <class P>
+struct MaskMode<P, std::void_t<decltype(P::kMask)>> {
+  static constexpr int value = P::kMask;
+};
 
 // A work-item's candidate sequence as a compact column table in LDS: entry
 // k < nk is (first slot of the k-th non-empty column, sequence position of
@@ -295,10 +317,15 @@ struct SeqCursor1 {
 // evaluated and masked), so the compiler keeps the eight loads in flight
 // together instead of sinking some of them into conditional blocks that
 // wait for each load in turn; hit() must be branch-free as well.
-template <int G, class LoadS, class LoadF, class Hit, class Add, class BlockEnd>
+// get_mask(b, mask) may supply block b's hit mask (b = position / 64; u0
+// must be a multiple of 64) instead of screening it; put_mask(b, mask)
+// receives each mask the sweep screened.
+template <int G, class LoadS, class LoadF, class Hit, class Add, class BlockEnd, class GetMask,
+          class PutMask>
 __device__ __forceinline__ void sweep_seq(const SeqTable& st, uint32_t m, uint32_t u0,
                                           uint32_t u1, LoadS&& load_s, LoadF&& load_f, Hit&& hit,
-                                          Add&& add, BlockEnd&& block_end) {
+                                          Add&& add, BlockEnd&& block_end, GetMask&& get_mask,
+                                          PutMask&& put_mask) {
   if (u0 >= u1) return;
   SeqCursor c1;
   c1.init(st, u0, m);
@@ -308,18 +335,21 @@ __device__ __forceinline__ void sweep_seq(const SeqTable& st, uint32_t m, uint32
   for (uint32_t bb = u0; bb < u1; bb += 64) {
     const uint32_t be = u1 - bb < 64u ? u1 : bb + 64u;
     uint64_t mask = 0;
+    if (!get_mask(bb >> 6, mask)) {
 #pragma unroll 1
-    for (uint32_t u = bb; u < be; u += 8) {
-      uint32_t j[8];
-      c1.slots8(u, be - 1u, j);
-      decltype(load_s(0u)) d[8];
+      for (uint32_t u = bb; u < be; u += 8) {
+        uint32_t j[8];
+        c1.slots8(u, be - 1u, j);
+        decltype(load_s(0u)) d[8];
 #pragma unroll
-      for (int t = 0; t < 8; ++t) d[t] = load_s(j[t]);
-      uint32_t bits = 0;
+        for (int t = 0; t < 8; ++t) d[t] = load_s(j[t]);
+        uint32_t bits = 0;
 #pragma unroll
-      for (int t = 0; t < 8; ++t) bits |= (hit(j[t], d[t]) ? 1u : 0u) << t;
-      const uint32_t valid = be - u >= 8u ? 0xFFu : (1u << (be - u)) - 1u;
-      mask |= (uint64_t)(bits & valid) << (u - bb);
+        for (int t = 0; t < 8; ++t) bits |= (hit(j[t], d[t]) ? 1u : 0u) << t;
+        const uint32_t valid = be - u >= 8u ? 0xFFu : (1u << (be - u)) - 1u;
+        mask |= (uint64_t)(bits & valid) << (u - bb);
+      }
+      put_mask(bb >> 6, mask);
     }
     uint32_t last = 0;
 #pragma unroll 1
@@ -374,6 +404,10 @@ struct HeavyLists {
   uint4* items;        // per chunk: (i, chunk index, partial slot, 0); i = ~0u: unused
   float* partials;     // kPartialWords floats per partial slot
   uint32_t item_cap;   // capacity of items / partials
+  // hit masks of the current iteration: block b of target i at
+  // masks[b * mask_stride + i], b < kMaskBlocks (null: not kept)
+  uint64_t* masks;
+  uint32_t mask_stride;
 };
 constexpr int kPartialWords = 8;
 constexpr int kCounterLine = 64;  // words

@@ -33,6 +33,31 @@ constexpr int kRankBuckets = 64;  // one wavefront scans them
 inline int blocks_for(long long n) { return (int)((n + kBlock - 1) / kBlock); }
 
 // ---------------------------------------------------------------------------
+// The hit-mask hooks of sweep_seq for target ti with m candidates under
+// policy P (MaskMode, pbf_device.h).
+template <class P>
+struct MaskIO {
+  const HeavyLists& hl;
+  uint32_t ti, m;
+  __device__ __forceinline__ bool kept() const {
+    return hl.masks != nullptr && m <= kMaskBlocks * kChunk;
+  }
+  __device__ __forceinline__ bool get(uint32_t b, uint64_t& mask) const {
+    if constexpr (MaskMode<P>::value == kMaskLoad) {
+      if (kept()) {
+        mask = hl.masks[(size_t)b * hl.mask_stride + ti];
+        return true;
+      }
+    }
+    return false;
+  }
+  __device__ __forceinline__ void put(uint32_t b, uint64_t mask) const {
+    if constexpr (MaskMode<P>::value == kMaskStore) {
+      if (kept()) hl.masks[(size_t)b * hl.mask_stride + ti] = mask;
+    }
+  }
+};
+
 // Generic neighbour-pass kernels over a pass policy.
 // ---------------------------------------------------------------------------
 
@@ -173,12 +198,15 @@ __global__ __launch_bounds__(kBlock) void k_light(typename P::Args a, int n, Ste
   // and added to the total in order (also for a heavy target whose chunk
   // list was full)
   typename P::Acc s, part;
+  const MaskIO<P> mio{hl, ti, ms};
   sweep_seq<P::kHitBatch>(
       sts, ms, 0u, ms, ls, lf, hit, [&](uint32_t j, const auto& d) { P::add(part, t, j, d, c, a); },
       [&] {
         P::accumulate(s, part);
         part = typename P::Acc{};
-      });
+      },
+      [&](uint32_t b, uint64_t& mk) { return mio.get(b, mk); },
+      [&](uint32_t b, uint64_t mk) { mio.put(b, mk); });
   P::finish(a, t, s, c, sc.at(ti, c));
 }
 
@@ -210,11 +238,14 @@ __global__ __launch_bounds__(kBlock) void k_chunks(typename P::Args a, StepConst
     typename P::Acc part;
     const uint32_t s0 = it.y * kChunk;
     const uint32_t s1 = s0 + kChunk < m ? s0 + kChunk : m;
+    const MaskIO<P> mio{hl, it.x, m};
     sweep_seq<P::kHitBatch>(
         st, m, s0, s1, [&](uint32_t j) { return P::load_s(a, j); },
         [&](uint32_t j) { return P::load_f(a, j); },
         [&](uint32_t j, const auto& d) { return P::hit(t, j, d, c); },
-        [&](uint32_t j, const auto& d) { P::add(part, t, j, d, c, a); }, [] {});
+        [&](uint32_t j, const auto& d) { P::add(part, t, j, d, c, a); }, [] {},
+        [&](uint32_t b, uint64_t& mk) { return mio.get(b, mk); },
+        [&](uint32_t b, uint64_t mk) { mio.put(b, mk); });
     static_assert(sizeof(part) <= kPartialWords * sizeof(float), "partial too large");
     *reinterpret_cast<typename P::Acc*>(hl.partials + (size_t)it.z * kPartialWords) = part;
   }

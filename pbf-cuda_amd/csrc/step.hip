@@ -34,6 +34,7 @@ namespace {
 // term and lambda = -(rho/pho0 - 1) / (gradj_l2 + |grad_i|^2 + eps).
 struct LambdaPass {
   static constexpr bool kXcdRanges = true;  // one contiguous range per XCD
+  static constexpr int kMask = kMaskStore;  // hit masks for the delta-p pass
   // hits reloaded per round of the sweep
   static constexpr int kHitBatch = 4;
   struct Args {
@@ -115,6 +116,7 @@ struct LambdaPass {
 // replaces thrust::copy_n at fluids/Simulator.cu:247-248).
 struct DeltaPass {
   static constexpr bool kXcdRanges = true;  // one contiguous range per XCD
+  static constexpr int kMask = kMaskLoad;   // the lambda pass's hit masks
   // hits reloaded per round of the sweep
   static constexpr int kHitBatch = 4;
   struct Args {

@@ -79,7 +79,7 @@ typedef struct pbf_solver pbf_solver;
  * times the initial box cells; here the caller states `capacity` (maximum
  * particles) and `max_cells` (maximum grid cells over the solver's life;
  * <= 0 means 4x the cells of the initial box, as the reference).
- * Allocates device scratch (hipMalloc), so it synchronises: ~(500 B per
+ * Allocates device scratch (hipMalloc), so it synchronises: ~(530 B per
  * particle of capacity + 8 B per cell), most of it the work list of crowded
  * cells (8 chunk items of 48 B per particle, enough for piles of ~1/3 of a
  * million particles in one cell; beyond the list a crowded target is summed

@@ -314,6 +314,37 @@ def test_sweep_dam_4m_piled_frame_bitexact(pbf_mod, orc, device):
                                   m["nvel"].view(np.uint32))
 
 
+def test_hit_masks_around_their_256_candidate_limit(pbf_mod, orc, device):
+    """A compressed lattice (spacing 0.45h) whose targets have from ~100 to
+    ~300 candidates, on both sides of the 256 up to which the lambda pass
+    keeps hit masks for the delta-p pass: bit-exact against the FP32 mirror
+    for three frames, with the adaptive heavy threshold and with every target
+    up to 256 candidates in the chunk kernels (threshold 64)."""
+    p = P.dam_params()
+    base = scenes.block((24, 24, 24), (2.0, 2.0, 2.0), seed=57, vel_sigma=0.3)
+    sc = scenes.Scene((base.pos * np.float32(0.9)).astype(np.float32), base.vel, base.iid,
+                      base.ulim, base.llim)
+    pos, vel = sc.as_layout(4)
+    m0 = mirror(orc, p, sc, pos, vel, sc.iid)
+    # candidate counts of the first pass (27 cells around each sorted key)
+    D = (20, 20, 20)
+    cnt = np.bincount(m0["keys"].astype(np.int64), minlength=D[0] * D[1] * D[2]).reshape(D)
+    pad = np.pad(cnt, 1)
+    nb = sum(pad[dx:dx + D[0], dy:dy + D[1], dz:dz + D[2]]
+             for dx in range(3) for dy in range(3) for dz in range(3))
+    m = nb.reshape(-1)[m0["keys"].astype(np.int64)]
+    assert (m <= 256).sum() > 1000 and (m > 256).sum() > 1000, (m.min(), m.max())
+    for thr in (0, 64):
+        run = Run(pbf_mod, sc, p)
+        run.sim.set_heavy_threshold(thr)
+        P4, V4, I = pos, vel, sc.iid
+        for f in range(3):
+            g = run.step()
+            mm = mirror(orc, p, sc, P4, V4, I)
+            assert_bitexact(g, mm, f"threshold {thr} frame {f}")
+            P4, V4, I = mm["npos"], mm["nvel"], mm["iid"]
+
+
 def test_cpp_host_program_equals_run_frames(pbf_mod, device, tmp_path):
     """pbf-cuda_amd/examples/stepsim, FluidSystem::stepSimulate written as a
     C++ host program over include/pbf_simulator.hpp (setLim + step per frame,

@@ -79,11 +79,12 @@ typedef struct pbf_solver pbf_solver;
  * times the initial box cells; here the caller states `capacity` (maximum
  * particles) and `max_cells` (maximum grid cells over the solver's life;
  * <= 0 means 4x the cells of the initial box, as the reference).
- * Allocates device scratch (hipMalloc), so it synchronises: ~(530 B per
+ * Allocates device scratch (hipMalloc), so it synchronises: ~(750 B per
  * particle of capacity + 8 B per cell), most of it the work list of crowded
  * cells (8 chunk items of 48 B per particle, enough for piles of ~1/3 of a
  * million particles in one cell; beyond the list a crowded target is summed
- * by its own work-item, correct but slow).
+ * by its own work-item, correct but slow) and the hit masks the two passes of
+ * an iteration share (256 B per particle).
  */
 int pbf_create(const pbf_params* params, pbf_vec3 ulim, pbf_vec3 llim,
                int capacity, long long max_cells, int layout,

@@ -147,12 +147,13 @@ constexpr int kLevelWords = (kLevels + 2) / 3;
 // and every pair delta-p adds (KERNAL_EPS <= r < h, j != i) passes lambda's
 // screening test (r^2 < h^2 padded): a pass policy with kMask = kMaskStore
 // (lambda) records the 64-bit hit mask of each 64-candidate block of targets
-// with at most kMaskBlocks blocks, and one with kMask = kMaskLoad (delta-p)
+// with at most kMaskBlocks blocks (2048 candidates: all but the deepest corner
+// piles), and one with kMask = kMaskLoad (delta-p)
 // uses those masks instead of screening.  Its add() still tests every pair it
 // is handed, so the extra bits (coincident pairs, i itself) add nothing and
 // the sums are those of its own screening.
 constexpr int kMaskNone = 0, kMaskStore = 1, kMaskLoad = 2;
-constexpr uint32_t kMaskBlocks = 4;
+constexpr uint32_t kMaskBlocks = 32;
 template <class P, class = void>
 struct MaskMode {
   static constexpr int value = kMaskNone;

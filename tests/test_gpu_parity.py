@@ -314,16 +314,22 @@ def test_sweep_dam_4m_piled_frame_bitexact(pbf_mod, orc, device):
                                   m["nvel"].view(np.uint32))
 
 
-def test_hit_masks_around_their_256_candidate_limit(pbf_mod, orc, device):
-    """A compressed lattice (spacing 0.45h) whose targets have from ~100 to
-    ~300 candidates, on both sides of the 256 up to which the lambda pass
-    keeps hit masks for the delta-p pass: bit-exact against the FP32 mirror
-    for three frames, with the adaptive heavy threshold and with every target
-    up to 256 candidates in the chunk kernels (threshold 64)."""
+def test_hit_masks_around_their_candidate_limit(pbf_mod, orc, device):
+    """The lambda pass keeps hit masks for the delta-p pass of targets with at
+    most 2048 candidates (32 blocks of 64).  A compressed lattice (spacing
+    0.45h, ~60 to ~340 candidates) with two piles of identical positions in
+    its corners (1,800 and 2,300 particles) has targets on both sides of that
+    limit and with every block count up to it: bit-exact against the FP32
+    mirror for three frames, with the adaptive heavy threshold and with every
+    target above 64 candidates in the chunk kernels."""
     p = P.dam_params()
     base = scenes.block((24, 24, 24), (2.0, 2.0, 2.0), seed=57, vel_sigma=0.3)
-    sc = scenes.Scene((base.pos * np.float32(0.9)).astype(np.float32), base.vel, base.iid,
-                      base.ulim, base.llim)
+    piles = [np.tile(np.array([[0.03, 0.03, 0.03]], np.float32), (1800, 1)),
+             np.tile(np.array([[1.97, 1.97, 0.03]], np.float32), (2300, 1))]
+    pos3 = np.concatenate([(base.pos * np.float32(0.9)).astype(np.float32)] + piles)
+    vel3 = np.concatenate([base.vel] + [np.zeros_like(q) for q in piles])
+    sc = scenes.Scene(pos3, vel3, np.arange(pos3.shape[0], dtype=np.uint32), base.ulim,
+                      base.llim)
     pos, vel = sc.as_layout(4)
     m0 = mirror(orc, p, sc, pos, vel, sc.iid)
     # candidate counts of the first pass (27 cells around each sorted key)
@@ -333,7 +339,8 @@ def test_hit_masks_around_their_256_candidate_limit(pbf_mod, orc, device):
     nb = sum(pad[dx:dx + D[0], dy:dy + D[1], dz:dz + D[2]]
              for dx in range(3) for dy in range(3) for dz in range(3))
     m = nb.reshape(-1)[m0["keys"].astype(np.int64)]
-    assert (m <= 256).sum() > 1000 and (m > 256).sum() > 1000, (m.min(), m.max())
+    assert (m <= 256).sum() > 1000 and ((m > 256) & (m <= 2048)).sum() > 1000, (m.min(), m.max())
+    assert (m > 2048).sum() > 1000, m.max()
     for thr in (0, 64):
         run = Run(pbf_mod, sc, p)
         run.sim.set_heavy_threshold(thr)

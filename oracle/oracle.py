@@ -73,7 +73,7 @@ def load():
     lib.orc_step.restype = ctypes.c_int
     lib.orc_density_error.argtypes = [ctypes.POINTER(OrcParams), ctypes.POINTER(OrcBox),
                                       ctypes.c_int, _F, ctypes.c_int, ctypes.c_int,
-                                      ctypes.c_double * 2]
+                                      ctypes.c_double * 2, _D]
     lib.orc_density_error.restype = ctypes.c_int
     lib.orc_step_f32.argtypes = lib.orc_step.argtypes
     lib.orc_step_f32.restype = ctypes.c_int
@@ -198,13 +198,19 @@ def step(params, ulim, llim, pos: np.ndarray, vel: np.ndarray, iid: np.ndarray, 
     return out
 
 
-def density_error(params, ulim, llim, x: np.ndarray, threads: int = 0):
+def density_error(params, ulim, llim, x: np.ndarray, threads: int = 0,
+                  per_particle: bool = False):
+    """(mean, max) of |rho_i/pho0 - 1| with FP64 rho on a fresh grid; with
+    per_particle, also the per-particle errors in input order."""
     x = np.ascontiguousarray(x, np.float32)
     out = (ctypes.c_double * 2)()
+    err = np.zeros(x.shape[0]) if per_particle else None
     rc = load().orc_density_error(ctypes.byref(params_c(params)), ctypes.byref(box_c(ulim, llim)),
-                                  x.shape[0], _fp(x), x.shape[1], int(threads), out)
+                                  x.shape[0], _fp(x), x.shape[1], int(threads), out, _dp(err))
     if rc != 0:
         raise ValueError("orc_density_error rejected its input")
+    if per_particle:
+        return float(out[0]), float(out[1]), err
     return float(out[0]), float(out[1])
 
 

@@ -1194,8 +1194,8 @@ int orc_step_f32(const orc_params* p, const orc_box* box, int n, float* pos, flo
 }
 
 int orc_density_error(const orc_params* p, const orc_box* box, int n, const float* x, int stride,
-                      int threads, double out[2]) {
-  return density_impl<double>(p, box, n, x, stride, threads, out, nullptr);
+                      int threads, double out[2], double* err_out) {
+  return density_impl<double>(p, box, n, x, stride, threads, out, err_out);
 }
 
 int orc_density_error_f32(const orc_params* p, const orc_box* box, int n, const float* x,

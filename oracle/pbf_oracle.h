@@ -93,9 +93,11 @@ int orc_step_f32(const orc_params* p, const orc_box* box, int n, float* pos, flo
                  float* vel, float* nvel, uint32_t* iid, const orc_options* opt, orc_diag* diag);
 
 /* mean/max |rho_i/pho0 - 1| with rho from a fresh grid over `x` (stride
- * floats per particle), self included, plus the wall term.  FP64 rho. */
+ * floats per particle), self included, plus the wall term.  FP64 rho;
+ * err_out (may be NULL) receives |rho_i/pho0 - 1| per particle in input
+ * order. */
 int orc_density_error(const orc_params* p, const orc_box* box, int n, const float* x,
-                      int stride, int threads, double out[2]);
+                      int stride, int threads, double out[2], double* err_out);
 
 /* Same with FP32 rho (the GPU's operation sequence); err_out (may be NULL)
  * receives |rho_i/pho0 - 1| per particle in input order. */

@@ -12,10 +12,21 @@ inputs resident in HBM, bracketed by torch.cuda.synchronize(), timed with
 HIP events on the launch stream.  The launch shapes come from the library's
 autotuner run on the warmed-up state; the timed steps replay a recorded HIP
 graph (the eager figure is reported beside it, from a second simulation
-over the same frames).  Also measured: the 64-scene batched rate, the
-backward/forward ratio of the differentiable rollout, and the step time of
-one GPU's share of an 8-way x-slab split (slab plus one-cell halo, stepped
-as its own scene).  One JSON line on stdout.
+over the same frames).  Also measured: the step rate of later frame
+windows, the 64-scene batched rate, the backward/forward ratio of the
+differentiable rollout (16,384 particles, 30 frames), the 4,194,304-particle
+sweeping-wall workload with its backward over the last 10 frames, and the
+step time of one GPU's share of an 8-way x-slab split (slab plus one-cell
+halo, stepped as its own scene).  One JSON line on stdout.
+
+`roofline` describes the lambda pass (light + chunk + combine kernels of one
+lambda launch), the pass with the largest share of the step: algorithmic
+bytes 20 B per particle (read x 12 B, write lambda and rho 8 B), divided by
+its mean launch duration measured with HIP events on the launch stream over
+the timed frames of the eager run.  `traffic` and the VALU figures come from
+the rocprofv3 summaries committed under profiles/ (counter runs of this same
+command); `python bench.py --summarize DIR` turns a rocprofv3 output folder
+of this command into the per-kernel figures of the timed frames.
 """
 import argparse
 import json
@@ -29,11 +40,14 @@ sys.path.insert(0, os.path.join(ROOT, "pbf-cuda_amd"))
 
 METRIC = "particle-solver-iters/sec at 1M particles; batched rate; backward/forward ratio"
 HBM_PEAK_GBS = 8000.0
-# Algorithmic bytes per particle of one delta-p launch: read x (12 B) and
-# lambda (4 B), write x (12 B).
-DELTA_BYTES_PER_PARTICLE = 28.0
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_delta_single_dam_1m_traffic.md")
-SQ_FILE = os.path.join(ROOT, "profiles", "neighbour_kernels_single_dam_1m_sq_counters.md")
+# Algorithmic bytes per particle of one lambda launch: read x (12 B), write
+# lambda and rho (8 B).
+LAMBDA_BYTES_PER_PARTICLE = 20.0
+# Non-packed FP32 VALU issue peak: 256 CUs x 64 lanes x 2.4 GHz, i.e. one
+# wave64 instruction per CU per clock (the build keeps packed FP32 off).
+VALU_WAVE_INSTS_PER_S = 256 * 2.4e9
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "lambda_pass_single_dam_1m_traffic.md")
+SQ_FILE = os.path.join(ROOT, "profiles", "lambda_pass_single_dam_1m_sq_counters.md")
 
 
 def parse_args():
@@ -42,52 +56,66 @@ def parse_args():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=3,
+                    help="timed CPU oracle steps per configuration (after one warm-up step)")
     ap.add_argument("--no-graph", action="store_true", help="time eager steps only")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batched rate")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the backward/forward ratio")
     ap.add_argument("--no-slab", action="store_true",
                     help="skip the one-slab share of an 8-way x-slab split")
-    ap.add_argument("--sweep", action="store_true",
-                    help="also run the 4,194,304-particle sweeping-wall workload: 120 frames, "
-                         "6 iterations, backward over the last 10 frames")
+    ap.add_argument("--no-sweep", action="store_true",
+                    help="skip the 4,194,304-particle sweeping-wall workload (120 frames, 6 "
+                         "iterations, backward over the last 10 frames)")
+    ap.add_argument("--windows", type=int, default=5,
+                    help="later windows of `steps` frames timed after the headline window")
     ap.add_argument("--copy-only", action="store_true",
                     help="only run a 256 MiB device-to-device copy 3 times (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters against known bytes)")
+    ap.add_argument("--summarize", metavar="DIR",
+                    help="no GPU work: summarise a rocprofv3 output folder of this command "
+                         "(kernel trace or counter collection CSV) over the timed frames")
     return ap.parse_args()
 
 
-def measured_traffic():
-    """HBM bytes per k_delta launch from the committed rocprofv3 summary, or None."""
+def _profile_fields(path, keys):
+    """Numbers `key: value` from a committed rocprofv3 summary, or None."""
     try:
-        text = open(TRAFFIC_FILE).read()
-    except OSError:
-        return None
-    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", text)
-    return float(m.group(1)) if m else None
-
-
-def measured_valu():
-    """VALU issue figures of the delta-p light kernel from the committed
-    rocprofv3 SQ counter summary, or None: wave-instructions per launch, the
-    time they take at 256 CUs x 128 FP32 lanes x 2.4 GHz, and the traced
-    launch duration."""
-    try:
-        text = open(SQ_FILE).read()
+        text = open(path).read()
     except OSError:
         return None
     got = {}
-    for key in ("valu_wave_insts_per_launch", "valu_issue_us_per_launch", "traced_launch_us"):
-        m = re.search(key + r":\s*([0-9.eE+]+)", text)
+    for key in keys:
+        m = re.search(re.escape(key) + r":\s*([0-9.eE+-]+)", text)
         if not m:
             return None
         got[key] = float(m.group(1))
-    return {"kernel": "k_light<DeltaPass> (rocprofv3 SQ_INSTS_VALU, timed frames)",
+    return got
+
+
+def measured_traffic():
+    """HBM bytes per lambda launch (FETCH_SIZE + WRITE_SIZE, scaled by the
+    copy-kernel factors) from the committed rocprofv3 summary, or None."""
+    got = _profile_fields(TRAFFIC_FILE, ("traffic_bytes_per_launch",))
+    return None if got is None else got["traffic_bytes_per_launch"]
+
+
+def measured_valu():
+    """VALU issue of the lambda light kernel from the committed rocprofv3 SQ
+    summary, against the non-packed peak (one wave64 instruction per CU per
+    clock), or None."""
+    got = _profile_fields(SQ_FILE, ("valu_wave_insts_per_launch", "traced_launch_us",
+                                    "lane_utilisation"))
+    if got is None:
+        return None
+    issue_us = got["valu_wave_insts_per_launch"] / VALU_WAVE_INSTS_PER_S * 1e6
+    return {"kernel": "k_light<LambdaPass> (rocprofv3 SQ_INSTS_VALU over the timed frames)",
             "wave_insts_per_launch": got["valu_wave_insts_per_launch"],
-            "issue_us": got["valu_issue_us_per_launch"], "launch_us": got["traced_launch_us"],
-            "frac": got["valu_issue_us_per_launch"] / got["traced_launch_us"],
-            "peak": "256 CUs x 128 FP32 lanes x 2.4 GHz"}
+            "issue_us": round(issue_us, 2), "launch_us": got["traced_launch_us"],
+            "frac": issue_us / got["traced_launch_us"],
+            "lane_utilisation": got["lane_utilisation"],
+            "peak": "256 CUs x 64 FP32 lanes x 2.4 GHz (non-packed)",
+            "source": os.path.relpath(SQ_FILE, ROOT)}
 
 
 def step_bytes(n, niter, cells):
@@ -100,29 +128,37 @@ def step_bytes(n, niter, cells):
     return n * (240 + 48 * niter) + 8 * cells
 
 
-def cpu_baseline(p, sc, pos_np, vel_np, iid_np, steps):
-    """FP64 oracle (the checker, test infrastructure) timed on the host cores
-    on the same frame the GPU last stepped."""
+def cpu_baseline(cases, steps):
+    """FP64 oracle (the checker, test infrastructure) timed on the host cores:
+    one warm-up step then `steps` timed steps of each (name, params, scene,
+    pos, vel, iid, sample) case, single-threaded and on all OpenMP threads.
+    value = the 1M single dam on all threads."""
     sys.path.insert(0, ROOT)
     from oracle import oracle as orc
 
     cores = orc.max_threads()
-    orc.step(p, sc.ulim, sc.llim, pos_np[:4096], vel_np[:4096], iid_np[:4096])  # load, warm
-    t0 = time.perf_counter()
-    done = 0
-    pos, vel, iid = pos_np, vel_np, iid_np
-    for _ in range(steps):
-        r = orc.step(p, sc.ulim, sc.llim, pos, vel, iid, threads=cores)
-        pos, vel, iid = r["npos"], r["nvel"], r["iid"]
-        done += 1
-        if time.perf_counter() - t0 > 30.0:
-            break
-    dt = (time.perf_counter() - t0) / done
-    return {"value": sc.n * p.niter / dt, "unit": "particle-solver-iters/s", "cores": cores,
+    figures = []
+    for name, p, sc, pos0, vel0, iid0, sample in cases:
+        for threads in (1, cores):
+            pos, vel, iid = pos0, vel0, iid0
+            r = orc.step(p, sc.ulim, sc.llim, pos, vel, iid, threads=threads)  # warm-up
+            pos, vel, iid = r["npos"], r["nvel"], r["iid"]
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                r = orc.step(p, sc.ulim, sc.llim, pos, vel, iid, threads=threads)
+                pos, vel, iid = r["npos"], r["nvel"], r["iid"]
+            dt = (time.perf_counter() - t0) / steps
+            figures.append({"workload": name, "threads": threads, "steps": steps,
+                            "s_per_step": round(dt, 4), "rate": sc.n * p.niter / dt,
+                            "sample": sample})
+            print(f"cpu: {name} {threads} thread(s) {dt:.3f} s/step", file=sys.stderr,
+                  flush=True)
+    head = [f for f in figures if f["workload"] == "single_dam_1m" and f["threads"] == cores][0]
+    return {"value": head["rate"], "unit": "particle-solver-iters/s", "cores": cores,
             "kind": "port",
-            "sample": f"{done} FP64 oracle step(s) of the {sc.n}-particle single dam from the "
-                      f"GPU's state after the timed frames, {cores} OpenMP threads; "
-                      f"{dt:.2f} s per step"}
+            "sample": f"FP64 oracle, {steps} timed steps after 1 warm-up step, "
+                      f"{cores} OpenMP threads, {head['sample']}",
+            "figures": figures}
 
 
 class SingleRun:
@@ -194,17 +230,21 @@ def timed(run, steps, stream):
 GRAD_NOTE = ("with the dam constants the exact derivative of the rollout grows by ~1e3-1e4 per "
              "frame (Delta-p clamped at MAX_DP makes the map piecewise smooth); the FP64 "
              "oracle's gradient of the 30-frame 16K rollout is ~1e110, beyond the FP32 range, so "
-             "non-finite gradients here are expected; gradient parity is tested at short horizons")
+             "non-finite gradients are expected there; with the reference constants the 30-frame "
+             "gradient is finite and is checked in tests/test_gpu_adjoint.py against the oracle's "
+             "adjoint of the same trajectory (1e-4) and, over 15 frames, against FP64 central "
+             "differences (1e-3)")
 
 
 def adjoint_ratio(pbf, p, dev, stream, reps=3, store=True):
     """Backward / forward time of the 30-frame differentiable rollout of the
-    16,384-particle scene (dam constants, 4 iterations): forward = the tape's
+    16,384-particle scene (4 iterations, constants `p`): forward = the tape's
     30 frames, backward = dL/d(pos0, vel0, pho0) through the 30 frames.
-    store=True: the forward run keeps every frame's iterations, sort and grid
-    on the device and the backward pass reads them; store=False: it keeps
-    only per-frame checkpoints and the backward pass recomputes each frame's
-    iterations."""
+    store=False: the forward run keeps only per-frame checkpoints and the
+    backward pass recomputes each frame's solver iterations (the mode the
+    workload specifies); store=True: the forward run keeps every frame's
+    iterations, sort and grid on the device and the backward pass reads
+    them."""
     import numpy as np
     import torch
 
@@ -224,6 +264,7 @@ def adjoint_ratio(pbf, p, dev, stream, reps=3, store=True):
     gpos = torch.zeros(n, 4, device=dev)
     gp0 = torch.empty(n * 4, device=dev)
     gv0 = torch.empty(n * 4, device=dev)
+    loss = [0.0]
 
     def forward():
         tape.forward(pos0, vel0, iid0, stream)
@@ -231,7 +272,9 @@ def adjoint_ratio(pbf, p, dev, stream, reps=3, store=True):
     def backward():
         tape.read(nf, pos=xf, iid=iidf, stream=stream)
         x = xf.view(n, 4)[:, :3].double()
-        gpos[:, :3] = (2.0 * (x - tgt[iidf.long()]) / n).float()
+        dx = x - tgt[iidf.long()]
+        loss[0] = dx
+        gpos[:, :3] = (2.0 * dx / n).float()
         return tape.backward(gpos.view(-1), None, gp0, gv0, stream)
 
     def plain():
@@ -262,16 +305,41 @@ def adjoint_ratio(pbf, p, dev, stream, reps=3, store=True):
         t_b = time_it(backward)
         t_p = time_it(plain)
     gv = gv0.view(n, 4)[:, :3]
+    finite = bool(torch.isfinite(gv).all().item()) and bool(np.isfinite(grho))
     return {"value": t_b / t_f, "unit": "backward time / forward time",
             "tape": "stored iterations" if store else "checkpoints, iterations recomputed",
             "forward_ms": t_f, "backward_ms": t_b, "plain_forward_ms": t_p,
             "backward_over_plain_forward": t_b / t_p,
-            "gradients_finite": bool(torch.isfinite(gv).all().item()) and bool(np.isfinite(grho)),
-            "gradients_note": GRAD_NOTE,
+            "loss": float((loss[0] ** 2).sum().item() / n),
+            "grad_vel0_norm": float(gv.double().norm().item()), "grad_pho0": float(grho),
+            "gradients_finite": finite}
+
+
+def c4_workload(pbf, dev, stream):
+    """The differentiable workload under both constant sets: the reference
+    defaults (finite, checked gradients; the headline) and the dam constants
+    (timing only, see GRAD_NOTE); each with the recomputing and the storing
+    tape."""
+    out = {}
+    for name, p in (("reference_constants", pbf.reference_defaults()),
+                    ("dam_constants", pbf.dam_params(niter=4))):
+        rec = adjoint_ratio(pbf, p, dev, stream, store=False)
+        sto = adjoint_ratio(pbf, p, dev, stream, store=True)
+        out[name] = {"recompute": rec, "stored": sto}
+        print(f"adjoint {name}: forward {rec['forward_ms']:.2f} ms, backward "
+              f"{rec['backward_ms']:.2f} ms, ratio {rec['value']:.3f} (stored tape "
+              f"{sto['value']:.3f}), finite {rec['gradients_finite']}", file=sys.stderr,
+              flush=True)
+    head = out["reference_constants"]["recompute"]
+    return {"value": head["value"], "unit": "backward time / forward time",
+            "tape": head["tape"], "constants": "reference defaults (FluidSystem.cpp:15-25)",
+            "forward_ms": head["forward_ms"], "backward_ms": head["backward_ms"],
+            "gradients_finite": head["gradients_finite"],
             "workload": "differentiable_16k: 16,384 particles (32x16x32 jittered lattice, "
-                        "v0 ~ N(0, 0.5^2)), box 40h x 20h x 40h, 30 frames, 4 iterations, dam "
-                        "constants; loss = mean squared distance to a seeded target cloud; "
-                        "gradients w.r.t. pos0, vel0, pho0"}
+                        "v0 ~ N(0, 0.5^2)), box 40h x 20h x 40h, 30 frames, 4 iterations; loss = "
+                        "mean squared distance to a seeded target cloud; gradients w.r.t. pos0, "
+                        "vel0, pho0",
+            "gradients_note": GRAD_NOTE, "by_constants": out}
 
 
 def slab_share(pbf, p, pos, vel, iid, ulim, llim, shapes, dev, stream, parts=8, warmup=3,
@@ -386,16 +454,23 @@ def sweep_workload(pbf, dev, stream, frames=120, back=10):
             gp0 = torch.empty(n * 4, device=dev)
             gv0 = torch.empty(n * 4, device=dev)
             ms_b, grho = span(lambda: tape.backward(g.view(-1), None, gp0, gv0, stream))
-            gv = gv0.view(n, 4)[:, :3]
+            gv = gv0.view(n, 4)[:, :3].double()
             key = "stored" if store else "recompute"
             out[key] = {"tape_forward_ms": ms_f, "backward_ms": ms_b,
                         "backward_forward_ratio": ms_b / ms_f, "tape_device_bytes": int(tape_bytes),
                         "gradients_finite": bool(torch.isfinite(gv).all().item())
                         and bool(np.isfinite(grho)),
-                        "gradients_note": GRAD_NOTE}
+                        "grad_vel_norm": float(gv.norm().item()), "grad_pho0": float(grho)}
+            if store:
+                gv_stored = gv.clone()
+            else:
+                out["recompute"]["rel_diff_to_stored_tape"] = float(
+                    ((gv - gv_stored).norm() / gv_stored.norm()).item())
             tape.close()
             del tape, xf, iidf, g, gp0, gv0
-    out["backward_forward_ratio"] = out["stored"]["backward_forward_ratio"]
+    out["backward_forward_ratio"] = out["recompute"]["backward_forward_ratio"]
+    out["tape"] = "checkpoints, iterations recomputed"
+    out["gradients_note"] = GRAD_NOTE
     out["workload"] = ("sweep_dam_4m: 256x64x256 jittered lattice in a 160h x 64h x 160h box, "
                        "+x wall 160h + 20h sin(2 pi f/120), dam constants, 6 iterations; "
                        f"frames 0..{plain - 1} plain, {plain}..{frames - 1} on a tape, "
@@ -403,8 +478,93 @@ def sweep_workload(pbf, dev, stream, frames=120, back=10):
     return out
 
 
+def _short_kernel(name):
+    n = name.replace("pbf::(anonymous namespace)::", "").replace("pbf::", "")
+    if n.startswith("void "):
+        n = n[5:]
+    return n.split("(")[0]
+
+
+def summarize(folder, warmup, steps, windows, n=1048576):
+    """Per-kernel figures of the timed frames from a rocprofv3 output folder
+    of this command (no GPU work).  Finds the single-dam steps by their
+    predict kernel (k_predict_key<4, OneScene> over n particles): the eager
+    run's `warmup` steps, the autotuner's steps, its `steps` timed steps, then
+    the graph run's `warmup` + steps * (1 + windows) steps.  Prints, for the
+    eager and the graph run's headline window, each kernel's launches per
+    step and mean duration (kernel trace) or mean counter values per launch
+    (counter collection), and the lambda pass's totals per launch."""
+    import collections
+    import csv
+    import glob
+
+    trace = glob.glob(os.path.join(folder, "**", "*kernel_trace.csv"), recursive=True)
+    ctrs = glob.glob(os.path.join(folder, "**", "*counter_collection.csv"), recursive=True)
+    if not trace and not ctrs:
+        sys.exit(f"--summarize: no kernel_trace.csv or counter_collection.csv under {folder}")
+    disp = {}  # dispatch id -> [name, grid, start, end, {counter: value}]
+    for path in trace + ctrs:
+        for r in csv.DictReader(open(path)):
+            d = int(r["Dispatch_Id"])
+            if d not in disp:
+                grid = int(r.get("Grid_Size_X") or r.get("Grid_Size") or 0)
+                disp[d] = [r["Kernel_Name"], grid, int(r["Start_Timestamp"]),
+                           int(r["End_Timestamp"]), {}]
+            if "Counter_Name" in r:
+                c = disp[d][4]
+                c[r["Counter_Name"]] = c.get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+    order = sorted(disp)
+    pred = [k for k in order if "k_predict_key<4, pbf::OneScene>" in disp[k][0]
+            and disp[k][1] >= n and disp[k][1] < n + 256]
+    tuned = len(pred) - 2 * warmup - steps * (2 + windows)
+    if tuned < 0:
+        sys.exit(f"--summarize: {len(pred)} single-dam steps, fewer than the command runs")
+    spans = {"eager": (warmup + tuned, warmup + tuned + steps),
+             "graph": (2 * warmup + tuned + steps, 2 * warmup + tuned + 2 * steps)}
+    lines = [f"# rocprofv3 summary of `bench.py --steps {steps} --warmup {warmup}` "
+             f"({len(pred)} single-dam steps, {tuned} of them by the autotuner)",
+             f"# frames {warmup}..{warmup + steps - 1} of the 1,048,576-particle single dam"]
+    for run, (a, b) in spans.items():
+        lo = pred[a]
+        # the window ends where the next step starts (the last step of the
+        # command: as many dispatches as the step before it)
+        hi = pred[b] if b < len(pred) else pred[b - 1] + (pred[b - 1] - pred[b - 2])
+        sel = [k for k in order if lo <= k < hi]
+        per = collections.OrderedDict()
+        for k in sel:
+            nm = _short_kernel(disp[k][0])
+            per.setdefault(nm, []).append(disp[k])
+        lines.append(f"\n## {run} run, {steps} steps (dispatches {lo}..{hi - 1})")
+        tot_ns = sum(d[3] - d[2] for k in sel for d in [disp[k]])
+        lines.append(f"kernel time per step: {tot_ns / steps / 1e3:.1f} us")
+        names = sorted(per, key=lambda x: -sum(d[3] - d[2] for d in per[x]))
+        for nm in names:
+            ds = per[nm]
+            mean_us = sum(d[3] - d[2] for d in ds) / len(ds) / 1e3
+            row = f"- {nm}: {len(ds) / steps:g} per step, mean {mean_us:.2f} us"
+            cnames = sorted({c for d in ds for c in d[4]})
+            if cnames:
+                row += "; per launch " + ", ".join(
+                    f"{c} {sum(d[4].get(c, 0.0) for d in ds) / len(ds):.6g}" for c in cnames)
+            lines.append(row)
+        lam = [nm for nm in per if "LambdaPass, pbf::OneScene" in nm]
+        if lam:
+            launches = len(per[[x for x in lam if x.startswith("k_light")][0]])
+            us = sum(d[3] - d[2] for x in lam for d in per[x]) / launches / 1e3
+            lines.append(f"lambda pass (k_light + k_chunks + k_combine) per launch: "
+                         f"{us:.2f} us over {launches} launches")
+            cn = sorted({c for x in lam for d in per[x] for c in d[4]})
+            for c in cn:
+                v = sum(d[4].get(c, 0.0) for x in lam for d in per[x]) / launches
+                lines.append(f"lambda pass {c} per launch: {v:.6g}")
+    print("\n".join(lines))
+
+
 def main():
     args = parse_args()
+    if args.summarize:
+        summarize(args.summarize, args.warmup, args.steps, args.windows)
+        return
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus}: the benchmark measures one GPU only "
               "(the step has no multi-GPU path; the single scene would shard as x-slabs, whose "
@@ -450,6 +610,7 @@ def main():
 
     # graph run: same frames from the same initial state, one graph launch per step
     ms_graph = None
+    windows = None
     if not args.no_graph:
         graph = SingleRun(pbf, p, sc, dev)
         graph.sim.set_shapes(*shapes)
@@ -458,19 +619,36 @@ def main():
             for _ in range(args.warmup):
                 graph.step(stream)
             ms_graph = timed(graph, args.steps, stream)
-        print(f"graph: {ms_graph:.4f} ms/step", file=sys.stderr, flush=True)
+            # later windows of the same run: the rate depends on how far the
+            # dam has spread (dense early frames are the slowest)
+            windows = [{"frames": [args.warmup, args.warmup + args.steps - 1],
+                        "ms_per_step": ms_graph,
+                        "rate": n * p.niter / (ms_graph * 1e-3)}]
+            for k in range(args.windows):
+                f0 = args.warmup + (k + 1) * args.steps
+                wms = timed(graph, args.steps, stream)
+                windows.append({"frames": [f0, f0 + args.steps - 1], "ms_per_step": wms,
+                                "rate": n * p.niter / (wms * 1e-3)})
+        print(f"graph: {ms_graph:.4f} ms/step; later windows "
+              + ", ".join(f"{w['frames'][0]}-{w['frames'][1]}: {w['ms_per_step']:.3f}"
+                          for w in windows[1:]), file=sys.stderr, flush=True)
         del graph
     ms = ms_graph if ms_graph is not None else ms_eager
     value = n * p.niter / (ms * 1e-3)
 
     phase_ms = {k: round(t / c, 4) for k, (t, c) in phases.items() if c}
-    d_ms = phases["delta"][0] / phases["delta"][1]
-    achieved = DELTA_BYTES_PER_PARTICLE * n / (d_ms * 1e-3) / 1e9
+    l_ms = phases["lambda"][0] / phases["lambda"][1]
+    achieved = LAMBDA_BYTES_PER_PARTICLE * n / (l_ms * 1e-3) / 1e9
     traffic = measured_traffic()
-    roofline = {"bound": "hbm", "kernel": "delta-p pass (k_light + k_chunks + k_combine)",
+    roofline = {"bound": "hbm", "kernel": "lambda pass (k_light + k_chunks + k_combine)",
                 "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                "launch_us": round(d_ms * 1e3, 2), "algorithmic_bytes": DELTA_BYTES_PER_PARTICLE * n}
+                "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT) if traffic else None,
+                "launch_us": round(l_ms * 1e3, 2),
+                "launch_timing": "HIP events on the launch stream around each lambda launch, "
+                                 f"eager run, frames {args.warmup}..{args.warmup + args.steps - 1}",
+                "algorithmic_bytes": LAMBDA_BYTES_PER_PARTICLE * n,
+                "algorithmic_bytes_per_particle": LAMBDA_BYTES_PER_PARTICLE}
 
     # batched: 64 scenes of uneven size in one launch sequence, graph mode
     batched = None
@@ -538,15 +716,10 @@ def main():
 
     adjoint = None
     if not args.no_adjoint:
-        adjoint = adjoint_ratio(pbf, p, dev, stream, store=True)
-        rec = adjoint_ratio(pbf, p, dev, stream, store=False)
-        adjoint["recompute"] = {k: rec[k] for k in ("value", "forward_ms", "backward_ms", "tape")}
-        print(f"adjoint: forward {adjoint['forward_ms']:.2f} ms, backward "
-              f"{adjoint['backward_ms']:.2f} ms (recomputing tape: {rec['forward_ms']:.2f} / "
-              f"{rec['backward_ms']:.2f} ms)", file=sys.stderr, flush=True)
+        adjoint = c4_workload(pbf, dev, stream)
 
     sweep = None
-    if args.sweep:
+    if not args.no_sweep:
         sweep = sweep_workload(pbf, dev, stream)
         st_, rc_ = sweep["stored"], sweep["recompute"]
         print(f"sweep: {sweep['plain_ms_per_frame']:.2f} ms/frame; stored tape backward "
@@ -560,7 +733,14 @@ def main():
         P4 = cur[0].cpu().numpy().reshape(n, 4)
         V4 = cur[1].cpu().numpy().reshape(n, 4)
         I = eager.iid.cpu().numpy().astype(np.uint32)
-        cpu = cpu_baseline(p, sc, P4, V4, I, args.cpu_steps)
+        c1 = scenes.double_dam()
+        p1, v1 = c1.as_layout(4)
+        done = args.warmup + args.steps
+        cases = [("single_dam_1m", p, sc, P4, V4, I,
+                  f"1,048,576-particle single dam from the GPU's state after frame {done - 1}"),
+                 ("double_dam_32k", p, c1, p1, v1, c1.iid,
+                  "32,768-particle double dam from its initial state")]
+        cpu = cpu_baseline(cases, args.cpu_steps)
 
     sb = step_bytes(n, p.niter, 256 * 128 * 128)
     out = {
@@ -595,10 +775,10 @@ def main():
         "phase_ms": phase_ms,
         "batched_rate": batched,
         "backward_forward_ratio": adjoint,
+        "sweep_workload": sweep,
+        "frame_windows": windows,
         "slab_share": slab,
     }
-    if sweep is not None:
-        out["sweep_workload"] = sweep
     print(json.dumps(out), flush=True)
 
 

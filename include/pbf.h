@@ -221,14 +221,31 @@ int pbf_step_batched(pbf_solver* s, float* pos, float* npos, float* vel, float* 
  * Graph mode (no counterpart in the reference).  With enable != 0, pbf_step
  * records its launch sequence into a HIP graph the first time it sees an
  * (arrays, n, stream) combination, and replays that graph on later calls
- * with the same combination: one graph launch per step instead of ~30
- * kernel launches.  Up to 4 graphs are kept (a tic-toc needs 2);
- * pbf_set_params, pbf_set_lim, pbf_set_shapes and pbf_set_graph drop them.
- * Steps on the null stream, or on a stream the caller is capturing, run
- * eagerly.  With timing enabled, graph steps record PBF_PHASE_STEP only.
- * Results are identical to eager steps.
+ * with the same combination: one graph launch per step instead of ~40
+ * kernel launches.  Up to 4 graphs are kept (a tic-toc needs 2).  A graph
+ * stays valid across parameter, box and shape changes: when pbf_set_params
+ * (with new values), pbf_set_lim or pbf_set_shapes changed what the
+ * sequence depends on, the next step with that combination records it
+ * again and updates the instantiated graph in place, so the reference's
+ * calling pattern (loadParams every frame, a sweeping setLim,
+ * fluids/FluidSystem.cpp:99-120) keeps replaying two graphs.
+ * pbf_set_graph drops them.  Steps on the null stream, or on a stream the
+ * caller is capturing, run eagerly.  With timing enabled, graph steps record
+ * PBF_PHASE_STEP only.  Results are identical to eager steps.
  */
 int pbf_set_graph(pbf_solver* s, int enable);
+
+/* Graph bookkeeping of a solver since pbf_create: graphs recorded from
+ * scratch and instantiated, in-place updates after a parameter / box /
+ * shape change, replays, and the graphs currently kept. */
+typedef struct pbf_graph_info {
+  long long recorded;
+  long long instantiated;
+  long long updated;
+  long long replayed;
+  int live;
+} pbf_graph_info;
+int pbf_get_graph_info(const pbf_solver* s, pbf_graph_info* out);
 
 /*
  * Launch shapes of the neighbour passes (no counterpart in the reference):

@@ -90,11 +90,21 @@ struct Timer {
 
 }  // namespace
 
+// Everything besides the arrays that a recorded step's launch sequence
+// depends on: the step constants (parameters and box), the parameter block
+// (iteration count) and the launch shapes, as the bytes of those objects
+// (StepConsts has tail padding, which make_consts zero-fills; a byte array
+// keeps it through copies, so memcmp compares values only).
+struct GraphSig {
+  unsigned char b[sizeof(pbf::StepConsts) + sizeof(pbf_params) + sizeof(pbf::Shapes)];
+};
+
 // A recorded step: the launch sequence of one (arrays, n, stream)
-// combination as an instantiated HIP graph.
+// combination as an instantiated HIP graph, recorded under `sig`.
 struct GraphEntry {
   const void* key[6] = {};
   int n = 0;
+  GraphSig sig;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   unsigned long long used = 0;
@@ -121,6 +131,7 @@ struct pbf_solver {
   bool graph_on = false;
   std::vector<GraphEntry> graphs;
   unsigned long long graph_clock = 0;
+  pbf_graph_info ginfo{};  // counts of graph recordings, updates, replays
 
   void* arena = nullptr;
   uint32_t *key0 = nullptr, *val0 = nullptr, *key1 = nullptr, *val1 = nullptr;
@@ -493,16 +504,72 @@ void enqueue_step(pbf_solver* s, const StepArrays& a, const pbf::StepConsts& c, 
   te(PBF_PHASE_STEP);
 }
 
-// The recorded graph of this (arrays, n, stream) combination, recording it
-// (stream capture of enqueue_step) when there is none; the least recently
-// used of kMaxGraphs entries makes room.
+// Records the step's launch sequence on `st` (stream capture of
+// enqueue_step) into *graph.
+int capture_step(pbf_solver* s, const StepArrays& a, const pbf::StepConsts& c, hipStream_t st,
+                 hipGraph_t* graph) {
+  (void)hipGetLastError();
+  hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return fail_hip(s, e, "hipStreamBeginCapture");
+  enqueue_step(s, a, c, st, false);
+  const hipError_t le = hipGetLastError();
+  *graph = nullptr;
+  e = hipStreamEndCapture(st, graph);
+  if (le != hipSuccess || e != hipSuccess) {
+    if (*graph) (void)hipGraphDestroy(*graph);
+    *graph = nullptr;
+    return fail_hip(s, le != hipSuccess ? le : e, "recording the step graph");
+  }
+  return PBF_OK;
+}
+
+GraphSig make_sig(const pbf_solver* s, const pbf::StepConsts& c) {
+  GraphSig g;
+  std::memcpy(g.b, &c, sizeof(c));
+  std::memcpy(g.b + sizeof(c), &s->prm, sizeof(pbf_params));
+  std::memcpy(g.b + sizeof(c) + sizeof(pbf_params), &s->shapes, sizeof(pbf::Shapes));
+  return g;
+}
+
+// The recorded graph of this (arrays, n, stream) combination.  None: the
+// step is recorded and instantiated (the least recently used of kMaxGraphs
+// entries makes room).  One recorded under other constants, parameters or
+// shapes (a moving box, a changed rest density, ...): the step is recorded
+// again and the instantiated graph updated in place
+// (hipGraphExecUpdate; the launch sequence has the same topology), or
+// re-instantiated if the update is refused (a different sort pass count).
 int find_or_record(pbf_solver* s, const StepArrays& a, const pbf::StepConsts& c, hipStream_t st,
                    GraphEntry** out) {
   const void* key[6] = {a.pos, a.npos, a.vel, a.nvel, a.iid, (const void*)st};
+  const GraphSig sig = make_sig(s, c);
   for (auto& g : s->graphs) {
     if (g.n == a.n && std::memcmp(g.key, key, sizeof(key)) == 0) {
       g.used = ++s->graph_clock;
       *out = &g;
+      if (std::memcmp(&g.sig, &sig, sizeof(sig)) == 0) return PBF_OK;
+      hipGraph_t graph = nullptr;
+      int rc = capture_step(s, a, c, st, &graph);
+      if (rc) return rc;
+      hipGraphNode_t bad = nullptr;
+      hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+      hipError_t e = hipGraphExecUpdate(g.exec, graph, &bad, &res);
+      if (e == hipSuccess && res == hipGraphExecUpdateSuccess) {
+        ++s->ginfo.updated;
+      } else {
+        (void)hipGetLastError();
+        hipGraphExec_t exec = nullptr;
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+          (void)hipGraphDestroy(graph);
+          return fail_hip(s, e, "hipGraphInstantiate");
+        }
+        (void)hipGraphExecDestroy(g.exec);
+        g.exec = exec;
+        ++s->ginfo.instantiated;
+      }
+      (void)hipGraphDestroy(g.graph);
+      g.graph = graph;
+      g.sig = sig;
       return PBF_OK;
     }
   }
@@ -514,19 +581,11 @@ int find_or_record(pbf_solver* s, const StepArrays& a, const pbf::StepConsts& c,
     (void)hipGraphDestroy(s->graphs[lru].graph);
     s->graphs.erase(s->graphs.begin() + (long)lru);
   }
-  (void)hipGetLastError();
-  hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
-  if (e != hipSuccess) return fail_hip(s, e, "hipStreamBeginCapture");
-  enqueue_step(s, a, c, st, false);
-  const hipError_t le = hipGetLastError();
   hipGraph_t graph = nullptr;
-  e = hipStreamEndCapture(st, &graph);
-  if (le != hipSuccess || e != hipSuccess) {
-    if (graph) (void)hipGraphDestroy(graph);
-    return fail_hip(s, le != hipSuccess ? le : e, "recording the step graph");
-  }
+  int rc = capture_step(s, a, c, st, &graph);
+  if (rc) return rc;
   hipGraphExec_t exec = nullptr;
-  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
   if (e != hipSuccess) {
     (void)hipGraphDestroy(graph);
     return fail_hip(s, e, "hipGraphInstantiate");
@@ -534,10 +593,13 @@ int find_or_record(pbf_solver* s, const StepArrays& a, const pbf::StepConsts& c,
   GraphEntry g;
   std::memcpy(g.key, key, sizeof(key));
   g.n = a.n;
+  g.sig = sig;
   g.graph = graph;
   g.exec = exec;
   g.used = ++s->graph_clock;
   s->graphs.push_back(g);
+  ++s->ginfo.recorded;
+  ++s->ginfo.instantiated;
   *out = &s->graphs.back();
   return PBF_OK;
 }
@@ -780,6 +842,9 @@ int pbf_set_params(pbf_solver* s, const pbf_params* params) {
   if (!s) return fail(nullptr, PBF_ERR_INVALID_ARGUMENT, "solver is NULL");
   int rc = validate_params(s, params);
   if (rc) return rc;
+  // the reference reloads its parameters every frame (FluidSystem.cpp:102):
+  // unchanged parameters change nothing
+  if (std::memcmp(params, &s->prm, sizeof(pbf_params)) == 0) return PBF_OK;
   pbf::StepConsts c;
   rc = make_consts(s, *params, s->ulim, s->llim, c);
   if (rc) return rc;
@@ -801,7 +866,6 @@ int pbf_set_params(pbf_solver* s, const pbf_params* params) {
     return fail(s, PBF_ERR_CAPACITY, "h gives %d cells > max_cells %lld", c.cells, s->max_cells);
   }
   s->prm = *params;
-  drop_graphs(s);
   return PBF_OK;
 }
 
@@ -819,9 +883,6 @@ int pbf_set_lim(pbf_solver* s, pbf_vec3 ulim, pbf_vec3 llim) {
   if (rc) return rc;
   if (c.cells > s->max_cells)
     return fail(s, PBF_ERR_CAPACITY, "box needs %d cells > max_cells %lld", c.cells, s->max_cells);
-  if (ulim.x != s->ulim.x || ulim.y != s->ulim.y || ulim.z != s->ulim.z || llim.x != s->llim.x ||
-      llim.y != s->llim.y || llim.z != s->llim.z)
-    drop_graphs(s);
   s->ulim = ulim;
   s->llim = llim;
   return PBF_OK;
@@ -877,6 +938,7 @@ int pbf_step(pbf_solver* s, float* pos, float* npos, float* vel, float* nvel, ui
       e = hipGraphLaunch(ge->exec, st);
       tend(s, PBF_PHASE_STEP, st);
       if (e != hipSuccess) return fail_hip(s, e, "hipGraphLaunch");
+      ++s->ginfo.replayed;
       s->cnt_dirty = false;
       return PBF_OK;
     }
@@ -903,6 +965,13 @@ int pbf_set_graph(pbf_solver* s, int enable) {
   return PBF_OK;
 }
 
+int pbf_get_graph_info(const pbf_solver* s, pbf_graph_info* out) {
+  if (!s || !out) return fail(nullptr, PBF_ERR_INVALID_ARGUMENT, "NULL argument");
+  *out = s->ginfo;
+  out->live = (int)s->graphs.size();
+  return PBF_OK;
+}
+
 int pbf_set_shapes(pbf_solver* s, const pbf_shapes* sh) {
   if (!s) return fail(nullptr, PBF_ERR_INVALID_ARGUMENT, "solver is NULL");
   if (!sh) return fail(s, PBF_ERR_INVALID_ARGUMENT, "shapes is NULL");
@@ -912,10 +981,6 @@ int pbf_set_shapes(pbf_solver* s, const pbf_shapes* sh) {
   if (sh->chunk_blocks < 64 || sh->chunk_blocks > 65536)
     return fail(s, PBF_ERR_INVALID_ARGUMENT, "chunk_blocks must be in [64, 65536], got %d",
                 sh->chunk_blocks);
-  if (sh->light_block != s->shapes.light_block || sh->chunk_blocks != s->shapes.chunk_blocks) {
-    DeviceGuard g(s->device);
-    drop_graphs(s);
-  }
   s->shapes.light_block = sh->light_block;
   s->shapes.chunk_blocks = sh->chunk_blocks;
   return PBF_OK;
@@ -926,10 +991,6 @@ int pbf_set_heavy_threshold(pbf_solver* s, int threshold) {
   if (threshold != 0 && (threshold < 64 || threshold > (1 << 30)))
     return fail(s, PBF_ERR_INVALID_ARGUMENT, "heavy threshold must be 0 or in [64, 2^30], got %d",
                 threshold);
-  if (threshold != s->shapes.heavy) {
-    DeviceGuard g(s->device);
-    drop_graphs(s);
-  }
   s->shapes.heavy = threshold;
   return PBF_OK;
 }
@@ -1429,7 +1490,16 @@ int pbf_tape_backward(pbf_solver* s, pbf_tape* t, const float* gpos, const float
   if (!g.ok) return fail(s, PBF_ERR_HIP, "hipSetDevice(%d) failed", s->device);
   hipStream_t st = (hipStream_t)stream;
   const int n = t->n, L = t->layout, NI = t->niter;
-  const pbf_vec3 ul_saved = s->ulim, ll_saved = s->llim;
+  // every frame below runs under its own box; the caller's box is restored
+  // on every return
+  struct BoxRestore {
+    pbf_solver* s;
+    pbf_vec3 ul, ll;
+    ~BoxRestore() {
+      s->ulim = ul;
+      s->llim = ll;
+    }
+  } box_restore{s, s->ulim, s->llim};
   const size_t vb = (size_t)n * (size_t)L * 4, ib = (size_t)n * 4;
   hipError_t e;
   // seeds: dL/d(final state), caller layout
@@ -1464,11 +1534,7 @@ int pbf_tape_backward(pbf_solver* s, pbf_tape* t, const float* gpos, const float
     s->ulim = t->ul[(size_t)f];
     s->llim = t->ll[(size_t)f];
     pbf::StepConsts c;
-    if ((rc = step_consts(s, c))) {
-      s->ulim = ul_saved;
-      s->llim = ll_saved;
-      return rc;
-    }
+    if ((rc = step_consts(s, c))) return rc;
     const uint32_t* cs = s->cs;
     const float4 *x_final = s->xa, *vr = s->vr;
     const uint32_t *skey = nullptr, *sval = nullptr;
@@ -1535,8 +1601,6 @@ int pbf_tape_backward(pbf_solver* s, pbf_tape* t, const float* gpos, const float
     if ((e = hipGetLastError()) != hipSuccess) return fail_hip(s, e, "kernel launch");
     s->cnt_dirty = false;
   }
-  s->ulim = ul_saved;
-  s->llim = ll_saved;
   pbf::launch_sum_f64(t->r0, n, t->r0_sum + 1, t->r0_sum, st);
   double host = 0.0;
   if ((e = hipMemcpyAsync(&host, t->r0_sum, sizeof(double), hipMemcpyDeviceToHost, st)) != hipSuccess)

@@ -62,6 +62,14 @@ class PbfShapes(ctypes.Structure):
     _fields_ = [("light_block", ctypes.c_int), ("chunk_blocks", ctypes.c_int)]
 
 
+class PbfGraphInfo(ctypes.Structure):
+    """struct pbf_graph_info: graph recordings, in-place updates, replays."""
+
+    _fields_ = [("recorded", ctypes.c_longlong), ("instantiated", ctypes.c_longlong),
+                ("updated", ctypes.c_longlong), ("replayed", ctypes.c_longlong),
+                ("live", ctypes.c_int)]
+
+
 class PBFError(RuntimeError):
     """A non-zero status from libpbf.so, with the library's message."""
 
@@ -98,6 +106,7 @@ _SIGNATURES = [
     ("pbf_get_scenes", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
     ("pbf_step_batched", ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P]),
     ("pbf_set_graph", ctypes.c_int, [_P, ctypes.c_int]),
+    ("pbf_get_graph_info", ctypes.c_int, [_P, ctypes.POINTER(PbfGraphInfo)]),
     ("pbf_set_shapes", ctypes.c_int, [_P, ctypes.POINTER(PbfShapes)]),
     ("pbf_get_shapes", ctypes.c_int, [_P, ctypes.POINTER(PbfShapes)]),
     ("pbf_set_heavy_threshold", ctypes.c_int, [_P, ctypes.c_int]),

@@ -8,6 +8,13 @@ adjoint kernels (pbf_tape_backward): gradients with respect to pos0, vel0
 (in their given order) and rho0, per-frame checkpoints, inner iterations
 recomputed.  The frame map is differentiated with its discrete choices
 (sort order, grid, candidate lists, cutoffs, active clamps) held fixed.
+
+The solver's own parameters are left as they were: the forward run and the
+backward pass each set the rest density to rho0 for their duration and put
+the caller's parameters back afterwards (the library keeps its recorded
+graphs across such changes).  rho0 is read on the host (the step constants
+are formed there), so a device rho0 costs one synchronisation; pass a CPU
+tensor to avoid it.
 """
 from __future__ import annotations
 
@@ -18,17 +25,21 @@ class _Rollout(torch.autograd.Function):
     @staticmethod
     def forward(ctx, pos0, vel0, rho0, sim, tape, iid0):
         n, L = tape.n, sim.layout
-        p = sim.saveParams()
-        sim.loadParams(p.replace(pho0=float(rho0)))
+        saved = sim.saveParams()
+        run = saved.replace(pho0=float(rho0))
         stream = torch.cuda.current_stream()
-        tape.forward(pos0.contiguous(), vel0.contiguous(), iid0, stream)
+        sim.loadParams(run)
+        try:
+            tape.forward(pos0.contiguous(), vel0.contiguous(), iid0, stream)
+        finally:
+            sim.loadParams(saved)
         xf = torch.empty(n * L, device=pos0.device)
         iidf = torch.empty(n, dtype=torch.int32, device=pos0.device)
         tape.read(tape.nframes, pos=xf, iid=iidf, stream=stream)
         x_by_iid = torch.empty(n, 3, device=pos0.device)
         x_by_iid[iidf.long()] = xf.view(n, L)[:, :3]
         ctx.save_for_backward(iidf)
-        ctx.sim, ctx.tape = sim, tape
+        ctx.sim, ctx.tape, ctx.params = sim, tape, run
         ctx.rho_dtype = rho0.dtype
         return x_by_iid
 
@@ -42,7 +53,12 @@ class _Rollout(torch.autograd.Function):
         gpos[:, :3] = g_by_iid.float()[iidf.long()]
         gpos0 = torch.empty(n * L, device=dev)
         gvel0 = torch.empty(n * L, device=dev)
-        grho = tape.backward(gpos.view(-1), None, gpos0, gvel0, torch.cuda.current_stream())
+        saved = sim.saveParams()
+        sim.loadParams(ctx.params)  # the parameters of the forward run
+        try:
+            grho = tape.backward(gpos.view(-1), None, gpos0, gvel0, torch.cuda.current_stream())
+        finally:
+            sim.loadParams(saved)
         return (gpos0.view(n, L), gvel0.view(n, L), torch.tensor(grho, dtype=ctx.rho_dtype,
                                                                   device=dev), None, None, None)
 

@@ -204,6 +204,14 @@ class Simulator:
         h = self._handle()
         _lib.check(self._lib.pbf_set_graph(h, 1 if enable else 0), h)
 
+    def graph_info(self) -> dict:
+        """Graph bookkeeping since creation: recorded, instantiated, updated
+        (in place after a parameter / box / shape change), replayed, live."""
+        gi = _lib.PbfGraphInfo()
+        h = self._handle()
+        _lib.check(self._lib.pbf_get_graph_info(h, ctypes.byref(gi)), h)
+        return {k: int(getattr(gi, k)) for k, _ in gi._fields_}
+
     def set_shapes(self, light_block: int, chunk_blocks: int) -> None:
         """Launch shapes of the neighbour passes (results never depend on them)."""
         sh = PbfShapes(int(light_block), int(chunk_blocks))

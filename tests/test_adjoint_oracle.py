@@ -152,3 +152,31 @@ def test_adjoint_with_a_moving_wall(orc, pbf_mod):
     fd = (loss(sc.vel + 1e-6 * d) - loss(sc.vel - 1e-6 * d)) / 2e-6
     an = float((r["g_vel0"] * d).sum())
     assert abs(fd - an) <= 1e-5 * abs(an), (fd, an)
+
+
+@pytest.mark.slow
+def test_c4_fp64_adjoint_against_central_differences_over_30_frames(orc):
+    """The differentiable workload at its own size (16,384 particles, v0 ~
+    N(0, 0.5^2), 30 frames, reference constants): the oracle's FP64 adjoint
+    w.r.t. v0 (three random unit directions, JVP form) and pho0 agrees with
+    FP64 central differences (step 1e-4) within 1e-3."""
+    p = P.reference_defaults()
+    sc = scenes.differentiable()
+    tgt = scenes.target_cloud(sc)
+    nf = 30
+    o = orc.rollout(p, sc.ulim, sc.llim, sc.pos, sc.vel, sc.iid, nf, tgt)
+
+    def loss(v=sc.vel, rho0=0.0):
+        return orc.rollout(p, sc.ulim, sc.llim, sc.pos, v, sc.iid, nf, tgt, pho0=rho0,
+                           grad=False)["loss"]
+
+    rng = np.random.default_rng(3)
+    for _ in range(3):
+        d = rng.normal(size=sc.vel.shape)
+        d /= np.linalg.norm(d)
+        fd = (loss(sc.vel + 1e-4 * d) - loss(sc.vel - 1e-4 * d)) / 2e-4
+        an = float((o["g_vel0"] * d).sum())
+        assert abs(fd - an) <= 1e-3 * abs(fd), (fd, an)
+    rho0 = float(np.float32(p.pho0))
+    fd = (loss(rho0=rho0 + 1e-4) - loss(rho0=rho0 - 1e-4)) / 2e-4
+    assert abs(fd - o["g_rho0"]) <= 1e-3 * abs(fd), (fd, o["g_rho0"])

@@ -276,3 +276,85 @@ def test_tape_rejects_unknown_flags(pbf_mod, device):
     t = ctypes.c_void_p()
     assert lib.pbf_tape_create_ex(sim._handle(), sc.n, 2, 4, ctypes.byref(t)) == 1
     assert b"flags" in lib.pbf_last_error(sim._handle())
+
+
+# ---- the differentiable workload at its own size --------------------------
+# 16,384 particles (32x16x32 jittered lattice), v0 ~ N(0, 0.5^2), 30 frames,
+# 4 iterations; loss = mean squared distance to a seeded target cloud by
+# identity (scenes.differentiable / scenes.target_cloud).
+
+def _c4():
+    sc = scenes.differentiable()
+    return sc, scenes.target_cloud(sc)
+
+
+def _unit_dirs(shape, k, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(k):
+        d = rng.normal(size=shape)
+        out.append(d / np.linalg.norm(d))
+    return out
+
+
+@pytest.mark.parametrize("store", [False, True])
+def test_c4_adjoint_matches_oracle_over_30_frames(pbf_mod, orc, device, store):
+    """Reference constants, all 30 frames: the GPU's final state equals the
+    oracle's FP32 trajectory bit for bit and its gradients (checkpointed tape
+    with the iterations recomputed, and the stored tape) equal the oracle's
+    adjoint along that trajectory within 1e-4."""
+    p = P.reference_defaults()
+    sc, tgt = _c4()
+    print(f"c4 adjoint, reference constants, 30 frames, store={store}", flush=True)
+    gp, gv, gr, xf = gpu_grad(pbf_mod, sc, p, 30, tgt, store=store)
+    o = orc.rollout(p, sc.ulim, sc.llim, sc.pos, sc.vel, sc.iid, 30, tgt, fp32=True)
+    np.testing.assert_array_equal(xf.astype(np.float32), o["x_final"].astype(np.float32))
+    assert np.all(np.isfinite(gv)) and np.all(np.isfinite(gp)) and np.isfinite(gr)
+    assert rel(gv, o["g_vel0"]) < 1e-4, rel(gv, o["g_vel0"])
+    assert rel(gp, o["g_pos0"]) < 1e-4, rel(gp, o["g_pos0"])
+    assert abs(gr - o["g_rho0"]) < 1e-4 * abs(o["g_rho0"]), (gr, o["g_rho0"])
+
+
+def test_c4_gpu_gradient_against_fp64_central_differences(pbf_mod, orc, device):
+    """Reference constants: the GPU gradient w.r.t. v0 (three random unit
+    directions, JVP form) and pho0 against FP64 central differences (step
+    1e-4) of the FP64 rollout, within 1e-3, over 15 frames.  The FP32 and FP64 trajectories drift apart with the horizon
+    (gradient difference ~7e-6 at 5 frames, 6e-5 at 15, 2e-4 at 20, 5e-3 at
+    30, measured on the oracle), so the full 30 frames are checked against
+    the oracle's adjoint of the same trajectory above and the oracle's FP64
+    adjoint against these differences in tests/test_adjoint_oracle.py."""
+    p = P.reference_defaults()
+    sc, tgt = _c4()
+    nf = 15
+    print(f"c4 gradient vs FP64 central differences, {nf} frames", flush=True)
+    gp, gv, gr, _ = gpu_grad(pbf_mod, sc, p, nf, tgt)
+
+    def loss(v=sc.vel, rho0=0.0):
+        return orc.rollout(p, sc.ulim, sc.llim, sc.pos, v, sc.iid, nf, tgt, pho0=rho0,
+                           grad=False)["loss"]
+
+    for d in _unit_dirs(sc.vel.shape, 3, 3):
+        fd = (loss(sc.vel + 1e-4 * d) - loss(sc.vel - 1e-4 * d)) / 2e-4
+        an = float((gv * d).sum())
+        assert abs(fd - an) <= 1e-3 * abs(fd), (fd, an)
+    rho0 = float(np.float32(p.pho0))
+    fd = (loss(rho0=rho0 + 1e-4) - loss(rho0=rho0 - 1e-4)) / 2e-4
+    assert abs(fd - gr) <= 1e-3 * abs(fd), (fd, gr)
+
+
+@pytest.mark.parametrize("nframes", [1, 2])
+def test_c4_dam_constants_adjoint_matches_oracle(pbf_mod, orc, device, nframes):
+    """Dam constants (pho0 6378, eps 600, s_corr k 0.1): the exact derivative
+    grows by ~1e3-1e4 per frame, so only the first frames stay finite in
+    FP32.  There the GPU adjoint equals the oracle's adjoint along the same
+    FP32 trajectory within 1e-4."""
+    p = P.dam_params()
+    sc, tgt = _c4()
+    print(f"c4 adjoint, dam constants, {nframes} frame(s)", flush=True)
+    gp, gv, gr, xf = gpu_grad(pbf_mod, sc, p, nframes, tgt)
+    o = orc.rollout(p, sc.ulim, sc.llim, sc.pos, sc.vel, sc.iid, nframes, tgt, fp32=True)
+    np.testing.assert_array_equal(xf.astype(np.float32), o["x_final"].astype(np.float32))
+    assert np.all(np.isfinite(gv)) and np.all(np.isfinite(gp))
+    assert rel(gv, o["g_vel0"]) < 1e-4, rel(gv, o["g_vel0"])
+    assert rel(gp, o["g_pos0"]) < 1e-4, rel(gp, o["g_pos0"])
+    assert abs(gr - o["g_rho0"]) < 1e-4 * abs(o["g_rho0"]), (gr, o["g_rho0"])

@@ -5,7 +5,9 @@
   against the oracle's FP32 mirror), over several frames and both layouts,
   and one scene of the batch is checked against the mirror directly;
 * graph mode: recorded-and-replayed steps equal eager steps bit for bit,
-  replay after replay, with eager work and parameter changes in between;
+  replay after replay, with eager work and parameter changes in between,
+  and the reference's per-frame loadParams + sweeping setLim keep replaying
+  two graphs;
 * launch shapes and the autotuner: every shape gives the same bits; the
   tuner leaves its inputs untouched and installs one of the candidates.
 """
@@ -127,7 +129,7 @@ def test_graph_replays_equal_eager(pbf_mod, device):
     graph.sim.set_graph(True)
     stream = torch.cuda.Stream()
     for frame in range(8):
-        if frame == 5:  # a parameter change drops the recorded graphs
+        if frame == 5:  # a parameter change: the recorded graphs are updated
             for r in (eager, graph):
                 r.sim.loadParams(p.replace(c_XSPH=0.3))
         with torch.cuda.stream(stream):
@@ -136,6 +138,58 @@ def test_graph_replays_equal_eager(pbf_mod, device):
         assert_bitexact(ge, ee, f"graph frame {frame}")
         # eager work on the same solver between replays
         graph.sim.density_error(graph.a[0], graph.n, stream)
+
+
+def test_graph_survives_per_frame_params_and_sweeping_box(pbf_mod, device):
+    """The reference's frame loop (fluids/FluidSystem.cpp:99-120): loadParams
+    every frame, a sweeping setLim, tic-toc buffers.  Graph mode keeps two
+    graphs (updated in place when the box moves, a parameter really changes
+    or the launch shapes change) and every frame equals eager stepping bit
+    for bit, with eager work on the same solver between replays."""
+    p = P.dam_params()
+    sc = scenes.double_dam()
+    sw = pbf_mod.Sweep(sc.ulim, sc.llim, (0.5, 0.0, 0.0), (0.0, 0.0, 0.0), 2 * np.pi / 12, 0)
+    eager = Run(pbf_mod, sc, p, 4)
+    graph = Run(pbf_mod, sc, p, 4)
+    graph.sim.set_graph(True)
+    stream = torch.cuda.Stream()
+    frames = 14
+    for frame in range(frames):
+        pf = p if frame < 9 else p.replace(c_XSPH=0.3)
+        ul, ll = sw.box(frame)
+        for r in (eager, graph):
+            r.sim.loadParams(pf)
+            r.sim.setLim(ul, ll)
+        if frame == 11:
+            graph.sim.set_shapes(128, 1024)
+        with torch.cuda.stream(stream):
+            ge = graph.step(stream)
+        ee = eager.step()
+        assert_bitexact(ge, ee, f"graph frame {frame}")
+        graph.sim.density_error(graph.a[0], graph.n, stream)
+    info = graph.sim.graph_info()
+    print("graph info", info)
+    assert info["live"] == 2 and info["recorded"] == 2, info
+    assert info["replayed"] == frames, info
+    # every frame after the first two moved the box: the graph of its buffer
+    # pair was recorded again and updated (or, if refused, re-instantiated)
+    assert info["updated"] + info["instantiated"] - info["recorded"] == frames - 2, info
+
+
+def test_graph_kept_when_params_reloaded_unchanged(pbf_mod, device):
+    """loadParams with the values already set changes nothing: the recorded
+    graphs are replayed as they are."""
+    p = P.dam_params()
+    r = Run(pbf_mod, scenes.double_dam(), p, 4)
+    r.sim.set_graph(True)
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        for _ in range(6):
+            r.sim.loadParams(p)
+            r.sim.setLim(*r.sim.getLim())
+            r.step(stream)
+    info = r.sim.graph_info()
+    assert info == {"recorded": 2, "instantiated": 2, "updated": 0, "replayed": 6, "live": 2}, info
 
 
 def test_graph_steps_record_step_timing(pbf_mod, device):

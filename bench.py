@@ -236,6 +236,16 @@ GRAD_NOTE = ("with the dam constants the exact derivative of the rollout grows b
              "differences (1e-3)")
 
 
+SWEEP_GRAD_NOTE = ("by frame 110 the moving wall has piled thousands of particles into a box "
+                   "corner at distances just above KERNAL_EPS, where the spiky gradient's "
+                   "derivative scales as 1/r: the derivative of the frame map grows by ~1e5 per "
+                   "frame with the reference constants and ~1e7-1e8 with the dam constants, so "
+                   "through all 10 frames it exceeds the FP32 range (non-finite is expected); "
+                   "tests/test_gpu_sweep_adjoint.py checks finite gradients through the last 5 "
+                   "(reference) and 2 (dam) frames, both tapes, and the last frame against the "
+                   "oracle's adjoint at full size")
+
+
 def adjoint_ratio(pbf, p, dev, stream, reps=3, store=True):
     """Backward / forward time of the 30-frame differentiable rollout of the
     16,384-particle scene (4 iterations, constants `p`): forward = the tape's
@@ -470,7 +480,7 @@ def sweep_workload(pbf, dev, stream, frames=120, back=10):
             del tape, xf, iidf, g, gp0, gv0
     out["backward_forward_ratio"] = out["recompute"]["backward_forward_ratio"]
     out["tape"] = "checkpoints, iterations recomputed"
-    out["gradients_note"] = GRAD_NOTE
+    out["gradients_note"] = SWEEP_GRAD_NOTE
     out["workload"] = ("sweep_dam_4m: 256x64x256 jittered lattice in a 160h x 64h x 160h box, "
                        "+x wall 160h + 20h sin(2 pi f/120), dam constants, 6 iterations; "
                        f"frames 0..{plain - 1} plain, {plain}..{frames - 1} on a tape, "

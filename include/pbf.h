@@ -338,6 +338,25 @@ const char* pbf_last_error(const pbf_solver* s);
 /* Static name of a status code. */
 const char* pbf_status_string(int code);
 
+/*
+ * Allocation-free step.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The same frame as
+ * pbf_step (same results, bit for bit) on float4 arrays (PBF_LAYOUT_FLOAT4)
+ * without a solver: parameters and box are passed per call, and the step's
+ * scratch is a fixed device array inside the library (25 MiB per GPU, held
+ * until the process ends), so the call allocates nothing.  Limits:
+ * n <= 32768 particles and a box of at most 65536 cells (else
+ * PBF_ERR_CAPACITY).  All calls on one GPU share that GPU's scratch: calls
+ * whose work overlaps on the device (different streams, or host threads
+ * with work in flight on different streams) race; calls on one stream are
+ * ordered.  The scratch keeps the previous call's contents; each call clears
+ * what must start at zero with kernels (no memset), so it can be captured.
+ * Errors are reported by pbf_last_error(NULL).
+ */
+int pbf_step_noalloc(const pbf_params* params, pbf_vec3 ulim, pbf_vec3 llim, float* pos,
+                     float* npos, float* vel, float* nvel, uint32_t* iid, int n, void* stream);
+
 /* PBF_ABI_VERSION of the built library. */
 int pbf_abi_version(void);
 

@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -604,6 +605,93 @@ int find_or_record(pbf_solver* s, const StepArrays& a, const pbf::StepConsts& c,
   return PBF_OK;
 }
 
+// Offsets of the solver's scratch arrays in one arena for `capacity`
+// particles, `max_cells` cells and `nscenes` scenes (0: a single scene).
+struct ArenaLayout {
+  size_t key0, val0, key1, val1, iid, hist, tot, xp, x0s, xa, vr, rho, recs, items, part, cnt,
+      de, dp, dout, cnt2, cs, ts, geo, sid, mask;
+  size_t icap, total;
+};
+
+ArenaLayout arena_layout(int capacity, long long max_cells, int nscenes) {
+  const size_t N = (size_t)capacity;
+  const size_t tiles = (size_t)pbf::radix_tiles(capacity);
+  const size_t dblk = (size_t)pbf::density_partials(capacity);
+  ArenaLayout L{};
+  // chunk items of crowded targets (48 B each with their partial sums): 8
+  // per particle.  A pile of P particles in one cell needs ~P^2/64 items (the
+  // 4M-particle sweeping-wall run piles 17,000 into a corner cell and needs
+  // ~18M); targets beyond the list are summed by their own work-item, which
+  // is correct but serial
+  L.icap = std::min<size_t>(8 * N + 4096, 0xF0000000u);
+  size_t off = 0;
+  auto carve = [&](size_t bytes) {
+    const size_t o = off;
+    off = align_up(off + bytes);
+    return o;
+  };
+  L.key0 = carve(4 * N);
+  L.val0 = carve(4 * N);
+  L.key1 = carve(4 * N);
+  L.val1 = carve(4 * N);
+  L.iid = carve(4 * N);
+  L.hist = carve(4 * 256 * tiles);
+  L.tot = carve(4 * 256);
+  L.xp = carve(16 * N);
+  L.x0s = carve(16 * N);
+  L.xa = carve(16 * N);
+  L.vr = carve(16 * N);
+  L.rho = carve(4 * N);
+  L.recs = carve(16 * N);
+  L.items = carve(16 * L.icap);
+  L.part = carve(4 * pbf::kPartialWords * L.icap);
+  L.cnt = carve(4 * pbf::kCounterWords);
+  L.de = carve(8 * N);
+  L.dp = carve(16 * dblk);
+  L.dout = carve(16);
+  L.cnt2 = carve(4 * (size_t)max_cells);
+  L.cs = carve(4 * ((size_t)max_cells + 1));
+  L.ts = carve(4 * ((size_t)pbf::scan_tiles(max_cells) + 1));
+  L.geo = carve(sizeof(pbf::SceneGeo) * (size_t)(nscenes > 0 ? nscenes : 1));
+  L.sid = carve(nscenes > 0 ? 2 * N : 2);
+  L.mask = carve(8 * pbf::kMaskBlocks * N);
+  L.total = off;
+  return L;
+}
+
+// Points the solver's scratch arrays into the arena at `b` (laid out by
+// arena_layout for `capacity` particles).
+void bind_arena(pbf_solver* s, char* b, const ArenaLayout& L, int capacity) {
+  s->key0 = (uint32_t*)(b + L.key0);
+  s->val0 = (uint32_t*)(b + L.val0);
+  s->key1 = (uint32_t*)(b + L.key1);
+  s->val1 = (uint32_t*)(b + L.val1);
+  s->iid_s = (uint32_t*)(b + L.iid);
+  s->hist = (uint32_t*)(b + L.hist);
+  s->totals = (uint32_t*)(b + L.tot);
+  s->xp = (float4*)(b + L.xp);
+  s->x0s = (float4*)(b + L.x0s);
+  s->xa = (float4*)(b + L.xa);
+  s->vr = (float4*)(b + L.vr);
+  s->rho = (float*)(b + L.rho);
+  s->counters = (uint32_t*)(b + L.cnt);
+  s->hl.counters = s->counters;
+  s->hl.recs = (uint4*)(b + L.recs);
+  s->hl.items = (uint4*)(b + L.items);
+  s->hl.partials = (float*)(b + L.part);
+  s->hl.item_cap = (uint32_t)L.icap;
+  s->hl.masks = (uint64_t*)(b + L.mask);
+  s->hl.mask_stride = (uint32_t)capacity;
+  s->dens_err = (double*)(b + L.de);
+  s->dens_partial = (double*)(b + L.dp);
+  s->dens_out = (double*)(b + L.dout);
+  s->cnt = (uint32_t*)(b + L.cnt2);
+  s->cs = (uint32_t*)(b + L.cs);
+  s->tile_sum = (uint32_t*)(b + L.ts);
+  s->geo_d = (pbf::SceneGeo*)(b + L.geo);
+  s->sid_d = (uint16_t*)(b + L.sid);
+}
+
 // Allocation of a solver for `capacity` particles whose initial grid has
 // `cells` cells.  The caller fills in the boxes.
 int create_impl(const pbf_params& params, int capacity, long long cells, long long max_cells,
@@ -631,69 +719,14 @@ int create_impl(const pbf_params& params, int capacity, long long cells, long lo
   s->layout = layout;
   s->max_cells = max_cells;
 
-  const size_t N = (size_t)capacity;
-  const size_t tiles = (size_t)pbf::radix_tiles(capacity);
-  const size_t dblk = (size_t)pbf::density_partials(capacity);
-  // chunk items of crowded targets (48 B each with their partial sums): 8
-  // per particle.  A pile of P particles in one cell needs ~P^2/64 items (the
-  // 4M-particle sweeping-wall run piles 17,000 into a corner cell and needs
-  // ~18M); targets beyond the list are summed by their own work-item, which
-  // is correct but serial
-  const size_t ICAP = std::min<size_t>(8 * N + 4096, 0xF0000000u);
-  size_t off = 0;
-  auto carve = [&](size_t bytes) {
-    const size_t o = off;
-    off = align_up(off + bytes);
-    return o;
-  };
-  const size_t o_key0 = carve(4 * N), o_val0 = carve(4 * N), o_key1 = carve(4 * N),
-               o_val1 = carve(4 * N), o_iid = carve(4 * N), o_hist = carve(4 * 256 * tiles),
-               o_tot = carve(4 * 256), o_xp = carve(16 * N), o_x0s = carve(16 * N),
-               o_xa = carve(16 * N), o_vr = carve(16 * N), o_rho = carve(4 * N),
-               o_recs = carve(16 * N),
-               o_items = carve(16 * ICAP), o_part = carve(4 * pbf::kPartialWords * ICAP),
-               o_cnt = carve(4 * pbf::kCounterWords), o_de = carve(8 * N), o_dp = carve(16 * dblk),
-               o_do = carve(16), o_cnt2 = carve(4 * (size_t)max_cells),
-               o_cs = carve(4 * ((size_t)max_cells + 1)),
-               o_ts = carve(4 * ((size_t)pbf::scan_tiles(max_cells) + 1)),
-               o_geo = carve(sizeof(pbf::SceneGeo) * (size_t)(nscenes > 0 ? nscenes : 1)),
-               o_sid = carve(nscenes > 0 ? 2 * N : 2),
-               o_mask = carve(8 * pbf::kMaskBlocks * N);
+  const ArenaLayout lay = arena_layout(capacity, max_cells, nscenes);
   void* arena = nullptr;
-  if ((e = hipMalloc(&arena, off)) != hipSuccess) {
+  if ((e = hipMalloc(&arena, lay.total)) != hipSuccess) {
     delete s;
     return fail_hip(nullptr, e, "hipMalloc of solver scratch");
   }
-  char* b = (char*)arena;
   s->arena = arena;
-  s->key0 = (uint32_t*)(b + o_key0);
-  s->val0 = (uint32_t*)(b + o_val0);
-  s->key1 = (uint32_t*)(b + o_key1);
-  s->val1 = (uint32_t*)(b + o_val1);
-  s->iid_s = (uint32_t*)(b + o_iid);
-  s->hist = (uint32_t*)(b + o_hist);
-  s->totals = (uint32_t*)(b + o_tot);
-  s->xp = (float4*)(b + o_xp);
-  s->x0s = (float4*)(b + o_x0s);
-  s->xa = (float4*)(b + o_xa);
-  s->vr = (float4*)(b + o_vr);
-  s->rho = (float*)(b + o_rho);
-  s->counters = (uint32_t*)(b + o_cnt);
-  s->hl.counters = s->counters;
-  s->hl.recs = (uint4*)(b + o_recs);
-  s->hl.items = (uint4*)(b + o_items);
-  s->hl.partials = (float*)(b + o_part);
-  s->hl.item_cap = (uint32_t)ICAP;
-  s->hl.masks = (uint64_t*)(b + o_mask);
-  s->hl.mask_stride = (uint32_t)N;
-  s->dens_err = (double*)(b + o_de);
-  s->dens_partial = (double*)(b + o_dp);
-  s->dens_out = (double*)(b + o_do);
-  s->cnt = (uint32_t*)(b + o_cnt2);
-  s->cs = (uint32_t*)(b + o_cs);
-  s->tile_sum = (uint32_t*)(b + o_ts);
-  s->geo_d = (pbf::SceneGeo*)(b + o_geo);
-  s->sid_d = (uint16_t*)(b + o_sid);
+  bind_arena(s, (char*)arena, lay, capacity);
   // the per-cell counts start (and stay, between steps) all zero; the heavy
   // threshold word starts at 0 (kLightMax)
   pbf::launch_fill_u32(s->cnt, max_cells, 0u, nullptr);
@@ -704,6 +737,49 @@ int create_impl(const pbf_params& params, int capacity, long long cells, long lo
     return fail_hip(nullptr, e, "clearing the cell counts");
   }
   *out = s;
+  return PBF_OK;
+}
+
+// ---- the allocation-free step ----------------------------------------------
+// Scratch of pbf_step_noalloc: a file-scope device array, one copy per GPU
+// (each device holds its own until the process ends).  Host code never uses
+// its name as an address: the device address comes from hipGetSymbolAddress,
+// taken once per device with that device current.
+constexpr int kNoAllocParticles = 32768;
+constexpr long long kNoAllocCells = 65536;
+constexpr size_t kNoAllocBytes = (size_t)25 << 20;
+constexpr int kNoAllocDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+// One solver record per device whose arrays point into that device's
+// scratch (bound on the first call on the device; no allocation).
+struct NoAllocCtx {
+  std::mutex mu;
+  bool ready = false;
+  pbf_solver s;
+};
+NoAllocCtx g_noalloc[kNoAllocDevices];
+
+int noalloc_bind(NoAllocCtx& ctx, int dev) {
+  if (ctx.ready) return PBF_OK;
+  const ArenaLayout lay = arena_layout(kNoAllocParticles, kNoAllocCells, 0);
+  if (lay.total > kNoAllocBytes)
+    return fail(nullptr, PBF_ERR_CAPACITY, "no-alloc scratch needs %zu bytes > %zu", lay.total,
+                kNoAllocBytes);
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || ptr == nullptr)
+    return fail_hip(nullptr, e != hipSuccess ? e : hipErrorInvalidSymbol,
+                    "hipGetSymbolAddress of the no-alloc scratch");
+  pbf_solver& s = ctx.s;
+  s.device = dev;
+  s.capacity = kNoAllocParticles;
+  s.layout = PBF_LAYOUT_FLOAT4;
+  s.max_cells = kNoAllocCells;
+  s.nscenes = 0;
+  s.arena = nullptr;  // not owned: the scratch array
+  bind_arena(&s, (char*)ptr, lay, kNoAllocParticles);
+  ctx.ready = true;
   return PBF_OK;
 }
 
@@ -1607,6 +1683,51 @@ int pbf_tape_backward(pbf_solver* s, pbf_tape* t, const float* gpos, const float
     return fail_hip(s, e, "hipMemcpyAsync");
   if ((e = hipStreamSynchronize(st)) != hipSuccess) return fail_hip(s, e, "hipStreamSynchronize");
   if (grho0) *grho0 = host;
+  return PBF_OK;
+}
+
+int pbf_step_noalloc(const pbf_params* params, pbf_vec3 ulim, pbf_vec3 llim, float* pos,
+                     float* npos, float* vel, float* nvel, uint32_t* iid, int n, void* stream) {
+  int rc = validate_params(nullptr, params);
+  if (rc) return rc;
+  if (n < 0 || n > kNoAllocParticles)
+    return fail(nullptr, PBF_ERR_CAPACITY, "n must be in [0, %d], got %d", kNoAllocParticles, n);
+  if (n == 0) return PBF_OK;
+  if (!pos || !npos || !vel || !nvel || !iid)
+    return fail(nullptr, PBF_ERR_INVALID_ARGUMENT, "NULL particle array");
+  {
+    const void* p[5] = {pos, npos, vel, nvel, iid};
+    for (int a = 0; a < 5; ++a)
+      for (int b = a + 1; b < 5; ++b)
+        if (p[a] == p[b])
+          return fail(nullptr, PBF_ERR_INVALID_ARGUMENT, "particle arrays must be distinct");
+  }
+  pbf::StepConsts c;
+  if ((rc = make_consts(nullptr, *params, ulim, llim, c))) return rc;
+  if (c.cells > kNoAllocCells)
+    return fail(nullptr, PBF_ERR_CAPACITY, "box needs %d cells > %lld", c.cells, kNoAllocCells);
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail_hip(nullptr, e, "hipGetDevice");
+  if (dev < 0 || dev >= kNoAllocDevices)
+    return fail(nullptr, PBF_ERR_STATE, "device %d beyond the %d the no-alloc step serves", dev,
+                kNoAllocDevices);
+  NoAllocCtx& ctx = g_noalloc[dev];
+  std::lock_guard<std::mutex> lock(ctx.mu);
+  if ((rc = noalloc_bind(ctx, dev))) return rc;
+  pbf_solver& s = ctx.s;
+  s.prm = *params;
+  s.ulim = ulim;
+  s.llim = llim;
+  hipStream_t st = (hipStream_t)stream;
+  // the scratch keeps the previous call's contents: the per-cell counts and
+  // the work-list counters must start at zero (fill kernels, so the call can
+  // be captured into a graph)
+  pbf::launch_fill_u32(s.cnt, kNoAllocCells, 0u, st);
+  pbf::launch_fill_u32(s.counters, pbf::kCounterWords, 0u, st);
+  const StepArrays arr{pos, npos, vel, nvel, iid, n};
+  enqueue_step(&s, arr, c, st, false);
+  if ((e = hipGetLastError()) != hipSuccess) return fail_hip(nullptr, e, "kernel launch");
   return PBF_OK;
 }
 

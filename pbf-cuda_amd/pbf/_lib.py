@@ -125,6 +125,8 @@ _SIGNATURES = [
     ("pbf_tape_read", ctypes.c_int, [_P, ctypes.c_int, _P, _P, _P, _P]),
     ("pbf_tape_backward", ctypes.c_int,
      [_P, _P, _P, _P, _P, _P, ctypes.POINTER(ctypes.c_double), _P]),
+    ("pbf_step_noalloc", ctypes.c_int,
+     [ctypes.POINTER(PbfParams), PbfVec3, PbfVec3, _P, _P, _P, _P, _P, ctypes.c_int, _P]),
     ("pbf_last_error", ctypes.c_char_p, [_P]),
     ("pbf_status_string", ctypes.c_char_p, [ctypes.c_int]),
     ("pbf_abi_version", ctypes.c_int, []),

@@ -159,11 +159,11 @@ struct BwdXsphPass {
   static __device__ __forceinline__ Target target(const Args& a, uint32_t i, const StepConsts& c) {
     return Target{a.x[i], a.vr[i], a.gv[i], cell_of_key(a.key[i], c.g), i};
   }
-  static __device__ __forceinline__ float4 load_s(const Args& a, uint32_t j) { return a.x[j]; }
+  static __device__ __forceinline__ const float4* screen(const Args& a) { return a.x; }
   static __device__ __forceinline__ Nb load_f(const Args& a, uint32_t j) {
     return Nb{a.x[j], a.vr[j], a.gv[j]};
   }
-  static __device__ __forceinline__ bool hit(const Target& t, uint32_t, const float4& xj,
+  static __device__ __forceinline__ bool hit(const Target& t, const float3& xj,
                                              const StepConsts& c) {
     float dx, dy, dz;
     return dist2(t.x, xj, dx, dy, dz) < c.h2;
@@ -257,15 +257,16 @@ struct BwdDeltaPass {
   static __device__ __forceinline__ Target target(const Args& a, uint32_t i, const StepConsts& c) {
     return Target{a.xl[i], a.A[i], cell_of_key(a.key[i], c.g), i};
   }
-  static __device__ __forceinline__ float4 load_s(const Args& a, uint32_t j) { return a.xl[j]; }
+  static __device__ __forceinline__ const float4* screen(const Args& a) { return a.xl; }
   static __device__ __forceinline__ Nb load_f(const Args& a, uint32_t j) {
     return Nb{a.xl[j], a.A[j]};
   }
-  static __device__ __forceinline__ bool hit(const Target& t, uint32_t j, const float4& xj,
+  static __device__ __forceinline__ bool hit(const Target& t, const float3& xj,
                                              const StepConsts& c) {
     float dx, dy, dz;
+    // i itself (r2 = 0) and coincident pairs add nothing
     const float r2 = dist2(t.x, xj, dx, dy, dz);
-    return (j != t.i) & (r2 < c.h2_outer) & (r2 >= c.keps2_lo);
+    return (r2 < c.h2_outer) & (r2 >= c.keps2_lo);
   }
   static __device__ __forceinline__ void add(Acc& s, const Target& t, uint32_t j, const Nb& nb,
                                              const StepConsts& c, const Args& a) {
@@ -350,11 +351,11 @@ struct BwdLambdaPass {
   static __device__ __forceinline__ Target target(const Args& a, uint32_t i, const StepConsts& c) {
     return Target{a.xr[i], a.sd[i], cell_of_key(a.key[i], c.g), i};
   }
-  static __device__ __forceinline__ float4 load_s(const Args& a, uint32_t j) { return a.xr[j]; }
+  static __device__ __forceinline__ const float4* screen(const Args& a) { return a.xr; }
   static __device__ __forceinline__ Nb load_f(const Args& a, uint32_t j) {
     return Nb{a.xr[j], a.sd[j]};
   }
-  static __device__ __forceinline__ bool hit(const Target& t, uint32_t, const float4& xj,
+  static __device__ __forceinline__ bool hit(const Target& t, const float3& xj,
                                              const StepConsts& c) {
     float dx, dy, dz;
     // coincident pairs (and i itself) have d = 0: every term below vanishes

@@ -163,6 +163,34 @@ struct MaskMode<P, std::void_t<decltype(P::kMask)>> {
   static constexpr int value = P::kMask;
 };
 
+// A float4 array as a raw buffer resource, addressed by byte offsets: a
+// 32-bit offset per load instead of a 64-bit address, and loads past the
+// array's n elements return 0 instead of touching other memory.  The loads
+// go through the LLVM buffer-load intrinsic with an explicit 3 x float
+// result (12 bytes, buffer_load_dwordx3).
+typedef int pbf_i32x4 __attribute__((ext_vector_type(4)));
+typedef float pbf_f32x3 __attribute__((ext_vector_type(3)));
+__device__ pbf_f32x3 pbf_raw_buffer_load_f32x3(pbf_i32x4 rsrc, int voffset, int soffset,
+                                               int aux) __asm("llvm.amdgcn.raw.buffer.load.v3f32");
+struct Float4Buf {
+  pbf_i32x4 r;
+};
+__device__ __forceinline__ Float4Buf float4_buf(const float4* p, uint32_t n) {
+  const uint64_t bytes = (uint64_t)n * 16u;
+  const uint64_t a = (uint64_t)p;
+  Float4Buf b;
+  b.r.x = (int)(uint32_t)(a & 0xFFFFFFFFull);
+  b.r.y = (int)(uint32_t)(a >> 32);  // stride 0
+  b.r.z = (int)(bytes > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)bytes);
+  b.r.w = 0x00020000;  // the raw (untyped) buffer format of gfx9
+  return b;
+}
+// x, y, z of the element at byte offset `off` (a 12-byte load).
+__device__ __forceinline__ float3 load_xyz_at(const Float4Buf& b, uint32_t off) {
+  const pbf_f32x3 v = pbf_raw_buffer_load_f32x3(b.r, (int)off, 0, 0);
+  return make_float3(v.x, v.y, v.z);
+}
+
 // A work-item's candidate sequence as a compact column table in LDS: entry
 // k < nk is (first slot of the k-th non-empty column, sequence position of
 // its first candidate); entry nk is (0, m).  `tab` points at entry 0 of this
@@ -263,20 +291,25 @@ struct SeqCursor {
     while (u >= n0) shift();
     return b0 + (u - c0);
   }
-  // Slots of positions p0 .. p0+7, clamped to last_valid: branch-free while
-  // the batch spans at most the current and the next column.
-  __device__ __forceinline__ void slots8(uint32_t p0, uint32_t last_valid, uint32_t j[8]) {
+  // Byte offsets (16-byte elements) of positions p0 .. p0+7: off[t] - 16 t
+  // is the offset of the current column's element for t < n0 - p0 and of
+  // the next column's beyond, so the constant 16 t folds into the load
+  // instruction and each position costs one select.  Positions at or past
+  // the end of the sequence get offsets the caller's valid-bit mask
+  // discards (a buffer load returns 0 beyond its range).  A batch spanning
+  // more than two columns takes the per-position slot walk.
+  __device__ __forceinline__ void offsets8(uint32_t p0, uint32_t last_valid, uint32_t off[8]) {
     const uint32_t last = p0 + 7u < last_valid ? p0 + 7u : last_valid;
     if (last < n1) {
+      const uint32_t s = n0 - p0;  // positions of the current column in the batch
+      const uint32_t oa = (b0 + (p0 - c0)) << 4;
+      const uint32_t ob = (b1 + p0 - n0) << 4;  // wraps for t < s: not selected
 #pragma unroll
-      for (int t = 0; t < 8; ++t) {
-        const uint32_t p = p0 + t < last_valid ? p0 + t : last_valid;
-        j[t] = p < n0 ? b0 + (p - c0) : b1 + (p - n0);
-      }
+      for (int t = 0; t < 8; ++t) off[t] = ((uint32_t)t < s ? oa : ob) + 16u * (uint32_t)t;
       if (last >= n0) shift();
     } else {
 #pragma unroll
-      for (int t = 0; t < 8; ++t) j[t] = slot(p0 + t < last_valid ? p0 + t : last_valid);
+      for (int t = 0; t < 8; ++t) off[t] = slot(p0 + t < last_valid ? p0 + t : last_valid) << 4;
     }
   }
 };
@@ -307,7 +340,8 @@ struct SeqCursor1 {
 
 // Visits sequence positions [u0, u1) of a candidate sequence of m positions
 // in order,
-// calling add() only for candidates passing the cheap test hit().  Blocks of
+// calling add() only for candidates passing the cheap test hit() on their
+// x, y, z (12-byte buffer loads from sb).  Blocks of
 // 64 positions are screened eight loads at a time (across column
 // boundaries, so a sparse neighbourhood costs a few memory round trips, not
 // one or two per column), then the hits are reloaded G at a time (a pass
@@ -321,12 +355,12 @@ struct SeqCursor1 {
 // get_mask(b, mask) may supply block b's hit mask (b = position / 64; u0
 // must be a multiple of 64) instead of screening it; put_mask(b, mask)
 // receives each mask the sweep screened.
-template <int G, class LoadS, class LoadF, class Hit, class Add, class BlockEnd, class GetMask,
+template <int G, class LoadF, class Hit, class Add, class BlockEnd, class GetMask,
           class PutMask>
 __device__ __forceinline__ void sweep_seq(const SeqTable& st, uint32_t m, uint32_t u0,
-                                          uint32_t u1, LoadS&& load_s, LoadF&& load_f, Hit&& hit,
-                                          Add&& add, BlockEnd&& block_end, GetMask&& get_mask,
-                                          PutMask&& put_mask) {
+                                          uint32_t u1, const Float4Buf& sb, LoadF&& load_f,
+                                          Hit&& hit, Add&& add, BlockEnd&& block_end,
+                                          GetMask&& get_mask, PutMask&& put_mask) {
   if (u0 >= u1) return;
   SeqCursor c1;
   c1.init(st, u0, m);
@@ -339,14 +373,14 @@ __device__ __forceinline__ void sweep_seq(const SeqTable& st, uint32_t m, uint32
     if (!get_mask(bb >> 6, mask)) {
 #pragma unroll 1
       for (uint32_t u = bb; u < be; u += 8) {
-        uint32_t j[8];
-        c1.slots8(u, be - 1u, j);
-        decltype(load_s(0u)) d[8];
+        uint32_t off[8];
+        c1.offsets8(u, be - 1u, off);
+        float3 d[8];
 #pragma unroll
-        for (int t = 0; t < 8; ++t) d[t] = load_s(j[t]);
+        for (int t = 0; t < 8; ++t) d[t] = load_xyz_at(sb, off[t]);
         uint32_t bits = 0;
 #pragma unroll
-        for (int t = 0; t < 8; ++t) bits |= (hit(j[t], d[t]) ? 1u : 0u) << t;
+        for (int t = 0; t < 8; ++t) bits |= (hit(d[t]) ? 1u : 0u) << t;
         const uint32_t valid = be - u >= 8u ? 0xFFu : (1u << (be - u)) - 1u;
         mask |= (uint64_t)(bits & valid) << (u - bb);
       }

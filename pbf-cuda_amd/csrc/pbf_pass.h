@@ -5,9 +5,10 @@
 //   Args                 kernel arguments (device arrays)
 //   Target, Acc          a target's own data; its running sum (<= 8 floats)
 //   target(a, i, c)      load target i
-//   load_s(a, j)         candidate data for the screening test
+//   screen(a)            the float4 array whose x, y, z the screening tests
 //   load_f(a, j)         candidate data for the pair term
-//   hit(t, j, d, c)      cheap test: false only if the pair adds exactly 0
+//   hit(t, d, c)         cheap test on the candidate's x, y, z d: false only
+//                        if the pair adds exactly 0 (i itself has d == t.x)
 //   add(acc, t, j, d, c, a) the pair term (the body of the reference's loop)
 //   accumulate(tot, p)   tot += a chunk's partial sum
 //   finish(a, t, s, c, g) store the result of target t
@@ -191,16 +192,16 @@ __global__ __launch_bounds__(kBlock) void k_light(typename P::Args a, int n, Ste
   const uint32_t ti = (uint32_t)(block_base + src);
   const SeqTable sts{&seq_dyn[src], nb};
   t = P::target(a, ti, c);
-  auto ls = [&](uint32_t j) { return P::load_s(a, j); };
+  const Float4Buf sb = float4_buf(P::screen(a), (uint32_t)n);
   auto lf = [&](uint32_t j) { return P::load_f(a, j); };
-  auto hit = [&](uint32_t j, const auto& d) { return P::hit(t, j, d, c); };
+  auto hit = [&](const float3& d) { return P::hit(t, d, c); };
   // the sweep's blocks are the canonical chunks: each is summed from zero
   // and added to the total in order (also for a heavy target whose chunk
   // list was full)
   typename P::Acc s, part;
   const MaskIO<P> mio{hl, ti, ms};
   sweep_seq<P::kHitBatch>(
-      sts, ms, 0u, ms, ls, lf, hit, [&](uint32_t j, const auto& d) { P::add(part, t, j, d, c, a); },
+      sts, ms, 0u, ms, sb, lf, hit, [&](uint32_t j, const auto& d) { P::add(part, t, j, d, c, a); },
       [&] {
         P::accumulate(s, part);
         part = typename P::Acc{};
@@ -214,7 +215,7 @@ __global__ __launch_bounds__(kBlock) void k_light(typename P::Args a, int n, Ste
 // kChunk consecutive candidates of a heavy target.  Grid-stride over the
 // items; the grid size is a launch shape.
 template <class P, class S>
-__global__ __launch_bounds__(kBlock) void k_chunks(typename P::Args a, StepConsts c, S sc,
+__global__ __launch_bounds__(kBlock) void k_chunks(typename P::Args a, int n, StepConsts c, S sc,
                                                    const uint32_t* __restrict__ cs, HeavyLists hl,
                                                    int pass) {
   __shared__ uint2 seq[10][kBlock];
@@ -229,6 +230,7 @@ __global__ __launch_bounds__(kBlock) void k_chunks(typename P::Args a, StepConst
   if (vb < 0) return;
   const uint32_t stride = (uint32_t)active * kBlock;
   const SeqTable st{&seq[0][threadIdx.x], kBlock};
+  const Float4Buf sb = float4_buf(P::screen(a), (uint32_t)n);
   for (uint32_t u = (uint32_t)vb * kBlock + threadIdx.x; u < total; u += stride) {
     const uint4 it = hl.items[u];
     if (it.x == ~0u) continue;
@@ -240,9 +242,8 @@ __global__ __launch_bounds__(kBlock) void k_chunks(typename P::Args a, StepConst
     const uint32_t s1 = s0 + kChunk < m ? s0 + kChunk : m;
     const MaskIO<P> mio{hl, it.x, m};
     sweep_seq<P::kHitBatch>(
-        st, m, s0, s1, [&](uint32_t j) { return P::load_s(a, j); },
-        [&](uint32_t j) { return P::load_f(a, j); },
-        [&](uint32_t j, const auto& d) { return P::hit(t, j, d, c); },
+        st, m, s0, s1, sb, [&](uint32_t j) { return P::load_f(a, j); },
+        [&](const float3& d) { return P::hit(t, d, c); },
         [&](uint32_t j, const auto& d) { P::add(part, t, j, d, c, a); }, [] {},
         [&](uint32_t b, uint64_t& mk) { return mio.get(b, mk); },
         [&](uint32_t b, uint64_t mk) { mio.put(b, mk); });
@@ -318,8 +319,8 @@ void launch_pass_s(const typename P::Args& a, int n, const StepConsts& c, const 
   const int lb = sh.light_block;
   k_light<P, S><<<(n + lb - 1) / lb, lb, 10 * lb * sizeof(uint2), st>>>(a, n, c, sc, cs, hl, pass,
                                                                        (uint32_t)sh.heavy);
-  k_chunks<P, S><<<grid_for_items(hl.item_cap, sh.chunk_blocks), kBlock, 0, st>>>(a, c, sc, cs,
-                                                                                    hl, pass);
+  k_chunks<P, S><<<grid_for_items(hl.item_cap, sh.chunk_blocks), kBlock, 0, st>>>(a, n, c, sc,
+                                                                                    cs, hl, pass);
   k_combine<P, S><<<grid_for_items(n, 512), kBlock, 0, st>>>(a, n, c, sc, hl, pass);
 }
 

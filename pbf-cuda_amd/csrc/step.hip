@@ -53,11 +53,9 @@ struct LambdaPass {
   static __device__ __forceinline__ Target target(const Args& a, uint32_t i, const StepConsts&) {
     return Target{a.x[i], i};
   }
-  static __device__ __forceinline__ float3 load_s(const Args& a, uint32_t j) {
-    return load_xyz(a.x, j);
-  }
+  static __device__ __forceinline__ const float4* screen(const Args& a) { return a.x; }
   static __device__ __forceinline__ float4 load_f(const Args& a, uint32_t j) { return a.x[j]; }
-  static __device__ __forceinline__ bool hit(const Target& t, uint32_t, const float3& xj,
+  static __device__ __forceinline__ bool hit(const Target& t, const float3& xj,
                                              const StepConsts& c) {
     float dx, dy, dz;
     return dist2(t.x, xj, dx, dy, dz) < c.h2_outer;
@@ -134,16 +132,15 @@ struct DeltaPass {
   static __device__ __forceinline__ Target target(const Args& a, uint32_t i, const StepConsts&) {
     return Target{a.xl[i], i};
   }
-  static __device__ __forceinline__ float3 load_s(const Args& a, uint32_t j) {
-    return load_xyz(a.xl, j);
-  }
+  static __device__ __forceinline__ const float4* screen(const Args& a) { return a.xl; }
   static __device__ __forceinline__ float4 load_f(const Args& a, uint32_t j) { return a.xl[j]; }
-  static __device__ __forceinline__ bool hit(const Target& t, uint32_t j, const float3& xj,
+  static __device__ __forceinline__ bool hit(const Target& t, const float3& xj,
                                              const StepConsts& c) {
     float dx, dy, dz;
-    // pairs closer than KERNAL_EPS (coincident piles) add nothing either
+    // pairs closer than KERNAL_EPS (coincident piles, and i itself at r2 = 0)
+    // add nothing
     const float r2 = dist2(t.x, xj, dx, dy, dz);
-    return (j != t.i) & (r2 < c.h2_outer) & (r2 >= c.keps2_lo);
+    return (r2 < c.h2_outer) & (r2 >= c.keps2_lo);
   }
   static __device__ __forceinline__ void add(Acc& s, const Target& t, uint32_t j, const float4& xj,
                                              const StepConsts& c, const Args&) {
@@ -250,13 +247,11 @@ struct XsphPass {
   static __device__ __forceinline__ Target target(const Args& a, uint32_t i, const StepConsts&) {
     return Target{a.xa[i], a.vr[i], i};
   }
-  static __device__ __forceinline__ float3 load_s(const Args& a, uint32_t j) {
-    return load_xyz(a.xa, j);
-  }
+  static __device__ __forceinline__ const float4* screen(const Args& a) { return a.xa; }
   static __device__ __forceinline__ Nb load_f(const Args& a, uint32_t j) {
     return Nb{a.xa[j], a.vr[j]};
   }
-  static __device__ __forceinline__ bool hit(const Target& t, uint32_t, const float3& xj,
+  static __device__ __forceinline__ bool hit(const Target& t, const float3& xj,
                                              const StepConsts& c) {
     float dx, dy, dz;
     return dist2(t.x, xj, dx, dy, dz) < c.h2;
@@ -307,9 +302,9 @@ struct DensityPass {
   static __device__ __forceinline__ Target target(const Args& a, uint32_t i, const StepConsts&) {
     return Target{a.x[i], i};
   }
-  static __device__ __forceinline__ float4 load_s(const Args& a, uint32_t j) { return a.x[j]; }
+  static __device__ __forceinline__ const float4* screen(const Args& a) { return a.x; }
   static __device__ __forceinline__ float4 load_f(const Args& a, uint32_t j) { return a.x[j]; }
-  static __device__ __forceinline__ bool hit(const Target& t, uint32_t, const float4& xj,
+  static __device__ __forceinline__ bool hit(const Target& t, const float3& xj,
                                              const StepConsts& c) {
     float dx, dy, dz;
     return dist2(t.x, xj, dx, dy, dz) < c.h2;

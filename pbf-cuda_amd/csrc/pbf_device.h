@@ -215,10 +215,16 @@ struct SeqTable {
 // together (a column outside the grid reads a valid dummy address and is
 // then treated as empty), so building the table costs one memory round trip,
 // not one per column.
-__device__ __forceinline__ uint32_t build_seq(const SeqTable& st, const Cell& q,
+// The nine candidate columns of query cell q as (first slot, length), length
+// 0 for a column outside the grid or empty; the 18 cell-start loads are
+// issued together.
+struct Cols {
+  uint32_t lo[9], len[9];
+};
+__device__ __forceinline__ uint32_t load_cols(Cols& cl, const Cell& q,
                                               const uint32_t* __restrict__ cs,
                                               const SceneGeo& g) {
-  uint32_t lo[9], hi[9];
+  uint32_t hi[9];
   bool in[9];
   const int z0 = q.z > 0 ? q.z - 1 : 0;
   const int z1 = q.z + 1 < g.dz ? q.z + 1 : g.dz - 1;
@@ -228,23 +234,44 @@ __device__ __forceinline__ uint32_t build_seq(const SeqTable& st, const Cell& q,
     const int Y = q.y + k % 3 - 1;
     in[k] = (X >= 0) & (X < g.dx) & (Y >= 0) & (Y < g.dy);
     const uint32_t col = g.base + (in[k] ? (uint32_t)((X * g.dy + Y) * g.dz) : 0u);
-    lo[k] = cs[col + z0];
+    cl.lo[k] = cs[col + z0];
     hi[k] = cs[col + z1 + 1];
   }
-  // branch-free compaction: every column writes its entry at the next free
-  // index, which only advances past non-empty columns (an empty column's
-  // entry is overwritten by the next one or by the closing entry)
+  uint32_t m = 0;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    cl.len[k] = (in[k] & (hi[k] > cl.lo[k])) ? hi[k] - cl.lo[k] : 0u;
+    m += cl.len[k];
+  }
+  return m;
+}
+
+// Writes the column table of `cl`, column k's first slot mapped by
+// slot_of(k, first slot), and returns the candidate count m.  Branch-free
+// compaction: every column writes its entry at the next free index, which
+// only advances past non-empty columns (an empty column's entry is
+// overwritten by the next one or by the closing entry).
+template <class SlotOf>
+__device__ __forceinline__ uint32_t write_seq(const SeqTable& st, const Cols& cl,
+                                              SlotOf&& slot_of) {
   uint32_t m = 0;
   int nk = 0;
 #pragma unroll
   for (int k = 0; k < 9; ++k) {
-    const uint32_t len = (in[k] & (hi[k] > lo[k])) ? hi[k] - lo[k] : 0u;
-    st.tab[nk * st.stride] = make_uint2(lo[k], m);
-    nk += len ? 1 : 0;
-    m += len;
+    st.tab[nk * st.stride] = make_uint2(slot_of(k, cl.lo[k]), m);
+    nk += cl.len[k] ? 1 : 0;
+    m += cl.len[k];
   }
   st.tab[nk * st.stride] = make_uint2(0u, m);
   return m;
+}
+
+__device__ __forceinline__ uint32_t build_seq(const SeqTable& st, const Cell& q,
+                                              const uint32_t* __restrict__ cs,
+                                              const SceneGeo& g) {
+  Cols cl;
+  load_cols(cl, q, cs, g);
+  return write_seq(st, cl, [](int, uint32_t lo) { return lo; });
 }
 
 // Maps increasing sequence positions to sorted slots (positions must be < m
@@ -253,27 +280,30 @@ __device__ __forceinline__ uint32_t build_seq(const SeqTable& st, const Cell& q,
 // positions [n0, n1)); a column exists after the current one exactly when
 // n0 < m, so the lookahead never reads past the table's closing entry.
 struct SeqCursor {
-  SeqTable st;
-  int k;
+  const uint2* e;  // the current column's entry
+  int stride;      // entries apart in LDS
   uint32_t m, b0, c0, n0, b1, n1;
+  // With no column after the current one, b1 = n0 makes offsets8 map the
+  // positions past the sequence's end to slots p (< m + 8): in range of a
+  // staged workgroup's padded stage as well as of the particle arrays.
   __device__ __forceinline__ void load_next() {
     if (n0 < m) {
-      b1 = st.at(k + 1).x;
-      n1 = st.at(k + 2).y;
+      b1 = e[stride].x;
+      n1 = e[2 * stride].y;
     } else {
-      b1 = 0u;
+      b1 = n0;
       n1 = n0;
     }
   }
   __device__ __forceinline__ void init(const SeqTable& t, uint32_t u, uint32_t total) {
-    st = t;
+    e = t.tab;
+    stride = t.stride;
     m = total;
-    k = 0;
-    uint2 e0 = st.at(0), e1 = st.at(1);
+    uint2 e0 = e[0], e1 = e[stride];
     while (u >= e1.y) {  // only when starting inside the sequence
-      ++k;
+      e += stride;
       e0 = e1;
-      e1 = st.at(k + 1);
+      e1 = e[stride];
     }
     b0 = e0.x;
     c0 = e0.y;
@@ -281,7 +311,7 @@ struct SeqCursor {
     load_next();
   }
   __device__ __forceinline__ void shift() {
-    ++k;
+    e += stride;
     b0 = b1;
     c0 = n0;
     n0 = n1;
@@ -293,11 +323,11 @@ struct SeqCursor {
   }
   // Byte offsets (16-byte elements) of positions p0 .. p0+7: off[t] - 16 t
   // is the offset of the current column's element for t < n0 - p0 and of
-  // the next column's beyond, so the constant 16 t folds into the load
-  // instruction and each position costs one select.  Positions at or past
-  // the end of the sequence get offsets the caller's valid-bit mask
-  // discards (a buffer load returns 0 beyond its range).  A batch spanning
-  // more than two columns takes the per-position slot walk.
+  // the next column's beyond, so each position costs one select.
+  // Positions past the batch's last valid one get offsets the caller's
+  // valid-bit mask discards: at most 7 elements past the end of the last
+  // column or of the sequence's slot range (see load_next).  A batch
+  // spanning more than two columns takes the per-position slot walk.
   __device__ __forceinline__ void offsets8(uint32_t p0, uint32_t last_valid, uint32_t off[8]) {
     const uint32_t last = p0 + 7u < last_valid ? p0 + 7u : last_valid;
     if (last < n1) {
@@ -314,40 +344,44 @@ struct SeqCursor {
   }
 };
 
-// The reload cursor: positions only (no lookahead), fewer registers.
+// The reload cursor: positions only (no lookahead), fewer registers; the
+// slot of position u in the current column is u + delta.
 struct SeqCursor1 {
-  SeqTable st;
-  int k;
-  uint32_t b, cum, next;
+  const uint2* e;
+  int stride;
+  uint32_t delta, next;
   __device__ __forceinline__ void from(const SeqCursor& c) {
-    st = c.st;
-    k = c.k;
-    b = c.b0;
-    cum = c.c0;
+    e = c.e;
+    stride = c.stride;
+    delta = c.b0 - c.c0;
     next = c.n0;
   }
   __device__ __forceinline__ uint32_t slot(uint32_t u) {
     while (u >= next) {
-      ++k;
-      const uint2 e0 = st.at(k);
-      b = e0.x;
-      cum = e0.y;
-      next = st.at(k + 1).y;
+      e += stride;
+      const uint2 e0 = e[0];
+      delta = e0.x - e0.y;
+      next = e[stride].y;
     }
-    return b + (u - cum);
+    return u + delta;
   }
 };
 
 // Visits sequence positions [u0, u1) of a candidate sequence of m positions
 // in order,
 // calling add() only for candidates passing the cheap test hit() on their
-// x, y, z (12-byte buffer loads from sb).  Blocks of
-// 64 positions are screened eight loads at a time (across column
-// boundaries, so a sparse neighbourhood costs a few memory round trips, not
-// one or two per column), then the hits are reloaded G at a time (a pass
-// policy's kHitBatch) and accumulated in ascending order.  Candidates
-// failing hit() contribute
-// exactly nothing in add(), so the sums equal the plain in-order sum.
+// x, y, z (load_s(byte offset of the slot's 16-byte element)).  The
+// positions are taken in groups
+// of NB blocks of 64: each block of the group is screened eight loads at a
+// time (across column boundaries, so a sparse neighbourhood costs a few
+// memory round trips, not one or two per column) into a 64-bit hit mask,
+// then the hits of the whole group are reloaded G at a time (a pass
+// policy's kHitBatch) and accumulated in ascending order, with block_end()
+// called once per block, in order, after its last hit.  Running the hits of
+// several blocks through one loop keeps the lanes of a wavefront busy: they
+// wait on the largest hit count of the group instead of that of every
+// block.  Candidates failing hit() contribute exactly nothing in add(), so
+// the sums equal the plain in-order sum.
 // The screening is branch-free after the loads (all eight tests are
 // evaluated and masked), so the compiler keeps the eight loads in flight
 // together instead of sinking some of them into conditional blocks that
@@ -355,10 +389,10 @@ struct SeqCursor1 {
 // get_mask(b, mask) may supply block b's hit mask (b = position / 64; u0
 // must be a multiple of 64) instead of screening it; put_mask(b, mask)
 // receives each mask the sweep screened.
-template <int G, class LoadF, class Hit, class Add, class BlockEnd, class GetMask,
-          class PutMask>
+template <int G, int NB, class LoadS, class LoadF, class Hit, class Add, class BlockEnd,
+          class GetMask, class PutMask>
 __device__ __forceinline__ void sweep_seq(const SeqTable& st, uint32_t m, uint32_t u0,
-                                          uint32_t u1, const Float4Buf& sb, LoadF&& load_f,
+                                          uint32_t u1, LoadS&& load_s, LoadF&& load_f,
                                           Hit&& hit, Add&& add, BlockEnd&& block_end,
                                           GetMask&& get_mask, PutMask&& put_mask) {
   if (u0 >= u1) return;
@@ -367,47 +401,77 @@ __device__ __forceinline__ void sweep_seq(const SeqTable& st, uint32_t m, uint32
   SeqCursor1 c2;
   c2.from(c1);
 #pragma unroll 1
-  for (uint32_t bb = u0; bb < u1; bb += 64) {
-    const uint32_t be = u1 - bb < 64u ? u1 : bb + 64u;
-    uint64_t mask = 0;
-    if (!get_mask(bb >> 6, mask)) {
+  for (uint32_t gb = u0; gb < u1; gb += 64u * NB) {
+    uint64_t w[NB];
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+      const uint32_t bb = gb + 64u * (uint32_t)q;
+      uint64_t mask = 0;
+      if (bb < u1 && !get_mask(bb >> 6, mask)) {
+        const uint32_t be = u1 - bb < 64u ? u1 : bb + 64u;
 #pragma unroll 1
-      for (uint32_t u = bb; u < be; u += 8) {
-        uint32_t off[8];
-        c1.offsets8(u, be - 1u, off);
-        float3 d[8];
+        for (uint32_t u = bb; u < be; u += 8) {
+          uint32_t off[8];
+          c1.offsets8(u, be - 1u, off);
+          float3 d[8];
 #pragma unroll
-        for (int t = 0; t < 8; ++t) d[t] = load_xyz_at(sb, off[t]);
-        uint32_t bits = 0;
+          for (int t = 0; t < 8; ++t) d[t] = load_s(off[t]);
+          uint32_t bits = 0;
 #pragma unroll
-        for (int t = 0; t < 8; ++t) bits |= (hit(d[t]) ? 1u : 0u) << t;
-        const uint32_t valid = be - u >= 8u ? 0xFFu : (1u << (be - u)) - 1u;
-        mask |= (uint64_t)(bits & valid) << (u - bb);
+          for (int t = 0; t < 8; ++t) bits |= (hit(d[t]) ? 1u : 0u) << t;
+          const uint32_t valid = be - u >= 8u ? 0xFFu : (1u << (be - u)) - 1u;
+          mask |= (uint64_t)(bits & valid) << (u - bb);
+        }
+        put_mask(bb >> 6, mask);
       }
-      put_mask(bb >> 6, mask);
+      w[q] = mask;
     }
-    uint32_t last = 0;
+    const uint32_t left = (u1 - gb + 63u) >> 6;
+    const uint32_t nblk = left < (uint32_t)NB ? left : (uint32_t)NB;
+    // the hits of the group: w[0] holds block q's remaining hits (the words
+    // move down as blocks run out)
+    uint32_t q = 0, done = 0, last = gb;
 #pragma unroll 1
-    while (mask) {
-      uint32_t j[G];
+    while (true) {
+      uint64_t any = w[0];
+#pragma unroll
+      for (int k = 1; k < NB; ++k) any |= w[k];
+      if (!any) break;
+      uint32_t j[G], qt[G];
       bool h[G];
 #pragma unroll
       for (int t = 0; t < G; ++t) {
-        h[t] = mask != 0u;
+        while (w[0] == 0u && q + 1u < nblk) {
+#pragma unroll
+          for (int k = 0; k + 1 < NB; ++k) w[k] = w[k + 1];
+          w[NB - 1] = 0u;
+          ++q;
+        }
+        h[t] = w[0] != 0u;
         // past the last hit, repeat it (the cursor only moves forward)
-        const uint32_t bit = h[t] ? (uint32_t)__builtin_ctzll(mask) : last;
-        last = bit;
-        mask &= mask - 1u;
-        j[t] = c2.slot(bb + bit);
+        const uint32_t pos = h[t] ? gb + 64u * q + (uint32_t)__builtin_ctzll(w[0]) : last;
+        last = pos;
+        w[0] &= w[0] - 1u;
+        j[t] = c2.slot(pos);
+        qt[t] = q;
       }
       decltype(load_f(0u)) d[G];
 #pragma unroll
       for (int t = 0; t < G; ++t) d[t] = load_f(j[t]);
 #pragma unroll
       for (int t = 0; t < G; ++t)
-        if (h[t]) add(j[t], d[t]);
+        if (h[t]) {
+          while (done < qt[t]) {
+            block_end();
+            ++done;
+          }
+          add(j[t], d[t]);
+        }
     }
-    block_end();
+    while (done < nblk) {
+      block_end();
+      ++done;
+    }
   }
 }
 

@@ -30,6 +30,40 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kRankBuckets = 64;  // one wavefront scans them
+// Blocks of 64 candidates whose hits the light kernels evaluate in one loop.
+#ifndef PBF_LIGHT_GROUP
+#define PBF_LIGHT_GROUP 2
+#endif
+constexpr int kLightGroup = PBF_LIGHT_GROUP;
+// A pass policy with kStage = true has its light kernel stage the
+// candidates of each workgroup in LDS (see k_light).  Capacity: kStagePerLane
+// float4 elements per work-item of the workgroup.
+template <class P, class = void>
+struct StageMode {
+  static constexpr bool value = false;
+};
+template <class P>
+struct StageMode<P, std::void_t<decltype(P::kStage)>> {
+  static constexpr bool value = P::kStage;
+};
+// The stage holds x, y and z in three arrays of kStagePerLane * workgroup
+// size + kStagePad floats (a screening batch reads up to 7 elements past the
+// end of a target's last column, masked off).
+#ifndef PBF_STAGE_PER_LANE
+#define PBF_STAGE_PER_LANE 8
+#endif
+constexpr int kStagePerLane = PBF_STAGE_PER_LANE;
+constexpr int kStagePad = 8;
+inline size_t light_lds_bytes(int lb, bool staged) {
+  return (size_t)10 * lb * sizeof(uint2) +
+         (staged ? ((size_t)kStagePerLane * lb + kStagePad) * 12 : 0);
+}
+
+// Minimum wavefronts per SIMD the neighbour kernels are compiled for (caps
+// their registers).
+#ifndef PBF_PASS_WAVES
+#define PBF_PASS_WAVES 1
+#endif
 
 inline int blocks_for(long long n) { return (int)((n + kBlock - 1) / kBlock); }
 
@@ -72,13 +106,21 @@ struct MaskIO {
 // stream).  If the chunk list is full the lane sums its target itself, still
 // chunk by chunk, so the canonical order holds either way.
 template <class P, class S>
-__global__ __launch_bounds__(kBlock) void k_light(typename P::Args a, int n, StepConsts c, S sc,
+__global__ __launch_bounds__(kBlock, PBF_PASS_WAVES) void k_light(typename P::Args a, int n,
+                                                                  StepConsts c, S sc,
                                                   const uint32_t* __restrict__ cs, HeavyLists hl,
                                                   int pass, uint32_t fixed_threshold) {
+  constexpr bool kStg = StageMode<P>::value;
   extern __shared__ uint2 seq_dyn[];
   __shared__ uint32_t key_sh[kBlock];
   __shared__ uint32_t order_sh[kBlock];
   __shared__ uint32_t bucket_sh[kRankBuckets];
+  // staging: per wavefront and column k the smallest first slot and the
+  // largest end slot of its light targets' column k; then per column the
+  // workgroup's range [st_lo, st_lo + size) and its first stage element
+  __shared__ uint32_t wlo_sh[kStg ? kBlock / 64 : 1][9], whi_sh[kStg ? kBlock / 64 : 1][9];
+  __shared__ uint32_t st_lo[9], st_base[10];
+  __shared__ int st_ok;
   uint32_t* rec_cnt = hl.counters + rec_counter(pass & 1);
   uint32_t* item_cnt = hl.counters + item_counter(pass & 1);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -95,10 +137,19 @@ __global__ __launch_bounds__(kBlock) void k_light(typename P::Args a, int n, Ste
   const SeqTable st{&seq_dyn[tid], nb};
   typename P::Target t{};
   uint32_t m = 0;
+  Cols cl;
   if (active) {
     t = P::target(a, (uint32_t)i, c);
     const SceneGeo g = sc.at((uint32_t)i, c);
-    m = build_seq(st, cell_of(t.x.x, t.x.y, t.x.z, g, c.h), cs, g);
+    const Cell q = cell_of(t.x.x, t.x.y, t.x.z, g, c.h);
+    if constexpr (kStg) {
+      m = load_cols(cl, q, cs, g);
+    } else {
+      m = build_seq(st, q, cs, g);
+    }
+  } else if constexpr (kStg) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) cl.lo[k] = cl.len[k] = 0u;
   }
   {
     // this pass's statistics for the next pass's threshold: per level, how
@@ -172,11 +223,78 @@ __global__ __launch_bounds__(kBlock) void k_light(typename P::Args a, int n, Ste
   const uint32_t key = active && (!is_heavy || in_lane) ? m + 1u : 0u;
   key_sh[tid] = key;
   if (tid < kRankBuckets) bucket_sh[tid] = 0u;
+  if constexpr (kStg) {
+    // the workgroup's candidate ranges, column by column, over its light
+    // targets (a heavy target summed in its lane reads global memory)
+    const bool light = active && !is_heavy;
+    const int wave = tid >> 6;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      const bool has = light && cl.len[k] != 0u;
+      uint32_t lo = has ? cl.lo[k] : 0xFFFFFFFFu;
+      uint32_t hi = has ? cl.lo[k] + cl.len[k] : 0u;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        const uint32_t ol = __shfl_xor(lo, off), oh = __shfl_xor(hi, off);
+        lo = ol < lo ? ol : lo;
+        hi = oh > hi ? oh : hi;
+      }
+      if ((tid & 63) == 0) {
+        wlo_sh[wave][k] = lo;
+        whi_sh[wave][k] = hi;
+      }
+    }
+  }
   __syncthreads();
+  if constexpr (kStg) {
+    if (tid == 0) {
+      uint32_t total = 0;
+      for (int k = 0; k < 9; ++k) {
+        uint32_t lo = 0xFFFFFFFFu, hi = 0u;
+        for (int w = 0; w < (nb >> 6); ++w) {
+          lo = wlo_sh[w][k] < lo ? wlo_sh[w][k] : lo;
+          hi = whi_sh[w][k] > hi ? whi_sh[w][k] : hi;
+        }
+        st_lo[k] = lo;
+        st_base[k] = total;
+        total += hi > lo ? hi - lo : 0u;
+      }
+      st_base[9] = total;
+      st_ok = total <= (uint32_t)(kStagePerLane * nb);
+    }
+  }
   const uint32_t bucket =
       key == 0u ? 0u : 1u + min((key - 1u) >> 3, (uint32_t)kRankBuckets - 2u);
   const uint32_t within = atomicAdd(&bucket_sh[bucket], 1u);
   __syncthreads();
+  const uint32_t stage_n = (uint32_t)(kStagePerLane * nb + kStagePad);
+  [[maybe_unused]] float* const stage = reinterpret_cast<float*>(seq_dyn + 10 * nb);
+  if constexpr (kStg) {
+    // column tables: a light target's columns as stage elements when the
+    // workgroup's ranges fit the stage, else (and for a heavy target summed
+    // in its lane) as global slots; then the ranges are copied in,
+    // consecutive work-items taking consecutive elements
+    const bool ok = st_ok != 0;
+    if (active) {
+      const bool staged = ok && !is_heavy;
+      write_seq(st, cl, [&](int k, uint32_t lo) {
+        return staged ? st_base[k] + (lo - st_lo[k]) : lo;
+      });
+    }
+    if (ok) {
+      const uint32_t total = st_base[9];
+      const float4* __restrict__ src_x = P::screen(a);
+      for (uint32_t e = (uint32_t)tid; e < total; e += (uint32_t)nb) {
+        int k = 0;
+#pragma unroll
+        for (int q = 1; q < 9; ++q) k += e >= st_base[q] ? 1 : 0;
+        const float4 v = src_x[st_lo[k] + (e - st_base[k])];
+        stage[e] = v.x;
+        stage[stage_n + e] = v.y;
+        stage[2 * stage_n + e] = v.z;
+      }
+    }
+  }
   if (tid < kRankBuckets) {  // the first wavefront scans the bucket counts
     uint32_t total;
     const uint32_t cnt_b = bucket_sh[tid];
@@ -192,22 +310,42 @@ __global__ __launch_bounds__(kBlock) void k_light(typename P::Args a, int n, Ste
   const uint32_t ti = (uint32_t)(block_base + src);
   const SeqTable sts{&seq_dyn[src], nb};
   t = P::target(a, ti, c);
-  const Float4Buf sb = float4_buf(P::screen(a), (uint32_t)n);
-  auto lf = [&](uint32_t j) { return P::load_f(a, j); };
   auto hit = [&](const float3& d) { return P::hit(t, d, c); };
   // the sweep's blocks are the canonical chunks: each is summed from zero
   // and added to the total in order (also for a heavy target whose chunk
   // list was full)
   typename P::Acc s, part;
   const MaskIO<P> mio{hl, ti, ms};
-  sweep_seq<P::kHitBatch>(
-      sts, ms, 0u, ms, sb, lf, hit, [&](uint32_t j, const auto& d) { P::add(part, t, j, d, c, a); },
-      [&] {
-        P::accumulate(s, part);
-        part = typename P::Acc{};
-      },
-      [&](uint32_t b, uint64_t& mk) { return mio.get(b, mk); },
-      [&](uint32_t b, uint64_t mk) { mio.put(b, mk); });
+  auto add = [&](uint32_t j, const auto& d) { P::add(part, t, j, d, c, a); };
+  auto block_end = [&] {
+    P::accumulate(s, part);
+    part = typename P::Acc{};
+  };
+  auto get_mask = [&](uint32_t b, uint64_t& mk) { return mio.get(b, mk); };
+  auto put_mask = [&](uint32_t b, uint64_t mk) { mio.put(b, mk); };
+  if constexpr (kStg) {
+    if (st_ok != 0 && ms <= thr) {
+      // the candidates are stage elements (slot = element index); add()
+      // sees the element index as j (pass policies with kStage use j only
+      // for the self test, which their distance test already decides)
+      sweep_seq<P::kHitBatch, kLightGroup>(
+          sts, ms, 0u, ms,
+          [&](uint32_t off) {
+            const uint32_t e = off >> 4;
+            return make_float3(stage[e], stage[stage_n + e], stage[2 * stage_n + e]);
+          },
+          [&](uint32_t j) {
+            return make_float4(stage[j], stage[stage_n + j], stage[2 * stage_n + j], 0.0f);
+          },
+          hit, add, block_end, get_mask, put_mask);
+      P::finish(a, t, s, c, sc.at(ti, c));
+      return;
+    }
+  }
+  const Float4Buf sb = float4_buf(P::screen(a), (uint32_t)n);
+  sweep_seq<P::kHitBatch, kLightGroup>(
+      sts, ms, 0u, ms, [&](uint32_t off) { return load_xyz_at(sb, off); },
+      [&](uint32_t j) { return P::load_f(a, j); }, hit, add, block_end, get_mask, put_mask);
   P::finish(a, t, s, c, sc.at(ti, c));
 }
 
@@ -215,7 +353,8 @@ __global__ __launch_bounds__(kBlock) void k_light(typename P::Args a, int n, Ste
 // kChunk consecutive candidates of a heavy target.  Grid-stride over the
 // items; the grid size is a launch shape.
 template <class P, class S>
-__global__ __launch_bounds__(kBlock) void k_chunks(typename P::Args a, int n, StepConsts c, S sc,
+__global__ __launch_bounds__(kBlock, PBF_PASS_WAVES) void k_chunks(typename P::Args a, int n,
+                                                                   StepConsts c, S sc,
                                                    const uint32_t* __restrict__ cs, HeavyLists hl,
                                                    int pass) {
   __shared__ uint2 seq[10][kBlock];
@@ -241,8 +380,9 @@ __global__ __launch_bounds__(kBlock) void k_chunks(typename P::Args a, int n, St
     const uint32_t s0 = it.y * kChunk;
     const uint32_t s1 = s0 + kChunk < m ? s0 + kChunk : m;
     const MaskIO<P> mio{hl, it.x, m};
-    sweep_seq<P::kHitBatch>(
-        st, m, s0, s1, sb, [&](uint32_t j) { return P::load_f(a, j); },
+    sweep_seq<P::kHitBatch, 1>(
+        st, m, s0, s1, [&](uint32_t off) { return load_xyz_at(sb, off); },
+        [&](uint32_t j) { return P::load_f(a, j); },
         [&](const float3& d) { return P::hit(t, d, c); },
         [&](uint32_t j, const auto& d) { P::add(part, t, j, d, c, a); }, [] {},
         [&](uint32_t b, uint64_t& mk) { return mio.get(b, mk); },
@@ -317,8 +457,8 @@ void launch_pass_s(const typename P::Args& a, int n, const StepConsts& c, const 
                    const uint32_t* cs, const HeavyLists& hl, int pass, const Shapes& sh,
                    hipStream_t st) {
   const int lb = sh.light_block;
-  k_light<P, S><<<(n + lb - 1) / lb, lb, 10 * lb * sizeof(uint2), st>>>(a, n, c, sc, cs, hl, pass,
-                                                                       (uint32_t)sh.heavy);
+  k_light<P, S><<<(n + lb - 1) / lb, lb, light_lds_bytes(lb, StageMode<P>::value), st>>>(
+      a, n, c, sc, cs, hl, pass, (uint32_t)sh.heavy);
   k_chunks<P, S><<<grid_for_items(hl.item_cap, sh.chunk_blocks), kBlock, 0, st>>>(a, n, c, sc,
                                                                                     cs, hl, pass);
   k_combine<P, S><<<grid_for_items(n, 512), kBlock, 0, st>>>(a, n, c, sc, hl, pass);

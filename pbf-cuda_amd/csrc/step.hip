@@ -35,6 +35,12 @@ namespace {
 struct LambdaPass {
   static constexpr bool kXcdRanges = true;  // one contiguous range per XCD
   static constexpr int kMask = kMaskStore;  // hit masks for the delta-p pass
+  // light workgroups stage their candidates' positions in LDS; add() uses j
+  // only for the self test, which r >= KERNAL_EPS already excludes
+#ifndef PBF_STAGE
+#define PBF_STAGE 0
+#endif
+  static constexpr bool kStage = PBF_STAGE != 0;
   // hits reloaded per round of the sweep
   static constexpr int kHitBatch = 4;
   struct Args {
@@ -76,11 +82,12 @@ struct LambdaPass {
         s.gx = s.gx + ax;
         s.gy = s.gy + ay;
         s.gz = s.gz + az;
-        if (j != t.i) {
-          s.gl2 = fmaf(ax, ax, s.gl2);
-          s.gl2 = fmaf(ay, ay, s.gl2);
-          s.gl2 = fmaf(az, az, s.gl2);
-        }
+        // j != i (Simulator_kernel.cuh:98): i itself has r = 0 < KERNAL_EPS
+        // and never reaches this branch, so no index test is needed (j may
+        // be a stage element index here)
+        s.gl2 = fmaf(ax, ax, s.gl2);
+        s.gl2 = fmaf(ay, ay, s.gl2);
+        s.gl2 = fmaf(az, az, s.gl2);
       }
     }
   }

@@ -581,6 +581,37 @@ __device__ __forceinline__ float3 load_xyz(const float4* __restrict__ a, uint32_
   return make_float3(p[0], p[1], p[2]);
 }
 
+// Correctly rounded square root and quotient for the operand ranges of the
+// pair terms, where no operand or result is zero, subnormal, infinite or NaN
+// and no exponent is extreme: there the compiler's IEEE sequences
+// (-fhip-fp32-correctly-rounded-divide-sqrt) apply no scaling and no
+// special-case fix-up, and what remains is exactly the sequence below, so
+// the results are the IEEE results bit for bit.  sqrt_pair: the hardware
+// estimate, then the neighbour below or above it when its residual says so
+// (inputs r2 in [KERNAL_EPS^2 * 0.99, h^2 * 1.0001); a smaller input, down
+// to 0, still yields a value below KERNAL_EPS, which is all the callers
+// test).  div_pair: reciprocal estimate, one Newton step, quotient with two
+// residual corrections (numerators from 1e-18 to 1e6 over denominators from
+// KERNAL_EPS to 1e4).
+__device__ __forceinline__ float sqrt_pair(float x) {
+  const float s = __builtin_amdgcn_sqrtf(x);
+  const float dn = __int_as_float(__float_as_int(s) - 1);
+  const float up = __int_as_float(__float_as_int(s) + 1);
+  float r = fmaf(-dn, s, x) <= 0.0f ? dn : s;
+  r = fmaf(-up, s, x) > 0.0f ? up : r;
+  return r;
+}
+__device__ __forceinline__ float div_pair(float a, float b) {
+  float r = __builtin_amdgcn_rcpf(b);
+  const float e = fmaf(-b, r, 1.0f);
+  r = fmaf(e, r, r);
+  float q = a * r;
+  const float rem = fmaf(-b, q, a);
+  q = fmaf(rem, r, q);
+  const float rem2 = fmaf(-b, q, a);
+  return fmaf(rem2, r, q);
+}
+
 // poly6 numerator times its coefficient, for r2 < h2: poly6 * (h2-r2)^3.
 __device__ __forceinline__ float poly6_of(float r2, const StepConsts& c) {
   const float t = c.h2 - r2;

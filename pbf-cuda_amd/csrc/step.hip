@@ -74,10 +74,10 @@ struct LambdaPass {
       if (r2 < c.h2) s.rho = s.rho + poly6_of(r2, c);
       // coincident pairs (piles clamped into a corner) have no gradient term:
       // skip their square root
-      const float r = r2 >= c.keps2_lo ? sqrtf(r2) : 0.0f;
+      const float r = r2 >= c.keps2_lo ? sqrt_pair(r2) : 0.0f;
       if (r >= c.keps && r < c.h) {
         const float u = c.h - r;
-        const float f = ((c.spiky_rho0 * u) * u) / r;
+        const float f = div_pair((c.spiky_rho0 * u) * u, r);
         const float ax = f * dx, ay = f * dy, az = f * dz;
         s.gx = s.gx + ax;
         s.gy = s.gy + ay;
@@ -155,12 +155,12 @@ struct DeltaPass {
     float dx, dy, dz;
     const float r2 = dist2(t.x, xj, dx, dy, dz);
     if (r2 < c.h2_outer) {
-      const float r = sqrtf(r2);
+      const float r = sqrt_pair(r2);
       if (r >= c.keps && r < c.h) {
         const float w = r2 < c.h2 ? poly6_of(r2, c) : 0.0f;
         const float corr = c.coef_corr * pow_corr(w, c);
         const float u = c.h - r;
-        const float gs = ((c.spiky * u) * u) / r;
+        const float gs = div_pair((c.spiky * u) * u, r);
         const float f = ((t.x.w + xj.w) + corr) * gs;
         s.px = fmaf(f, dx, s.px);
         s.py = fmaf(f, dy, s.py);
@@ -270,7 +270,7 @@ struct XsphPass {
     if (r2 < c.h2) {
       const float4 vj = nb.v;
       const float w = poly6_of(r2, c);
-      const float f = (2.0f * w) / (t.v.w + vj.w);
+      const float f = div_pair(2.0f * w, t.v.w + vj.w);
       s.ax = fmaf(vj.x - t.v.x, f, s.ax);
       s.ay = fmaf(vj.y - t.v.y, f, s.ay);
       s.az = fmaf(vj.z - t.v.z, f, s.az);

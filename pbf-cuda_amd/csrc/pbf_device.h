@@ -191,6 +191,9 @@ __device__ __forceinline__ float3 load_xyz_at(const Float4Buf& b, uint32_t off) 
   return make_float3(v.x, v.y, v.z);
 }
 
+// Entries of a column table: nine columns and the closing entry.
+constexpr int kSeqEntries = 10;
+
 // A work-item's candidate sequence as a compact column table in LDS: entry
 // k < nk is (first slot of the k-th non-empty column, sequence position of
 // its first candidate); entry nk is (0, m).  `tab` points at entry 0 of this
@@ -299,12 +302,25 @@ struct SeqCursor {
     e = t.tab;
     stride = t.stride;
     m = total;
-    uint2 e0 = e[0], e1 = e[stride];
-    while (u >= e1.y) {  // only when starting inside the sequence
-      e += stride;
-      e0 = e1;
-      e1 = e[stride];
+    if (u != 0u) {
+      // starting inside the sequence (a chunk item): the column holding u
+      // from all entries' first positions, read together.  Entries 0..nk-1
+      // are columns (first position < m), entry nk closes the table with m;
+      // entries after it are stale, so only the run of entries below m
+      // counts.
+      uint32_t y[kSeqEntries];
+#pragma unroll
+      for (int k = 0; k < kSeqEntries; ++k) y[k] = e[k * stride].y;
+      int col = 0;
+      bool live = true;
+#pragma unroll
+      for (int k = 1; k < kSeqEntries; ++k) {
+        live = live && y[k - 1] < total;
+        col += (live && y[k] <= u) ? 1 : 0;
+      }
+      e += col * stride;
     }
+    const uint2 e0 = e[0], e1 = e[stride];
     b0 = e0.x;
     c0 = e0.y;
     n0 = e1.y;

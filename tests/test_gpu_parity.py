@@ -320,8 +320,10 @@ def test_hit_masks_around_their_candidate_limit(pbf_mod, orc, device):
     0.45h, ~60 to ~340 candidates) with two piles of identical positions in
     its corners (1,800 and 2,300 particles) has targets on both sides of that
     limit and with every block count up to it: bit-exact against the FP32
-    mirror for three frames, with the adaptive heavy threshold and with every
-    target above 64 candidates in the chunk kernels."""
+    mirror for three frames, with the adaptive heavy threshold, with every
+    target above 64 candidates in the chunk kernels, and with no target in
+    the chunk kernels (threshold 2^30: the light lanes store and load mask
+    blocks 4..31 themselves)."""
     p = P.dam_params()
     base = scenes.block((24, 24, 24), (2.0, 2.0, 2.0), seed=57, vel_sigma=0.3)
     piles = [np.tile(np.array([[0.03, 0.03, 0.03]], np.float32), (1800, 1)),
@@ -341,7 +343,7 @@ def test_hit_masks_around_their_candidate_limit(pbf_mod, orc, device):
     m = nb.reshape(-1)[m0["keys"].astype(np.int64)]
     assert (m <= 256).sum() > 1000 and ((m > 256) & (m <= 2048)).sum() > 1000, (m.min(), m.max())
     assert (m > 2048).sum() > 1000, m.max()
-    for thr in (0, 64):
+    for thr in (0, 64, 1 << 30):
         run = Run(pbf_mod, sc, p)
         run.sim.set_heavy_threshold(thr)
         P4, V4, I = pos, vel, sc.iid

@@ -295,7 +295,7 @@ int pbf_autotune(pbf_solver* s, const float* pos, const float* vel, const uint32
  * lists, kernel cutoffs, active clamps).
  *
  * pbf_tape_create: tape for n particles and nframes frames under the
- *   solver's current niter (device memory ~ (36 nframes + 52 niter + 200) B
+ *   solver's current niter (device memory ~ (36 nframes + 52 niter + 600) B
  *   per particle).  Allocates and synchronises.
  * pbf_tape_create_ex: the same with flags.  PBF_TAPE_STORE: the forward run
  *   keeps every frame's solver iterations, sort and cell-start table, and

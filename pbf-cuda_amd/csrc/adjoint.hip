@@ -38,10 +38,27 @@ __device__ __forceinline__ Cell cell_of_key(uint32_t key, const SceneGeo& g) {
   return Cell{(int)x, (int)y, (int)(r - y * (uint32_t)g.dz)};
 }
 
+// Query cells packed 10 bits per axis (x << 20 | y << 10 | z), valid when
+// every grid dimension is at most 1024; a pointwise kernel computes them once
+// per particle and position set, so the pair terms compare bit fields
+// instead of dividing three coordinates per pair.
+__device__ __forceinline__ bool packed_query_cells(const StepConsts& c) {
+  return c.g.dx <= 1024 && c.g.dy <= 1024 && c.g.dz <= 1024;
+}
+__device__ __forceinline__ uint32_t pack_query_cell(const float4& x, const StepConsts& c) {
+  const Cell q = cell_of(x.x, x.y, x.z, c.g, c.h);
+  return ((uint32_t)q.x << 20) | ((uint32_t)q.y << 10) | (uint32_t)q.z;
+}
+
 // Whether a particle whose sorted cell is ci is a candidate of the particle
-// at position xj (ci in the 3x3x3 block around xj's query cell).
-__device__ __forceinline__ bool is_candidate_of(const Cell& ci, const float4& xj,
+// at position xj (ci in the 3x3x3 block around xj's query cell); qj is xj's
+// packed query cell (read only when the packing is valid).
+__device__ __forceinline__ bool is_candidate_of(const Cell& ci, const float4& xj, uint32_t qj,
                                                 const StepConsts& c) {
+  if (packed_query_cells(c)) {
+    const int qx = (int)(qj >> 20), qy = (int)((qj >> 10) & 1023u), qz = (int)(qj & 1023u);
+    return abs(ci.x - qx) <= 1 && abs(ci.y - qy) <= 1 && abs(ci.z - qz) <= 1;
+  }
   const Cell q = cell_of(xj.x, xj.y, xj.z, c.g, c.h);
   return abs(ci.x - q.x) <= 1 && abs(ci.y - q.y) <= 1 && abs(ci.z - q.z) <= 1;
 }
@@ -144,6 +161,7 @@ struct BwdXsphPass {
     float4* __restrict__ ex_x;       // scattered (dL/dx, dL/drho), kScatterCopies x ex_n
     float4* __restrict__ ex_v;       // scattered dL/dv, kScatterCopies x ex_n
     uint32_t ex_n;
+    const uint32_t* __restrict__ qc;  // packed query cells of x
   };
   struct Target {
     float4 x, vr, gv;
@@ -155,13 +173,14 @@ struct BwdXsphPass {
   };
   struct Nb {
     float4 x, vr, gv;
+    uint32_t q;
   };
   static __device__ __forceinline__ Target target(const Args& a, uint32_t i, const StepConsts& c) {
     return Target{a.x[i], a.vr[i], a.gv[i], cell_of_key(a.key[i], c.g), i};
   }
   static __device__ __forceinline__ const float4* screen(const Args& a) { return a.x; }
   static __device__ __forceinline__ Nb load_f(const Args& a, uint32_t j) {
-    return Nb{a.x[j], a.vr[j], a.gv[j]};
+    return Nb{a.x[j], a.vr[j], a.gv[j], a.qc[j]};
   }
   static __device__ __forceinline__ bool hit(const Target& t, const float3& xj,
                                              const StepConsts& c) {
@@ -193,7 +212,7 @@ struct BwdXsphPass {
     s.xy += fx * P.y;
     s.xz += fx * P.z;
     s.r -= fr;
-    if (is_candidate_of(t.cell, nb.x, c)) {
+    if (is_candidate_of(t.cell, nb.x, nb.q, c)) {
       // j's term about i, gathered here
       const float3 Bj = xyz(nb.gv);
       const float uj = -dot3(Bj, dv);
@@ -237,7 +256,8 @@ struct BwdDeltaPass {
   static constexpr int kHitBatch = 4;
   struct Args {
     const float4* __restrict__ xl;   // (x, lambda) of the iteration
-    const float4* __restrict__ A;    // adjoint of the correction sum (before / pho0)
+    const float4* __restrict__ A;    // (adjoint of the correction sum (before / pho0),
+                                     //  packed query cell of xl as float bits)
     const uint32_t* __restrict__ key;
     float4* __restrict__ out;        // gathered (dL/dx, dL/dlambda)
     float4* __restrict__ ex;         // scattered (dL/dx, dL/dlambda), kScatterCopies x ex_n
@@ -274,7 +294,7 @@ struct BwdDeltaPass {
     float dx, dy, dz;
     const float r2 = dist2(t.x, nb.x, dx, dy, dz);
     if (!(r2 < c.h2_outer)) return;
-    const float r = sqrtf(r2);
+    const float r = sqrt_pair(r2);
     if (!(r >= c.keps && r < c.h)) return;
     const float3 d = make_float3(dx, dy, dz);
     const float w = r2 < c.h2 ? poly6_of(r2, c) : 0.0f;
@@ -296,7 +316,7 @@ struct BwdDeltaPass {
     s.y += gi.y;
     s.z += gi.z;
     s.l += AG;
-    if (is_candidate_of(t.cell, nb.x, c)) {
+    if (is_candidate_of(t.cell, nb.x, __float_as_uint(nb.A.w), c)) {
       // j's term about i: A_j . (q G(x_j - x_i)) = -q A_j . G(d)
       const float3 Aj = xyz(nb.A);
       const float AGj = dot3(Aj, G);
@@ -336,6 +356,7 @@ struct BwdLambdaPass {
     float4* __restrict__ gx;         // adjoint of x: += gathered
     float4* __restrict__ ex;         // scattered dL/dx, kScatterCopies x ex_n
     uint32_t ex_n;
+    const uint32_t* __restrict__ qc;  // packed query cells of x
   };
   struct Target {
     float4 x, sd;
@@ -347,13 +368,14 @@ struct BwdLambdaPass {
   };
   struct Nb {
     float4 x, sd;
+    uint32_t q;
   };
   static __device__ __forceinline__ Target target(const Args& a, uint32_t i, const StepConsts& c) {
     return Target{a.xr[i], a.sd[i], cell_of_key(a.key[i], c.g), i};
   }
   static __device__ __forceinline__ const float4* screen(const Args& a) { return a.xr; }
   static __device__ __forceinline__ Nb load_f(const Args& a, uint32_t j) {
-    return Nb{a.xr[j], a.sd[j]};
+    return Nb{a.xr[j], a.sd[j], a.qc[j]};
   }
   static __device__ __forceinline__ bool hit(const Target& t, const float3& xj,
                                              const StepConsts& c) {
@@ -368,7 +390,7 @@ struct BwdLambdaPass {
     const float r2 = dist2(t.x, nb.x, dx, dy, dz);
     if (!(r2 < c.h2_outer)) return;
     const float3 d = make_float3(dx, dy, dz);
-    const bool mutual = is_candidate_of(t.cell, nb.x, c);
+    const bool mutual = is_candidate_of(t.cell, nb.x, nb.q, c);
     float3 gi = make_float3(0.0f, 0.0f, 0.0f);  // own terms' d/dx_i
     float3 gr = make_float3(0.0f, 0.0f, 0.0f);  // j's terms' d/dx_i
     if (r2 < c.h2) {
@@ -376,7 +398,7 @@ struct BwdLambdaPass {
       gi = make_float3(t.x.w * P.x, t.x.w * P.y, t.x.w * P.z);
       gr = make_float3(nb.x.w * P.x, nb.x.w * P.y, nb.x.w * P.z);
     }
-    const float r = sqrtf(r2);
+    const float r = sqrt_pair(r2);
     if (r >= c.keps && r < c.h && j != t.i) {
       const SpikyHess H = spiky_hess(c.spiky_rho0, r, recip(r), c);
       const float3 g = make_float3(H.phi * d.x, H.phi * d.y, H.phi * d.z);
@@ -441,6 +463,7 @@ __global__ __launch_bounds__(kBlock) void k_bwd_clamp(int n, StepConsts c,
                                                       const float4* __restrict__ dp,
                                                       float4* __restrict__ gxn,
                                                       float4* __restrict__ A,
+                                                      uint32_t* __restrict__ qc,
                                                       double* __restrict__ r0) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
@@ -456,8 +479,10 @@ __global__ __launch_bounds__(kBlock) void k_bwd_clamp(int n, StepConsts c,
     rr += (double)db * (-(double)uv[q] / (double)c.pho0);
     a[q] = db / c.pho0;
   }
+  const uint32_t q = pack_query_cell(x, c);
   gxn[i] = make_float4(e[0], e[1], e[2], 0.0f);
-  A[i] = make_float4(a[0], a[1], a[2], 0.0f);
+  A[i] = make_float4(a[0], a[1], a[2], __uint_as_float(q));
+  qc[i] = q;
   r0[i] += rr;
 }
 
@@ -500,6 +525,15 @@ __global__ __launch_bounds__(kBlock) void k_bwd_lambda_pre(
   gxn[i] = g;
   xr[i] = make_float4(x.x, x.y, x.z, gr);
   sdb[i] = make_float4(s.x, s.y, s.z, gD);
+}
+
+// Packed query cells of the final positions (for the XSPH pass).
+__global__ __launch_bounds__(kBlock) void k_bwd_qcell(int n, StepConsts c,
+                                                      const float4* __restrict__ x,
+                                                      uint32_t* __restrict__ qc) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  qc[i] = pack_query_cell(x[i], c);
 }
 
 // gx = gxn + scattered extras; clears the extras.
@@ -594,8 +628,9 @@ int fill_blocks(int n) {
 
 void launch_bwd_xsph(const BwdFrame& f, int n, const StepConsts& c, const uint32_t* cs,
                      const HeavyLists& hl, int pass, const Shapes& sh, hipStream_t st) {
+  k_bwd_qcell<<<blocks_for(n), kBlock, 0, st>>>(n, c, f.x_final, f.qc);
   launch_pass<BwdXsphPass>(BwdXsphPass::Args{f.x_final, f.vr, f.gv_out, f.key, f.out_x, f.out_v,
-                                             f.ex_x, f.ex_v, (uint32_t)n},
+                                             f.ex_x, f.ex_v, (uint32_t)n, f.qc},
                            n, c, Scenes{}, cs, hl, pass, sh, st);
   k_bwd_velocity<<<blocks_for(n), kBlock, 0, st>>>(n, c, f.gx_out, f.gv_out, f.out_x, f.out_v,
                                                    f.ex_x, f.ex_v, f.gx, f.gx0, f.grho_x);
@@ -604,13 +639,15 @@ void launch_bwd_xsph(const BwdFrame& f, int n, const StepConsts& c, const uint32
 void launch_bwd_iteration(const BwdFrame& f, const BwdIter& it, int last, int n,
                           const StepConsts& c, const uint32_t* cs, const HeavyLists& hl, int pass,
                           const Shapes& sh, hipStream_t st) {
-  k_bwd_clamp<<<blocks_for(n), kBlock, 0, st>>>(n, c, f.gx, it.xl, it.dp, f.gxn, f.A, f.r0);
+  k_bwd_clamp<<<blocks_for(n), kBlock, 0, st>>>(n, c, f.gx, it.xl, it.dp, f.gxn, f.A, f.qc,
+                                                f.r0);
   launch_pass<BwdDeltaPass>(BwdDeltaPass::Args{it.xl, f.A, f.key, f.out_x, f.ex_x, (uint32_t)n}, n, c,
                             Scenes{}, cs, hl, pass, sh, st);
   k_bwd_lambda_pre<<<blocks_for(n), kBlock, 0, st>>>(n, c, last, f.out_x, f.ex_x, it.xl, it.sd,
                                                      it.rho, f.grho_x, f.gxn, f.xr, f.sdb, f.r0);
-  launch_pass<BwdLambdaPass>(BwdLambdaPass::Args{f.xr, f.sdb, f.key, f.gxn, f.ex_x, (uint32_t)n}, n, c,
-                             Scenes{}, cs, hl, pass + 1, sh, st);
+  launch_pass<BwdLambdaPass>(
+      BwdLambdaPass::Args{f.xr, f.sdb, f.key, f.gxn, f.ex_x, (uint32_t)n, f.qc}, n, c, Scenes{},
+      cs, hl, pass + 1, sh, st);
   k_bwd_fold<<<blocks_for(n), kBlock, 0, st>>>(n, f.gxn, f.ex_x, f.gx);
 }
 

@@ -168,6 +168,7 @@ struct pbf_tape {
          *gxn = nullptr, *gx0 = nullptr, *A = nullptr, *xr = nullptr, *sdb = nullptr,
          *gxo = nullptr, *gvo = nullptr;
   float* grho_x = nullptr;
+  uint32_t* qc = nullptr;           // packed query cells
   float *gp[2] = {nullptr, nullptr}, *gv[2] = {nullptr, nullptr};  // inter-frame adjoints
   double *r0 = nullptr, *r0_sum = nullptr;  // r0_sum: total, then 1024 partial sums
   pbf_params prm{};                 // parameters of the forward run
@@ -1360,7 +1361,7 @@ int pbf_tape_create_ex(pbf_solver* s, int n, int nframes, int flags, pbf_tape** 
   size_t o_f4[12];
   for (int q = 0; q < 12; ++q)  // ex_x and ex_v (2, 3) hold kScatterCopies copies
     o_f4[q] = carve(16 * N * (q == 2 || q == 3 ? (size_t)pbf::kScatterCopies : 1));
-  const size_t o_grx = carve(4 * N), o_gp0 = carve(4 * L * N), o_gp1 = carve(4 * L * N),
+  const size_t o_grx = carve(4 * N), o_qc = carve(4 * N), o_gp0 = carve(4 * L * N), o_gp1 = carve(4 * L * N),
                o_gv0 = carve(4 * L * N), o_gv1 = carve(4 * L * N), o_r0 = carve(8 * N),
                o_r0s = carve(8 * (1 + 1024));
   std::vector<size_t> o_fxl, o_fsd, o_fdp, o_frho, o_fxa, o_fvr, o_fkey, o_fval, o_fcs;
@@ -1406,6 +1407,7 @@ int pbf_tape_create_ex(pbf_solver* s, int n, int nframes, int flags, pbf_tape** 
                      &t->gx0,   &t->A,     &t->xr,   &t->sdb,  &t->gxo, &t->gvo};
   for (int q = 0; q < 12; ++q) *f4[q] = (float4*)(b + o_f4[q]);
   t->grho_x = (float*)(b + o_grx);
+  t->qc = (uint32_t*)(b + o_qc);
   t->gp[0] = (float*)(b + o_gp0);
   t->gp[1] = (float*)(b + o_gp1);
   t->gv[0] = (float*)(b + o_gv0);
@@ -1663,6 +1665,7 @@ int pbf_tape_backward(pbf_solver* s, pbf_tape* t, const float* gpos, const float
     bf.xr = t->xr;
     bf.sdb = t->sdb;
     bf.grho_x = t->grho_x;
+    bf.qc = t->qc;
     bf.r0 = t->r0;
     pbf::launch_bwd_xsph(bf, n, c, cs, s->hl, pass++, s->shapes, st);
     for (int k = NI - 1; k >= 0; --k) {

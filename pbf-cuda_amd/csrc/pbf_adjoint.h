@@ -34,7 +34,8 @@ struct BwdFrame {
   float4* gx;             // dL/d(positions) of the current iteration
   float4* gxn;            // the same for the iteration before
   float4* gx0;            // dL/d(start positions) from v = (x - x0)/dt
-  float4* A;              // dL/d(correction sum)
+  float4* A;              // (dL/d(correction sum), packed query cell as float bits)
+  uint32_t* qc;           // packed query cells of the positions of the current pass
   float4 *xr, *sdb;       // lambda-pass inputs: (x, dL/drho), (S, dL/dD)
   float* grho_x;          // dL/drho of the last lambda pass, from XSPH
   double* r0;             // per-slot dL/dpho0 accumulators

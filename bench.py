@@ -101,15 +101,16 @@ def measured_traffic():
 
 
 def measured_valu():
-    """VALU issue of the lambda light kernel from the committed rocprofv3 SQ
-    summary, against the non-packed peak (one wave64 instruction per CU per
-    clock), or None."""
+    """VALU issue of the lambda pass from the committed rocprofv3 SQ summary,
+    against the non-packed peak (one wave64 instruction per CU per clock), or
+    None."""
     got = _profile_fields(SQ_FILE, ("valu_wave_insts_per_launch", "traced_launch_us",
                                     "lane_utilisation"))
     if got is None:
         return None
     issue_us = got["valu_wave_insts_per_launch"] / VALU_WAVE_INSTS_PER_S * 1e6
-    return {"kernel": "k_light<LambdaPass> (rocprofv3 SQ_INSTS_VALU over the timed frames)",
+    return {"kernel": "lambda pass (rocprofv3 SQ_INSTS_VALU of k_light + k_chunks + k_combine "
+                      "over the timed frames)",
             "wave_insts_per_launch": got["valu_wave_insts_per_launch"],
             "issue_us": round(issue_us, 2), "launch_us": got["traced_launch_us"],
             "frac": issue_us / got["traced_launch_us"],
@@ -557,7 +558,7 @@ def summarize(folder, warmup, steps, windows, n=1048576):
                 row += "; per launch " + ", ".join(
                     f"{c} {sum(d[4].get(c, 0.0) for d in ds) / len(ds):.6g}" for c in cnames)
             lines.append(row)
-        lam = [nm for nm in per if "LambdaPass, pbf::OneScene" in nm]
+        lam = [nm for nm in per if "<LambdaPass, OneScene>" in nm]
         if lam:
             launches = len(per[[x for x in lam if x.startswith("k_light")][0]])
             us = sum(d[3] - d[2] for x in lam for d in per[x]) / launches / 1e3

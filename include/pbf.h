@@ -295,13 +295,24 @@ int pbf_autotune(pbf_solver* s, const float* pos, const float* vel, const uint32
  * lists, kernel cutoffs, active clamps).
  *
  * pbf_tape_create: tape for n particles and nframes frames under the
- *   solver's current niter (device memory ~ (36 nframes + 52 niter + 600) B
+ *   solver's current niter (device memory ~ (36 nframes + 52 niter + 620) B
  *   per particle).  Allocates and synchronises.
  * pbf_tape_create_ex: the same with flags.  PBF_TAPE_STORE: the forward run
  *   keeps every frame's solver iterations, sort and cell-start table, and
  *   the backward pass reads them instead of recomputing each frame (device
  *   memory ~ nframes ((52 niter + 76) B per particle + 4 B per cell of the
  *   solver's largest box) more; the gradients are the same bits).
+ *   PBF_TAPE_GATHER, PBF_TAPE_SCATTER: how the backward pass collects the
+ *   pair terms the stale grid makes one-sided (j a candidate of i but not i
+ *   of j).  GATHER runs each backward neighbour pass over a grid built on
+ *   that pass's own positions, so every particle gathers all of its terms:
+ *   no atomics, and the gradients do not depend on thread timing; it needs
+ *   grids of at most 1024 cells per axis (else pbf_tape_backward fails with
+ *   PBF_ERR_INVALID_ARGUMENT).  SCATTER lets i add such a term to j with a
+ *   float atomic (the summation order, and so the last bits, can vary from
+ *   run to run).  Without either flag, tapes of 2^18 particles or more
+ *   gather when their grids allow it and smaller tapes scatter (there the
+ *   per-pass sort of the gather costs more than the atomics).
  * pbf_tape_forward: runs the frames from (pos, vel, iid) (caller layout,
  *   not modified); the final state is pbf_tape_state(t, nframes).  With a
  *   sweep, frame k runs in the box pbf_sweep_box(sweep, first_frame + k)
@@ -319,6 +330,8 @@ int pbf_autotune(pbf_solver* s, const float* pos, const float* vel, const uint32
  */
 typedef struct pbf_tape pbf_tape;
 #define PBF_TAPE_STORE 1
+#define PBF_TAPE_GATHER 2
+#define PBF_TAPE_SCATTER 4
 int pbf_tape_create(pbf_solver* s, int n, int nframes, pbf_tape** out);
 int pbf_tape_create_ex(pbf_solver* s, int n, int nframes, int flags, pbf_tape** out);
 void pbf_tape_destroy(pbf_tape* t);

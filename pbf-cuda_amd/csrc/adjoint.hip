@@ -21,6 +21,8 @@
 // that a later kernel folds in.  Gathered sums follow the canonical order of
 // the forward passes; only the (rare) scattered terms arrive in a
 // hardware-dependent order.
+#include <utility>
+
 #include "pbf_adjoint.h"
 #include "pbf_pass.h"
 
@@ -45,8 +47,7 @@ __device__ __forceinline__ Cell cell_of_key(uint32_t key, const SceneGeo& g) {
 __device__ __forceinline__ bool packed_query_cells(const StepConsts& c) {
   return c.g.dx <= 1024 && c.g.dy <= 1024 && c.g.dz <= 1024;
 }
-__device__ __forceinline__ uint32_t pack_query_cell(const float4& x, const StepConsts& c) {
-  const Cell q = cell_of(x.x, x.y, x.z, c.g, c.h);
+__device__ __forceinline__ uint32_t pack_cell(const Cell& q) {
   return ((uint32_t)q.x << 20) | ((uint32_t)q.y << 10) | (uint32_t)q.z;
 }
 
@@ -144,28 +145,73 @@ __device__ __forceinline__ float4 take_scattered(float4* __restrict__ ex, int i,
   return make_float4(v[0], v[1], v[2], v[3]);
 }
 
+// Two modes share each backward pass policy:
+//  * scatter (kGather = false): targets in the frame's sorted order sweep the
+//    sorted (stale) grid around their query cell, so every candidate's own
+//    term counts; j's term about i is gathered when i is a candidate of j
+//    and otherwise i's term about j is scattered to j (float atomics);
+//  * gather (kGather = true): targets and candidates in the order of a grid
+//    built on the pass's own positions (sorted by query cell).  A target
+//    sweeps that grid around its query cell, which holds every particle
+//    within h of it, and tests both memberships per pair from the packed
+//    (sorted cell, query cell) of each slot: its own term counts when j is
+//    its candidate, j's term about it when it is j's candidate.  Every term
+//    is gathered by the particle it belongs to, in a fixed order: no atomics.
+//    A pair within h whose query cells are two apart (possible only through
+//    rounding when |x_i - x_j| is within ~1e-6 h of h, where every term and
+//    derivative is ~0) is the only pair this sweep can miss.
+
+// Whether packed cells a and b are at most one cell apart on every axis.
+__device__ __forceinline__ bool within_one(uint32_t a, uint32_t b) {
+  const int dx = (int)(a >> 20) - (int)(b >> 20);
+  const int dy = (int)((a >> 10) & 1023u) - (int)((b >> 10) & 1023u);
+  const int dz = (int)(a & 1023u) - (int)(b & 1023u);
+  return (uint32_t)(dx + 1) <= 2u && (uint32_t)(dy + 1) <= 2u && (uint32_t)(dz + 1) <= 2u;
+}
+
+// Memberships of a pair (target t, candidate j): own = j is a candidate of
+// t (t's own term about j exists), other = t is a candidate of j.
+struct Members {
+  bool own, other;
+};
+template <bool kGather>
+__device__ __forceinline__ Members members(const Cell& ct, uint2 tc, const float4& xj, uint2 jc,
+                                           const StepConsts& c) {
+  if constexpr (kGather) {
+    // packed (sorted cell, query cell) of t and j
+    return Members{within_one(jc.x, tc.y), within_one(tc.x, jc.y)};
+  } else {
+    // the sweep ran over t's candidates; jc.y: j's packed query cell
+    return Members{true, is_candidate_of(ct, xj, jc.y, c)};
+  }
+}
+
 // ---- XSPH -----------------------------------------------------------------
+template <bool kGather>
 struct BwdXsphPass {
-  // workgroups in launch order, round-robin over the XCDs: a crowded cell's
-  // targets then scatter from all eight XCDs, each into its own copy
-  static constexpr bool kXcdRanges = false;
+  // scatter mode: workgroups in launch order, round-robin over the XCDs, so a
+  // crowded cell's targets scatter from all eight XCDs, each into its own
+  // copy; gather mode: one contiguous range per XCD
+  static constexpr bool kXcdRanges = kGather;
   // hits reloaded per round of the sweep
   static constexpr int kHitBatch = 4;
   struct Args {
     const float4* __restrict__ x;    // final positions
     const float4* __restrict__ vr;   // (v before XSPH, rho of the last lambda pass)
     const float4* __restrict__ gv;   // adjoint of v after XSPH
-    const uint32_t* __restrict__ key;  // sorted keys
+    const uint32_t* __restrict__ key;  // scatter: sorted keys
     float4* __restrict__ out_x;      // gathered (dL/dx, dL/drho)
     float4* __restrict__ out_v;      // gathered dL/dv (pair terms)
-    float4* __restrict__ ex_x;       // scattered (dL/dx, dL/drho), kScatterCopies x ex_n
-    float4* __restrict__ ex_v;       // scattered dL/dv, kScatterCopies x ex_n
+    float4* __restrict__ ex_x;       // scatter: (dL/dx, dL/drho), kScatterCopies x ex_n
+    float4* __restrict__ ex_v;       // scatter: dL/dv, kScatterCopies x ex_n
     uint32_t ex_n;
-    const uint32_t* __restrict__ qc;  // packed query cells of x
+    const uint32_t* __restrict__ qc;  // scatter: packed query cells of x
+    const uint2* __restrict__ cq;     // gather: packed (sorted cell, query cell)
   };
   struct Target {
     float4 x, vr, gv;
     Cell cell;
+    uint2 c;
     uint32_t i;
   };
   struct Acc {
@@ -173,14 +219,20 @@ struct BwdXsphPass {
   };
   struct Nb {
     float4 x, vr, gv;
-    uint32_t q;
+    uint2 c;
   };
   static __device__ __forceinline__ Target target(const Args& a, uint32_t i, const StepConsts& c) {
-    return Target{a.x[i], a.vr[i], a.gv[i], cell_of_key(a.key[i], c.g), i};
+    if constexpr (kGather)
+      return Target{a.x[i], a.vr[i], a.gv[i], Cell{}, a.cq[i], i};
+    else
+      return Target{a.x[i], a.vr[i], a.gv[i], cell_of_key(a.key[i], c.g), make_uint2(0u, 0u), i};
   }
   static __device__ __forceinline__ const float4* screen(const Args& a) { return a.x; }
   static __device__ __forceinline__ Nb load_f(const Args& a, uint32_t j) {
-    return Nb{a.x[j], a.vr[j], a.gv[j], a.qc[j]};
+    if constexpr (kGather)
+      return Nb{a.x[j], a.vr[j], a.gv[j], a.cq[j]};
+    else
+      return Nb{a.x[j], a.vr[j], a.gv[j], make_uint2(0u, a.qc[j])};
   }
   static __device__ __forceinline__ bool hit(const Target& t, const float3& xj,
                                              const StepConsts& c) {
@@ -192,6 +244,7 @@ struct BwdXsphPass {
     float dx, dy, dz;
     const float r2 = dist2(t.x, nb.x, dx, dy, dz);
     if (!(r2 < c.h2)) return;
+    const Members mb = members<kGather>(t.cell, t.c, nb.x, nb.c, c);
     const float3 d = make_float3(dx, dy, dz);
     const float W = poly6_of(r2, c);
     const float3 P = poly6_grad(d, r2, c);
@@ -205,15 +258,17 @@ struct BwdXsphPass {
     const float u = dot3(B, dv);
     const float fx = k2 * u;
     const float fr = kr * u;
-    s.vx -= f * B.x;
-    s.vy -= f * B.y;
-    s.vz -= f * B.z;
-    s.xx += fx * P.x;
-    s.xy += fx * P.y;
-    s.xz += fx * P.z;
-    s.r -= fr;
-    if (is_candidate_of(t.cell, nb.x, nb.q, c)) {
-      // j's term about i, gathered here
+    if (mb.own) {
+      s.vx -= f * B.x;
+      s.vy -= f * B.y;
+      s.vz -= f * B.z;
+      s.xx += fx * P.x;
+      s.xy += fx * P.y;
+      s.xz += fx * P.z;
+      s.r -= fr;
+    }
+    if (mb.other) {
+      // j's term about i
       const float3 Bj = xyz(nb.gv);
       const float uj = -dot3(Bj, dv);
       const float gx = k2 * uj;
@@ -224,11 +279,13 @@ struct BwdXsphPass {
       s.xy += gx * P.y;
       s.xz += gx * P.z;
       s.r -= kr * uj;
-    } else if (j != t.i) {
-      // j never visits i: deliver the own term's derivatives about j
-      const size_t e = scatter_copy(a.ex_n) + j;
-      scatter_add(a.ex_v, a.ex_n, e, f * B.x, f * B.y, f * B.z, 0.0f, false);
-      scatter_add(a.ex_x, a.ex_n, e, -fx * P.x, -fx * P.y, -fx * P.z, -fr, true);
+    } else if constexpr (!kGather) {
+      if (j != t.i) {
+        // j never visits i: deliver the own term's derivatives about j
+        const size_t e = scatter_copy(a.ex_n) + j;
+        scatter_add(a.ex_v, a.ex_n, e, f * B.x, f * B.y, f * B.z, 0.0f, false);
+        scatter_add(a.ex_x, a.ex_n, e, -fx * P.x, -fx * P.y, -fx * P.z, -fr, true);
+      }
     }
   }
   static __device__ __forceinline__ void accumulate(Acc& tot, const Acc& p) {
@@ -248,24 +305,27 @@ struct BwdXsphPass {
 };
 
 // ---- delta-p ----------------------------------------------------------------
+template <bool kGather>
 struct BwdDeltaPass {
-  // workgroups in launch order, round-robin over the XCDs: a crowded cell's
-  // targets then scatter from all eight XCDs, each into its own copy
-  static constexpr bool kXcdRanges = false;
+  // scatter mode: round-robin workgroups (see BwdXsphPass); gather mode: one
+  // contiguous range per XCD
+  static constexpr bool kXcdRanges = kGather;
   // hits reloaded per round of the sweep
   static constexpr int kHitBatch = 4;
   struct Args {
     const float4* __restrict__ xl;   // (x, lambda) of the iteration
     const float4* __restrict__ A;    // (adjoint of the correction sum (before / pho0),
-                                     //  packed query cell of xl as float bits)
-    const uint32_t* __restrict__ key;
+                                     //  scatter: packed query cell of xl as float bits)
+    const uint32_t* __restrict__ key;  // scatter: sorted keys
     float4* __restrict__ out;        // gathered (dL/dx, dL/dlambda)
-    float4* __restrict__ ex;         // scattered (dL/dx, dL/dlambda), kScatterCopies x ex_n
+    float4* __restrict__ ex;         // scatter: (dL/dx, dL/dlambda), kScatterCopies x ex_n
     uint32_t ex_n;
+    const uint2* __restrict__ cq;    // gather: packed (sorted cell, query cell)
   };
   struct Target {
     float4 x, A;
     Cell cell;
+    uint2 c;
     uint32_t i;
   };
   struct Acc {
@@ -273,13 +333,22 @@ struct BwdDeltaPass {
   };
   struct Nb {
     float4 x, A;
+    uint2 c;
   };
   static __device__ __forceinline__ Target target(const Args& a, uint32_t i, const StepConsts& c) {
-    return Target{a.xl[i], a.A[i], cell_of_key(a.key[i], c.g), i};
+    if constexpr (kGather)
+      return Target{a.xl[i], a.A[i], Cell{}, a.cq[i], i};
+    else
+      return Target{a.xl[i], a.A[i], cell_of_key(a.key[i], c.g), make_uint2(0u, 0u), i};
   }
   static __device__ __forceinline__ const float4* screen(const Args& a) { return a.xl; }
   static __device__ __forceinline__ Nb load_f(const Args& a, uint32_t j) {
-    return Nb{a.xl[j], a.A[j]};
+    if constexpr (kGather) {
+      return Nb{a.xl[j], a.A[j], a.cq[j]};
+    } else {
+      const float4 A = a.A[j];
+      return Nb{a.xl[j], A, make_uint2(0u, __float_as_uint(A.w))};
+    }
   }
   static __device__ __forceinline__ bool hit(const Target& t, const float3& xj,
                                              const StepConsts& c) {
@@ -296,6 +365,7 @@ struct BwdDeltaPass {
     if (!(r2 < c.h2_outer)) return;
     const float r = sqrt_pair(r2);
     if (!(r >= c.keps && r < c.h)) return;
+    const Members mb = members<kGather>(t.cell, t.c, nb.x, nb.c, c);
     const float3 d = make_float3(dx, dy, dz);
     const float w = r2 < c.h2 ? poly6_of(r2, c) : 0.0f;
     const float q = (t.x.w + nb.x.w) + c.coef_corr * pow_corr(w, c);
@@ -312,11 +382,13 @@ struct BwdDeltaPass {
     const float AG = dot3(Ai, G);
     const float3 HA = hess_apply(H, d, Ai);
     const float3 gi = make_float3(q * HA.x + AG * ds.x, q * HA.y + AG * ds.y, q * HA.z + AG * ds.z);
-    s.x += gi.x;
-    s.y += gi.y;
-    s.z += gi.z;
-    s.l += AG;
-    if (is_candidate_of(t.cell, nb.x, __float_as_uint(nb.A.w), c)) {
+    if (mb.own) {
+      s.x += gi.x;
+      s.y += gi.y;
+      s.z += gi.z;
+      s.l += AG;
+    }
+    if (mb.other) {
       // j's term about i: A_j . (q G(x_j - x_i)) = -q A_j . G(d)
       const float3 Aj = xyz(nb.A);
       const float AGj = dot3(Aj, G);
@@ -325,7 +397,7 @@ struct BwdDeltaPass {
       s.y -= q * HAj.y + AGj * ds.y;
       s.z -= q * HAj.z + AGj * ds.z;
       s.l -= AGj;
-    } else {
+    } else if constexpr (!kGather) {
       const size_t e = scatter_copy(a.ex_n) + j;
       scatter_add(a.ex, a.ex_n, e, -gi.x, -gi.y, -gi.z, AG, true);
     }
@@ -343,24 +415,29 @@ struct BwdDeltaPass {
 };
 
 // ---- lambda (density and gradient norms) ------------------------------------
+template <bool kGather>
 struct BwdLambdaPass {
-  // workgroups in launch order, round-robin over the XCDs: a crowded cell's
-  // targets then scatter from all eight XCDs, each into its own copy
-  static constexpr bool kXcdRanges = false;
+  // scatter mode: round-robin workgroups (see BwdXsphPass); gather mode: one
+  // contiguous range per XCD
+  static constexpr bool kXcdRanges = kGather;
   // hits reloaded per round of the sweep
   static constexpr int kHitBatch = 4;
   struct Args {
     const float4* __restrict__ xr;   // (x, dL/drho)
     const float4* __restrict__ sd;   // (S, dL/dD)
-    const uint32_t* __restrict__ key;
-    float4* __restrict__ gx;         // adjoint of x: += gathered
-    float4* __restrict__ ex;         // scattered dL/dx, kScatterCopies x ex_n
+    const uint32_t* __restrict__ key;  // scatter: sorted keys
+    const float4* gin;  // adjoint of x before the pass (sorted order)
+    float4* gout;       // gin + gathered (sorted order; may alias gin)
+    float4* __restrict__ ex;         // scatter: dL/dx, kScatterCopies x ex_n
     uint32_t ex_n;
-    const uint32_t* __restrict__ qc;  // packed query cells of x
+    const uint32_t* __restrict__ qc;   // scatter: packed query cells of x
+    const uint2* __restrict__ cq;      // gather: packed (sorted cell, query cell)
+    const uint32_t* __restrict__ slot;  // gather: sorted slot of each query slot
   };
   struct Target {
     float4 x, sd;
     Cell cell;
+    uint2 c;
     uint32_t i;
   };
   struct Acc {
@@ -368,14 +445,20 @@ struct BwdLambdaPass {
   };
   struct Nb {
     float4 x, sd;
-    uint32_t q;
+    uint2 c;
   };
   static __device__ __forceinline__ Target target(const Args& a, uint32_t i, const StepConsts& c) {
-    return Target{a.xr[i], a.sd[i], cell_of_key(a.key[i], c.g), i};
+    if constexpr (kGather)
+      return Target{a.xr[i], a.sd[i], Cell{}, a.cq[i], i};
+    else
+      return Target{a.xr[i], a.sd[i], cell_of_key(a.key[i], c.g), make_uint2(0u, 0u), i};
   }
   static __device__ __forceinline__ const float4* screen(const Args& a) { return a.xr; }
   static __device__ __forceinline__ Nb load_f(const Args& a, uint32_t j) {
-    return Nb{a.xr[j], a.sd[j], a.qc[j]};
+    if constexpr (kGather)
+      return Nb{a.xr[j], a.sd[j], a.cq[j]};
+    else
+      return Nb{a.xr[j], a.sd[j], make_uint2(0u, a.qc[j])};
   }
   static __device__ __forceinline__ bool hit(const Target& t, const float3& xj,
                                              const StepConsts& c) {
@@ -390,7 +473,7 @@ struct BwdLambdaPass {
     const float r2 = dist2(t.x, nb.x, dx, dy, dz);
     if (!(r2 < c.h2_outer)) return;
     const float3 d = make_float3(dx, dy, dz);
-    const bool mutual = is_candidate_of(t.cell, nb.x, nb.q, c);
+    const Members mb = members<kGather>(t.cell, t.c, nb.x, nb.c, c);
     float3 gi = make_float3(0.0f, 0.0f, 0.0f);  // own terms' d/dx_i
     float3 gr = make_float3(0.0f, 0.0f, 0.0f);  // j's terms' d/dx_i
     if (r2 < c.h2) {
@@ -411,15 +494,18 @@ struct BwdLambdaPass {
       const float kj = 2.0f * nb.sd.w;
       gr = make_float3(gr.x + kj * Hj.x, gr.y + kj * Hj.y, gr.z + kj * Hj.z);
     }
-    s.x += gi.x;
-    s.y += gi.y;
-    s.z += gi.z;
-    if (mutual) {
+    if (mb.own) {
+      s.x += gi.x;
+      s.y += gi.y;
+      s.z += gi.z;
+    }
+    if (mb.other) {
       s.x += gr.x;
       s.y += gr.y;
       s.z += gr.z;
-    } else if (j != t.i) {
-      scatter_add(a.ex, a.ex_n, scatter_copy(a.ex_n) + j, -gi.x, -gi.y, -gi.z, 0.0f, false);
+    } else if constexpr (!kGather) {
+      if (j != t.i)
+        scatter_add(a.ex, a.ex_n, scatter_copy(a.ex_n) + j, -gi.x, -gi.y, -gi.z, 0.0f, false);
     }
   }
   static __device__ __forceinline__ void accumulate(Acc& tot, const Acc& p) {
@@ -429,24 +515,34 @@ struct BwdLambdaPass {
   }
   static __device__ __forceinline__ void finish(const Args& a, const Target& t, const Acc& s,
                                                 const StepConsts&, const SceneGeo&) {
-    const float4 g = a.gx[t.i];
-    a.gx[t.i] = make_float4(g.x + s.x, g.y + s.y, g.z + s.z, 0.0f);
+    const uint32_t k = kGather ? a.slot[t.i] : t.i;
+    const float4 g = a.gin[k];
+    a.gout[k] = make_float4(g.x + s.x, g.y + s.y, g.z + s.z, 0.0f);
   }
 };
 
 // ---- pointwise kernels -------------------------------------------------------
+// In gather mode the pass outputs are in query order: work-item u handles
+// query slot u, whose sorted slot is perm[u]; sorted-order arrays are read
+// and written at perm[u].  In scatter mode both orders are the sorted one.
 
 // After the XSPH pass: dL/dx_final, dL/dx_start (from v = (x - x0)/dt) and
-// dL/drho of the last lambda pass; clears the extras.
+// dL/drho of the last lambda pass; scatter mode: adds and clears the extras.
+template <bool kGather>
 __global__ __launch_bounds__(kBlock) void k_bwd_velocity(
-    int n, StepConsts c, const float4* __restrict__ gx_out, const float4* __restrict__ gv_out,
-    const float4* __restrict__ out_x, const float4* __restrict__ out_v, float4* __restrict__ ex_x,
-    float4* __restrict__ ex_v, float4* __restrict__ gx, float4* __restrict__ gx0,
-    float* __restrict__ grho) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
-  const float4 ox = out_x[i], ov = out_v[i], ex = take_scattered(ex_x, i, n),
-               ev = take_scattered(ex_v, i, n), go = gx_out[i], gvo = gv_out[i];
+    int n, StepConsts c, const uint32_t* __restrict__ perm, const float4* __restrict__ gx_out,
+    const float4* __restrict__ gv_out, const float4* __restrict__ out_x,
+    const float4* __restrict__ out_v, float4* __restrict__ ex_x, float4* __restrict__ ex_v,
+    float4* __restrict__ gx, float4* __restrict__ gx0, float* __restrict__ grho) {
+  const int u = blockIdx.x * kBlock + threadIdx.x;
+  if (u >= n) return;
+  const int i = kGather ? (int)perm[u] : u;
+  float4 ex = make_float4(0.0f, 0.0f, 0.0f, 0.0f), ev = ex;
+  if constexpr (!kGather) {
+    ex = take_scattered(ex_x, u, n);
+    ev = take_scattered(ex_v, u, n);
+  }
+  const float4 ox = out_x[u], ov = out_v[u], go = gx_out[i], gvo = gv_out[i];
   const float vx = gvo.x + ov.x + ev.x, vy = gvo.y + ov.y + ev.y, vz = gvo.z + ov.z + ev.z;
   gx[i] = make_float4(go.x + ox.x + ex.x + vx * c.inv_dt, go.y + ox.y + ex.y + vy * c.inv_dt,
                       go.z + ox.z + ex.z + vz * c.inv_dt, 0.0f);
@@ -454,17 +550,32 @@ __global__ __launch_bounds__(kBlock) void k_bwd_velocity(
   grho[i] = ox.w + ex.w;
 }
 
+// Query cell of position x: written packed (qc, scatter mode) or as the sort
+// key of the pass's grid with its sorted slot (qkey, qval, gather mode).
+__device__ __forceinline__ uint32_t note_query_cell(int i, const float4& x, const StepConsts& c,
+                                                    uint32_t* __restrict__ qc,
+                                                    uint32_t* __restrict__ qkey,
+                                                    uint32_t* __restrict__ qval) {
+  const Cell q = cell_of(x.x, x.y, x.z, c.g, c.h);
+  const uint32_t packed = pack_cell(q);
+  if (qc) qc[i] = packed;
+  if (qkey) {
+    qkey[i] = cell_id(q, c.g);
+    qval[i] = (uint32_t)i;
+  }
+  return packed;
+}
+
 // Position update of iteration k (fluids/Simulator_kernel.cuh:184-192):
 // x' = clamp_box(x + clamp(dp, +-MAX_DP)).  gx = dL/dx' -> gxn = identity
-// part of dL/dx, A = dL/d(correction sum), r0 += dL/dpho0 through the / pho0.
-__global__ __launch_bounds__(kBlock) void k_bwd_clamp(int n, StepConsts c,
-                                                      const float4* __restrict__ gx,
-                                                      const float4* __restrict__ xl,
-                                                      const float4* __restrict__ dp,
-                                                      float4* __restrict__ gxn,
-                                                      float4* __restrict__ A,
-                                                      uint32_t* __restrict__ qc,
-                                                      double* __restrict__ r0) {
+// part of dL/dx, A = dL/d(correction sum), r0 += dL/dpho0 through the / pho0;
+// A.w holds the packed query cell of x, also noted for the passes (see
+// note_query_cell).
+__global__ __launch_bounds__(kBlock) void k_bwd_clamp(
+    int n, StepConsts c, const float4* __restrict__ gx, const float4* __restrict__ xl,
+    const float4* __restrict__ dp, float4* __restrict__ gxn, float4* __restrict__ A,
+    uint32_t* __restrict__ qc, uint32_t* __restrict__ qkey, uint32_t* __restrict__ qval,
+    double* __restrict__ r0) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   const float4 g = gx[i], x = xl[i], u = dp[i];
@@ -479,11 +590,70 @@ __global__ __launch_bounds__(kBlock) void k_bwd_clamp(int n, StepConsts c,
     rr += (double)db * (-(double)uv[q] / (double)c.pho0);
     a[q] = db / c.pho0;
   }
-  const uint32_t q = pack_query_cell(x, c);
+  const uint32_t q = note_query_cell(i, x, c, qc, qkey, qval);
   gxn[i] = make_float4(e[0], e[1], e[2], 0.0f);
   A[i] = make_float4(a[0], a[1], a[2], __uint_as_float(q));
-  qc[i] = q;
   r0[i] += rr;
+}
+
+// Query cells of the final positions (for the XSPH pass).
+__global__ __launch_bounds__(kBlock) void k_bwd_qcell(int n, StepConsts c,
+                                                      const float4* __restrict__ x,
+                                                      uint32_t* __restrict__ qc,
+                                                      uint32_t* __restrict__ qkey,
+                                                      uint32_t* __restrict__ qval) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  note_query_cell(i, x[i], c, qc, qkey, qval);
+}
+
+// Gather mode, once per frame: packed sorted (stale-grid) cell of each
+// sorted slot.
+__global__ __launch_bounds__(kBlock) void k_bwd_sorted_cells(int n, StepConsts c,
+                                                             const uint32_t* __restrict__ key,
+                                                             uint32_t* __restrict__ pc) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  pc[i] = pack_cell(cell_of_key(key[i], c.g));
+}
+
+// Gather mode: the inputs of an iteration's delta-p pass in query order
+// (skey: sorted query keys, perm: their sorted slots), with the per-cell
+// counts of the query grid (cnt all zero beforehand).
+__global__ __launch_bounds__(kBlock) void k_bwd_gather_iter(
+    int n, const uint32_t* __restrict__ skey, const uint32_t* __restrict__ perm,
+    const float4* __restrict__ xl, const float4* __restrict__ A, const uint32_t* __restrict__ pc,
+    float4* __restrict__ xq, float4* __restrict__ aq, uint2* __restrict__ cq,
+    uint32_t* __restrict__ cnt) {
+  const int u = blockIdx.x * kBlock + threadIdx.x;
+  if (blockIdx.x * kBlock >= n) return;  // whole workgroup past the end
+  count_sorted(skey, u, n, cnt);
+  if (u >= n) return;
+  const uint32_t s = perm[u] < (uint32_t)n ? perm[u] : 0u;
+  const float4 a = A[s];
+  xq[u] = xl[s];
+  aq[u] = a;
+  cq[u] = make_uint2(pc[s], __float_as_uint(a.w));
+}
+
+// Gather mode: the inputs of the XSPH pass in query order, with the per-cell
+// counts of the query grid (cnt all zero beforehand).
+__global__ __launch_bounds__(kBlock) void k_bwd_gather_xsph(
+    int n, StepConsts c, const uint32_t* __restrict__ skey, const uint32_t* __restrict__ perm,
+    const float4* __restrict__ x, const float4* __restrict__ vr, const float4* __restrict__ gv,
+    const uint32_t* __restrict__ pc, float4* __restrict__ xq, float4* __restrict__ vrq,
+    float4* __restrict__ gvq, uint2* __restrict__ cq, uint32_t* __restrict__ cnt) {
+  const int u = blockIdx.x * kBlock + threadIdx.x;
+  if (blockIdx.x * kBlock >= n) return;  // whole workgroup past the end
+  count_sorted(skey, u, n, cnt);
+  if (u >= n) return;
+  const uint32_t s = perm[u] < (uint32_t)n ? perm[u] : 0u;
+  const float4 p = x[s];
+  const Cell q = cell_of(p.x, p.y, p.z, c.g, c.h);
+  xq[u] = p;
+  vrq[u] = vr[s];
+  gvq[u] = gv[s];
+  cq[u] = make_uint2(pc[s], pack_cell(q));
 }
 
 // DensityBoundary slope (fluids/Simulator.cu:139-161) at wall distance d.
@@ -492,17 +662,23 @@ __device__ __forceinline__ float wall_slope(float d, const StepConsts& c) {
   return -c.wall_c * (c.h - d) * (c.h + 3.0f * d);
 }
 
-// lambda = -(rho/pho0 - 1)/D: from dL/dlambda (gathered + scattered) to
-// dL/drho, dL/dD; folds the delta-p pair terms into gxn; writes the lambda
-// pass's per-particle inputs; clears the extras.
+// lambda = -(rho/pho0 - 1)/D: from dL/dlambda (gathered, plus in scatter
+// mode the scattered extras, which it clears) to dL/drho, dL/dD; folds the
+// delta-p pair terms into gxn; writes the lambda pass's per-particle inputs
+// (xr, sdb: in the pass's order).
+template <bool kGather>
 __global__ __launch_bounds__(kBlock) void k_bwd_lambda_pre(
-    int n, StepConsts c, int last, const float4* __restrict__ out, float4* __restrict__ ex,
-    const float4* __restrict__ xl, const float4* __restrict__ sd, const float* __restrict__ rho,
+    int n, StepConsts c, int last, const uint32_t* __restrict__ perm,
+    const float4* __restrict__ out, float4* __restrict__ ex, const float4* __restrict__ xl,
+    const float4* __restrict__ sd, const float* __restrict__ rho,
     const float* __restrict__ grho_x, float4* __restrict__ gxn, float4* __restrict__ xr,
     float4* __restrict__ sdb, double* __restrict__ r0) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
-  const float4 o = out[i], e = take_scattered(ex, i, n), x = xl[i], s = sd[i];
+  const int u = blockIdx.x * kBlock + threadIdx.x;
+  if (u >= n) return;
+  const int i = kGather ? (int)perm[u] : u;
+  float4 e = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if constexpr (!kGather) e = take_scattered(ex, u, n);
+  const float4 o = out[u], x = xl[i], s = sd[i];
   const float gl = o.w + e.w;
   const float D = s.w, rh = rho[i];
   const float C = rh / c.pho0 - 1.0f;
@@ -523,20 +699,11 @@ __global__ __launch_bounds__(kBlock) void k_bwd_lambda_pre(
     g.z += k * (wall_slope(x.z - c.g.llz, c) - wall_slope(c.g.ulz - x.z, c));
   }
   gxn[i] = g;
-  xr[i] = make_float4(x.x, x.y, x.z, gr);
-  sdb[i] = make_float4(s.x, s.y, s.z, gD);
+  xr[u] = make_float4(x.x, x.y, x.z, gr);
+  sdb[u] = make_float4(s.x, s.y, s.z, gD);
 }
 
-// Packed query cells of the final positions (for the XSPH pass).
-__global__ __launch_bounds__(kBlock) void k_bwd_qcell(int n, StepConsts c,
-                                                      const float4* __restrict__ x,
-                                                      uint32_t* __restrict__ qc) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
-  qc[i] = pack_query_cell(x[i], c);
-}
-
-// gx = gxn + scattered extras; clears the extras.
+// Scatter mode: gx = gxn + scattered extras; clears the extras.
 __global__ __launch_bounds__(kBlock) void k_bwd_fold(int n, const float4* __restrict__ gxn,
                                                      float4* __restrict__ ex,
                                                      float4* __restrict__ gx) {
@@ -626,28 +793,85 @@ int fill_blocks(int n) {
 
 }  // namespace
 
+void launch_bwd_frame(const BwdFrame& f, int n, const StepConsts& c, hipStream_t st) {
+  if (f.q) k_bwd_sorted_cells<<<blocks_for(n), kBlock, 0, st>>>(n, c, f.key, f.q->pc);
+}
+
+namespace {
+// Gather mode: sorts the sorted slots by the query keys in q.k0 (slots in
+// q.v0) and returns (sorted keys, their sorted slots).
+std::pair<const uint32_t*, const uint32_t*> sort_query(const BwdQuery& q, int n,
+                                                       hipStream_t st) {
+  const int which =
+      radix_sort_pairs(q.k0, q.v0, q.k1, q.v1, n, q.key_bits, q.hist, q.totals, st);
+  return which ? std::make_pair(q.k1, q.v1) : std::make_pair(q.k0, q.v0);
+}
+}  // namespace
+
 void launch_bwd_xsph(const BwdFrame& f, int n, const StepConsts& c, const uint32_t* cs,
                      const HeavyLists& hl, int pass, const Shapes& sh, hipStream_t st) {
-  k_bwd_qcell<<<blocks_for(n), kBlock, 0, st>>>(n, c, f.x_final, f.qc);
-  launch_pass<BwdXsphPass>(BwdXsphPass::Args{f.x_final, f.vr, f.gv_out, f.key, f.out_x, f.out_v,
-                                             f.ex_x, f.ex_v, (uint32_t)n, f.qc},
-                           n, c, Scenes{}, cs, hl, pass, sh, st);
-  k_bwd_velocity<<<blocks_for(n), kBlock, 0, st>>>(n, c, f.gx_out, f.gv_out, f.out_x, f.out_v,
-                                                   f.ex_x, f.ex_v, f.gx, f.gx0, f.grho_x);
+  if (f.q) {
+    const BwdQuery& q = *f.q;
+    k_bwd_qcell<<<blocks_for(n), kBlock, 0, st>>>(n, c, f.x_final, nullptr, q.k0, q.v0);
+    const auto kv = sort_query(q, n, st);
+    k_bwd_gather_xsph<<<blocks_for(n), kBlock, 0, st>>>(n, c, kv.first, kv.second, f.x_final,
+                                                        f.vr, f.gv_out, q.pc, q.xq, q.vrq,
+                                                        q.gvq, q.cq, q.cnt);
+    launch_cell_starts(q.cnt, c.cells, q.tile_sum, q.cs, st);
+    using P = BwdXsphPass<true>;
+    launch_pass<P>(P::Args{q.xq, q.vrq, q.gvq, nullptr, f.out_x, f.out_v, nullptr, nullptr,
+                           (uint32_t)n, nullptr, q.cq},
+                   n, c, Scenes{}, q.cs, hl, pass, sh, st);
+    k_bwd_velocity<true><<<blocks_for(n), kBlock, 0, st>>>(n, c, kv.second, f.gx_out, f.gv_out,
+                                                           f.out_x, f.out_v, nullptr, nullptr,
+                                                           f.gx, f.gx0, f.grho_x);
+    return;
+  }
+  k_bwd_qcell<<<blocks_for(n), kBlock, 0, st>>>(n, c, f.x_final, f.qc, nullptr, nullptr);
+  using P = BwdXsphPass<false>;
+  launch_pass<P>(P::Args{f.x_final, f.vr, f.gv_out, f.key, f.out_x, f.out_v, f.ex_x, f.ex_v,
+                         (uint32_t)n, f.qc, nullptr},
+                 n, c, Scenes{}, cs, hl, pass, sh, st);
+  k_bwd_velocity<false><<<blocks_for(n), kBlock, 0, st>>>(n, c, nullptr, f.gx_out, f.gv_out,
+                                                          f.out_x, f.out_v, f.ex_x, f.ex_v,
+                                                          f.gx, f.gx0, f.grho_x);
 }
 
 void launch_bwd_iteration(const BwdFrame& f, const BwdIter& it, int last, int n,
                           const StepConsts& c, const uint32_t* cs, const HeavyLists& hl, int pass,
                           const Shapes& sh, hipStream_t st) {
+  if (f.q) {
+    const BwdQuery& q = *f.q;
+    k_bwd_clamp<<<blocks_for(n), kBlock, 0, st>>>(n, c, f.gx, it.xl, it.dp, f.gxn, f.A, nullptr,
+                                                  q.k0, q.v0, f.r0);
+    const auto kv = sort_query(q, n, st);
+    k_bwd_gather_iter<<<blocks_for(n), kBlock, 0, st>>>(n, kv.first, kv.second, it.xl, f.A, q.pc,
+                                                        q.xq, q.aq, q.cq, q.cnt);
+    launch_cell_starts(q.cnt, c.cells, q.tile_sum, q.cs, st);
+    using PD = BwdDeltaPass<true>;
+    launch_pass<PD>(PD::Args{q.xq, q.aq, nullptr, f.out_x, nullptr, (uint32_t)n, q.cq}, n, c,
+                    Scenes{}, q.cs, hl, pass, sh, st);
+    k_bwd_lambda_pre<true><<<blocks_for(n), kBlock, 0, st>>>(n, c, last, kv.second, f.out_x,
+                                                             nullptr, it.xl, it.sd, it.rho,
+                                                             f.grho_x, f.gxn, q.xr, q.sdb, f.r0);
+    using PL = BwdLambdaPass<true>;
+    launch_pass<PL>(PL::Args{q.xr, q.sdb, nullptr, f.gxn, f.gx, nullptr, (uint32_t)n, nullptr,
+                             q.cq, kv.second},
+                    n, c, Scenes{}, q.cs, hl, pass + 1, sh, st);
+    return;
+  }
   k_bwd_clamp<<<blocks_for(n), kBlock, 0, st>>>(n, c, f.gx, it.xl, it.dp, f.gxn, f.A, f.qc,
-                                                f.r0);
-  launch_pass<BwdDeltaPass>(BwdDeltaPass::Args{it.xl, f.A, f.key, f.out_x, f.ex_x, (uint32_t)n}, n, c,
-                            Scenes{}, cs, hl, pass, sh, st);
-  k_bwd_lambda_pre<<<blocks_for(n), kBlock, 0, st>>>(n, c, last, f.out_x, f.ex_x, it.xl, it.sd,
-                                                     it.rho, f.grho_x, f.gxn, f.xr, f.sdb, f.r0);
-  launch_pass<BwdLambdaPass>(
-      BwdLambdaPass::Args{f.xr, f.sdb, f.key, f.gxn, f.ex_x, (uint32_t)n, f.qc}, n, c, Scenes{},
-      cs, hl, pass + 1, sh, st);
+                                                nullptr, nullptr, f.r0);
+  using PD = BwdDeltaPass<false>;
+  launch_pass<PD>(PD::Args{it.xl, f.A, f.key, f.out_x, f.ex_x, (uint32_t)n, nullptr}, n, c,
+                  Scenes{}, cs, hl, pass, sh, st);
+  k_bwd_lambda_pre<false><<<blocks_for(n), kBlock, 0, st>>>(n, c, last, nullptr, f.out_x, f.ex_x,
+                                                            it.xl, it.sd, it.rho, f.grho_x, f.gxn,
+                                                            f.xr, f.sdb, f.r0);
+  using PL = BwdLambdaPass<false>;
+  launch_pass<PL>(PL::Args{f.xr, f.sdb, f.key, f.gxn, f.gxn, f.ex_x, (uint32_t)n, f.qc, nullptr,
+                           nullptr},
+                  n, c, Scenes{}, cs, hl, pass + 1, sh, st);
   k_bwd_fold<<<blocks_for(n), kBlock, 0, st>>>(n, f.gxn, f.ex_x, f.gx);
 }
 

@@ -35,6 +35,8 @@ constexpr float kKernelEps = 1e-4f;
 constexpr float kMaxDp = 0.1f;
 constexpr int kMaxIter = 1 << 16;
 constexpr long long kMaxCellsLimit = 1ll << 31;
+// Tapes from this many particles run the gather-mode adjoint by default.
+constexpr int kGatherMinParticles = 1 << 18;
 constexpr int kMaxParticles = 1 << 30;
 constexpr int kMaxScenes = 65535;
 
@@ -169,6 +171,13 @@ struct pbf_tape {
          *gxo = nullptr, *gvo = nullptr;
   float* grho_x = nullptr;
   uint32_t* qc = nullptr;           // packed query cells
+  // gather mode (PBF_TAPE_GATHER, or automatic): the query grid's sort
+  // buffers and scratch, the packed sorted cells; its query-order arrays
+  // live in the memory of ex_x and ex_v, which scatter mode needs all zero
+  int adjoint_flag = 0;             // PBF_TAPE_GATHER, PBF_TAPE_SCATTER or 0 (automatic)
+  bool ex_dirty = false;            // ex_x / ex_v hold gather-mode data
+  uint32_t *qk0 = nullptr, *qv0 = nullptr, *qk1 = nullptr, *qv1 = nullptr, *qhist = nullptr,
+           *qtotals = nullptr, *pc = nullptr;
   float *gp[2] = {nullptr, nullptr}, *gv[2] = {nullptr, nullptr};  // inter-frame adjoints
   double *r0 = nullptr, *r0_sum = nullptr;  // r0_sum: total, then 1024 partial sums
   pbf_params prm{};                 // parameters of the forward run
@@ -1325,7 +1334,10 @@ int pbf_tape_create_ex(pbf_solver* s, int n, int nframes, int flags, pbf_tape** 
                 n);
   if (nframes < 1 || nframes > 100000)
     return fail(s, PBF_ERR_INVALID_ARGUMENT, "nframes must be in [1, 100000], got %d", nframes);
-  if (flags & ~PBF_TAPE_STORE) return fail(s, PBF_ERR_INVALID_ARGUMENT, "unknown tape flags %d", flags);
+  if (flags & ~(PBF_TAPE_STORE | PBF_TAPE_GATHER | PBF_TAPE_SCATTER))
+    return fail(s, PBF_ERR_INVALID_ARGUMENT, "unknown tape flags %d", flags);
+  if ((flags & PBF_TAPE_GATHER) && (flags & PBF_TAPE_SCATTER))
+    return fail(s, PBF_ERR_INVALID_ARGUMENT, "PBF_TAPE_GATHER and PBF_TAPE_SCATTER exclude each other");
   DeviceGuard g(s->device);
   if (!g.ok) return fail(s, PBF_ERR_HIP, "hipSetDevice(%d) failed", s->device);
   pbf_tape* t = new (std::nothrow) pbf_tape();
@@ -1336,6 +1348,7 @@ int pbf_tape_create_ex(pbf_solver* s, int n, int nframes, int flags, pbf_tape** 
   t->niter = s->prm.niter;
   t->layout = s->layout;
   t->store = (flags & PBF_TAPE_STORE) != 0;
+  t->adjoint_flag = flags & (PBF_TAPE_GATHER | PBF_TAPE_SCATTER);
   const size_t N = (size_t)n, L = (size_t)s->layout, NI = (size_t)t->niter;
   size_t off = 0;
   auto carve = [&](size_t bytes) {
@@ -1361,6 +1374,9 @@ int pbf_tape_create_ex(pbf_solver* s, int n, int nframes, int flags, pbf_tape** 
   size_t o_f4[12];
   for (int q = 0; q < 12; ++q)  // ex_x and ex_v (2, 3) hold kScatterCopies copies
     o_f4[q] = carve(16 * N * (q == 2 || q == 3 ? (size_t)pbf::kScatterCopies : 1));
+  const size_t o_qk0 = carve(4 * N), o_qv0 = carve(4 * N), o_qk1 = carve(4 * N),
+               o_qv1 = carve(4 * N), o_qhist = carve(4 * 256 * (size_t)pbf::radix_tiles(n)),
+               o_qtot = carve(4 * 256), o_pc = carve(4 * N);
   const size_t o_grx = carve(4 * N), o_qc = carve(4 * N), o_gp0 = carve(4 * L * N), o_gp1 = carve(4 * L * N),
                o_gv0 = carve(4 * L * N), o_gv1 = carve(4 * L * N), o_r0 = carve(8 * N),
                o_r0s = carve(8 * (1 + 1024));
@@ -1408,6 +1424,13 @@ int pbf_tape_create_ex(pbf_solver* s, int n, int nframes, int flags, pbf_tape** 
   for (int q = 0; q < 12; ++q) *f4[q] = (float4*)(b + o_f4[q]);
   t->grho_x = (float*)(b + o_grx);
   t->qc = (uint32_t*)(b + o_qc);
+  t->qk0 = (uint32_t*)(b + o_qk0);
+  t->qv0 = (uint32_t*)(b + o_qv0);
+  t->qk1 = (uint32_t*)(b + o_qk1);
+  t->qv1 = (uint32_t*)(b + o_qv1);
+  t->qhist = (uint32_t*)(b + o_qhist);
+  t->qtotals = (uint32_t*)(b + o_qtot);
+  t->pc = (uint32_t*)(b + o_pc);
   t->gp[0] = (float*)(b + o_gp0);
   t->gp[1] = (float*)(b + o_gp1);
   t->gv[0] = (float*)(b + o_gv0);
@@ -1591,6 +1614,53 @@ int pbf_tape_backward(pbf_solver* s, pbf_tape* t, const float* gpos, const float
              : hipMemsetAsync(t->gv[0], 0, vb, st);
   if (e != hipSuccess) return fail_hip(s, e, "seeding the backward pass");
   pbf::launch_fill_f64(t->r0, n, st);
+  // Adjoint mode: gather (each pass over a grid of its own positions, no
+  // atomics) from kGatherMinParticles particles or on request, when every
+  // frame's grid has at most 1024 cells per axis; else scatter.
+  bool gather = t->adjoint_flag != PBF_TAPE_SCATTER &&
+                (t->adjoint_flag == PBF_TAPE_GATHER || n >= kGatherMinParticles);
+  for (int f = 0; gather && f < t->nframes; ++f) {
+    pbf::SceneGeo geo;
+    long long cells = 0;
+    if ((rc = make_geo(s, s->prm, t->ul[(size_t)f], t->ll[(size_t)f], 0, geo, cells))) return rc;
+    gather = geo.dx <= 1024 && geo.dy <= 1024 && geo.dz <= 1024;
+  }
+  if (t->adjoint_flag == PBF_TAPE_GATHER && !gather)
+    return fail(s, PBF_ERR_INVALID_ARGUMENT,
+                "PBF_TAPE_GATHER needs grids of at most 1024 cells per axis");
+  pbf::BwdQuery bq{};
+  if (gather) {
+    // the query-order arrays take the scatter sums' memory
+    t->ex_dirty = true;
+    char* ex = (char*)t->ex_x;
+    char* ev = (char*)t->ex_v;
+    const size_t v16 = 16 * (size_t)n;
+    bq.k0 = t->qk0;
+    bq.v0 = t->qv0;
+    bq.k1 = t->qk1;
+    bq.v1 = t->qv1;
+    bq.hist = t->qhist;
+    bq.totals = t->qtotals;
+    bq.cnt = s->cnt;
+    bq.tile_sum = s->tile_sum;
+    bq.cs = s->cs;
+    bq.pc = t->pc;
+    bq.xq = (float4*)ex;
+    bq.aq = (float4*)(ex + v16);
+    bq.xr = (float4*)(ex + 2 * v16);
+    bq.sdb = (float4*)(ex + 3 * v16);
+    bq.cq = (uint2*)(ex + 4 * v16);
+    bq.vrq = (float4*)ev;
+    bq.gvq = (float4*)(ev + v16);
+    if (s->cnt_dirty) {
+      pbf::launch_fill_u32(s->cnt, s->max_cells, 0u, st);
+      s->cnt_dirty = false;
+    }
+  } else if (t->ex_dirty) {
+    pbf::launch_fill_f4(t->ex_x, n * pbf::kScatterCopies, st);
+    pbf::launch_fill_f4(t->ex_v, n * pbf::kScatterCopies, st);
+    t->ex_dirty = false;
+  }
   mark_push("pbf.tape_backward");
   struct PopOnExit {
     ~PopOnExit() { mark_pop(); }
@@ -1666,6 +1736,9 @@ int pbf_tape_backward(pbf_solver* s, pbf_tape* t, const float* gpos, const float
     bf.sdb = t->sdb;
     bf.grho_x = t->grho_x;
     bf.qc = t->qc;
+    bf.q = gather ? &bq : nullptr;
+    bq.key_bits = key_bits_for(c.cells);
+    pbf::launch_bwd_frame(bf, n, c, st);
     bf.r0 = t->r0;
     pbf::launch_bwd_xsph(bf, n, c, cs, s->hl, pass++, s->shapes, st);
     for (int k = NI - 1; k >= 0; --k) {

@@ -11,14 +11,28 @@
 
 namespace pbf {
 
-// One frame's backward state.
-// Copies of the scattered pair sums: workgroup b adds to copy b % 8 (the XCD
+// Gather mode (see adjoint.hip): each pass runs over a grid built on its own
+// positions.  Device arrays of n entries unless noted.
+struct BwdQuery {
+  uint32_t *k0, *v0, *k1, *v1;    // radix sort buffers: query keys, sorted slots
+  uint32_t *hist, *totals;        // radix sort scratch (radix_sort_pairs)
+  int key_bits;                   // bits of the query keys
+  uint32_t *cnt, *tile_sum, *cs;  // per-cell counts (all zero between uses), scan tiles,
+                                  // cell starts (max cells + 1)
+  uint32_t* pc;                   // packed sorted cell of each sorted slot (per frame)
+  float4 *xq, *aq, *xr, *sdb;     // an iteration's pass inputs in query order
+  float4 *vrq, *gvq;              // the XSPH pass's inputs in query order (x in xq)
+  uint2* cq;                      // packed (sorted cell, query cell) in query order
+};
+
+// Scatter mode's copies of the scattered pair sums: workgroup b adds to copy b % 8 (the XCD
 // it runs on, as workgroups are dispatched round-robin), which spreads the
 // atomic adds a crowded cell sends to one particle over eight addresses
 // (same-address atomics serialise); the next pointwise kernel adds the
 // copies.
 constexpr int kScatterCopies = 8;
 
+// One frame's backward state.
 struct BwdFrame {
   // recorded forward (read)
   const float4* x_final;  // positions after the last iteration
@@ -29,8 +43,8 @@ struct BwdFrame {
   const float4* gv_out;   // dL/d(velocities after XSPH)
   // working arrays
   float4 *out_x, *out_v;  // gathered pair sums
-  float4 *ex_x, *ex_v;    // scattered pair sums: kScatterCopies copies of n entries,
-                          // component-major (all zero between kernels)
+  float4 *ex_x, *ex_v;    // scatter mode: scattered pair sums, kScatterCopies copies of n
+                          // entries, component-major (all zero between kernels)
   float4* gx;             // dL/d(positions) of the current iteration
   float4* gxn;            // the same for the iteration before
   float4* gx0;            // dL/d(start positions) from v = (x - x0)/dt
@@ -39,6 +53,7 @@ struct BwdFrame {
   float4 *xr, *sdb;       // lambda-pass inputs: (x, dL/drho), (S, dL/dD)
   float* grho_x;          // dL/drho of the last lambda pass, from XSPH
   double* r0;             // per-slot dL/dpho0 accumulators
+  const BwdQuery* q;      // gather mode's arrays; null: scatter mode
 };
 
 // One solver iteration's record.
@@ -49,6 +64,8 @@ struct BwdIter {
   const float4* dp;   // correction / pho0 before the clamps
 };
 
+// Once per frame before its passes (gather mode: the packed sorted cells).
+void launch_bwd_frame(const BwdFrame& f, int n, const StepConsts& c, hipStream_t st);
 // XSPH and velocity update: fills gx (dL/d final positions), gx0, grho_x.
 // Uses the neighbour passes `pass` (one pass number).
 void launch_bwd_xsph(const BwdFrame& f, int n, const StepConsts& c, const uint32_t* cs,

@@ -678,4 +678,35 @@ __device__ __forceinline__ void store_vec(float* __restrict__ a, uint32_t i, flo
   }
 }
 
+// Particles per cell, counted from the sorted keys (the input of the
+// cell-start scan).  Equal keys are contiguous after the sort, so a run that
+// lies inside one wavefront is that cell's only run and its first lane
+// stores the count; a run touching a wavefront boundary (at most two per
+// wavefront) may continue in the neighbouring wavefront, and each of its
+// parts is added atomically.  The counts must be zero beforehand (the scan
+// re-zeroes them).
+__device__ __forceinline__ void count_sorted(const uint32_t* __restrict__ skey, int s, int n,
+                                             uint32_t* __restrict__ cnt) {
+  const int lane = threadIdx.x & 63;
+  const bool valid = s < n;
+  const uint32_t k = valid ? skey[s] : 0u;
+  const uint32_t prev = __shfl_up(k, 1);
+  const uint64_t vm = __ballot(valid);
+  const bool head = valid && (lane == 0 || prev != k);
+  const uint64_t hm = __ballot(head);
+  if (head) {
+    // run = [lane, next head or end of the valid lanes)
+    const uint64_t later = lane == 63 ? 0ull : (hm >> (lane + 1)) << (lane + 1);
+    const int end = later ? __ffsll((unsigned long long)later) - 1 : 64 - __clzll(vm);
+    const int len = end - lane;
+    const int after = s + len;  // sorted index just past the run
+    const bool open_left = lane == 0 && s > 0 && skey[s - 1] == k;
+    const bool open_right = !later && after < n && skey[after] == k;
+    if (open_left || open_right)
+      atomicAdd(&cnt[k], (uint32_t)len);
+    else
+      cnt[k] = (uint32_t)len;
+  }
+}
+
 }  // namespace pbf

@@ -255,6 +255,8 @@ class Simulator:
 
 
 TAPE_STORE = 1
+TAPE_GATHER = 2
+TAPE_SCATTER = 4
 
 
 class Tape:
@@ -262,18 +264,25 @@ class Tape:
     its backward pass (pbf_tape_*).  store=True keeps every frame's solver
     iterations, sort and grid on the device (pbf_tape_create_ex with
     PBF_TAPE_STORE), so the backward pass recomputes nothing; the gradients
-    are the same bits either way."""
+    are the same bits either way.  adjoint: None (automatic), "gather" or
+    "scatter" (PBF_TAPE_GATHER / PBF_TAPE_SCATTER: how the backward pass
+    collects the pair terms the stale grid makes one-sided)."""
 
-    def __init__(self, sim: "Simulator", nparticle: int, nframes: int, store: bool = False):
+    def __init__(self, sim: "Simulator", nparticle: int, nframes: int, store: bool = False,
+                 adjoint: Optional[str] = None):
         self._lib = _lib.load()
         self.sim = sim
         self.n = int(nparticle)
         self.nframes = int(nframes)
         self.store = bool(store)
+        modes = {None: 0, "gather": TAPE_GATHER, "scatter": TAPE_SCATTER}
+        if adjoint not in modes:
+            raise ValueError(f"adjoint must be None, 'gather' or 'scatter', not {adjoint!r}")
+        self.adjoint = adjoint
         self._t = ctypes.c_void_p()
         h = sim._handle()
         _lib.check(self._lib.pbf_tape_create_ex(h, self.n, self.nframes,
-                                                TAPE_STORE if self.store else 0,
+                                                (TAPE_STORE if self.store else 0) | modes[adjoint],
                                                 ctypes.byref(self._t)), h)
 
     def close(self) -> None:

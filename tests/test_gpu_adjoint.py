@@ -7,7 +7,10 @@ pbf.autograd):
   forward exactly), so the comparison isolates the backward arithmetic;
 * the GPU gradient against FP64 central finite differences of the FP64
   rollout (reference constants, where the rollout is smooth at the step);
-* the autograd.Function wrapper gives the same gradients as the C ABI.
+* the autograd.Function wrapper gives the same gradients as the C ABI;
+* both ways of collecting the pair terms the stale grid makes one-sided
+  (scatter: float atomics; gather: each pass over a grid of its own
+  positions) match the oracle, the gather mode repeats bit for bit.
 The oracle's adjoint itself is pinned by finite differences in
 tests/test_adjoint_oracle.py.
 """
@@ -31,12 +34,12 @@ def _target(sc, seed=1):
     return sc.pos.astype(np.float64) + rng.normal(0, 0.05, (sc.n, 3))
 
 
-def gpu_grad(pbf_mod, sc, p, nframes, target, layout=4, store=False, sweep=None):
+def gpu_grad(pbf_mod, sc, p, nframes, target, layout=4, store=False, sweep=None, adjoint=None):
     """Loss gradients from the C ABI: (g_pos0, g_vel0 as N x 3 in input order,
     g_rho0, final positions by identity)."""
     n = sc.n
     sim = pbf_mod.Simulator(p, sc.ulim, sc.llim, capacity=n, layout=layout)
-    tape = pbf_mod.Tape(sim, n, nframes, store=store)
+    tape = pbf_mod.Tape(sim, n, nframes, store=store, adjoint=adjoint)
     pos, vel = sc.as_layout(layout)
     dev = "cuda:0"
     tape.forward(to_dev(pos, dev), to_dev(vel, dev), to_dev(sc.iid, dev), sweep=sweep)
@@ -83,15 +86,16 @@ def test_tape_forward_equals_steps(pbf_mod, device):
         np.testing.assert_array_equal(from_dev(xi).astype(np.uint32), g["iid"])
 
 
+@pytest.mark.parametrize("adjoint", ["scatter", "gather"])
 @pytest.mark.parametrize("pname,nframes,layout", [("reference", 3, 4), ("reference", 3, 3),
                                                   ("dam", 1, 4)])
 def test_adjoint_matches_oracle_on_the_same_trajectory(pbf_mod, orc, device, pname, nframes,
-                                                       layout):
+                                                       layout, adjoint):
     p = P.dam_params() if pname == "dam" else P.reference_defaults()
     sc = _scene()
     tgt = _target(sc)
-    print(f"adjoint {pname} frames {nframes} layout {layout} n {sc.n}", flush=True)
-    gp, gv, gr, xf = gpu_grad(pbf_mod, sc, p, nframes, tgt, layout)
+    print(f"adjoint {pname} frames {nframes} layout {layout} n {sc.n} {adjoint}", flush=True)
+    gp, gv, gr, xf = gpu_grad(pbf_mod, sc, p, nframes, tgt, layout, adjoint=adjoint)
     o = orc.rollout(p, sc.ulim, sc.llim, sc.pos, sc.vel, sc.iid, nframes, tgt, fp32=True)
     np.testing.assert_array_equal(xf.astype(np.float32), o["x_final"].astype(np.float32))
     assert rel(gv, o["g_vel0"]) < 1e-4, rel(gv, o["g_vel0"])
@@ -99,13 +103,16 @@ def test_adjoint_matches_oracle_on_the_same_trajectory(pbf_mod, orc, device, pna
     assert abs(gr - o["g_rho0"]) < 1e-3 * abs(o["g_rho0"]), (gr, o["g_rho0"])
 
 
-def test_adjoint_with_scattered_pairs(pbf_mod, orc, device):
+@pytest.mark.parametrize("adjoint", ["scatter", "gather"])
+def test_adjoint_with_scattered_pairs(pbf_mod, orc, device, adjoint):
     """Fast particles leave their stale cells, so candidate lists become
-    asymmetric and the scatter path carries terms; still equal to the oracle."""
+    asymmetric: the scatter mode adds the one-sided terms with atomics, the
+    gather mode finds them on its query grid; both equal the oracle."""
     p = P.reference_defaults()
     sc = scenes.block((8, 8, 8), (1.0, 1.0, 1.0), seed=9, vel_sigma=6.0)
     tgt = _target(sc)
-    gp, gv, gr, _ = gpu_grad(pbf_mod, sc, p, 2, tgt)
+    print(f"one-sided pairs, {adjoint}", flush=True)
+    gp, gv, gr, _ = gpu_grad(pbf_mod, sc, p, 2, tgt, adjoint=adjoint)
     o = orc.rollout(p, sc.ulim, sc.llim, sc.pos, sc.vel, sc.iid, 2, tgt, fp32=True)
     assert rel(gv, o["g_vel0"]) < 1e-4, rel(gv, o["g_vel0"])
     assert rel(gp, o["g_pos0"]) < 1e-4, rel(gp, o["g_pos0"])
@@ -183,14 +190,16 @@ def test_tape_argument_errors(pbf_mod, device):
     assert e.value.code == 5
 
 
-def test_adjoint_with_a_moving_wall_matches_oracle(pbf_mod, orc, device):
+@pytest.mark.parametrize("adjoint", ["scatter", "gather"])
+def test_adjoint_with_a_moving_wall_matches_oracle(pbf_mod, orc, device, adjoint):
     p = P.reference_defaults()
     sc = _scene()
     tgt = _target(sc)
     sw = pbf_mod.Sweep(sc.ulim, sc.llim, a_ulim=(-0.75, 0.0, 0.0), w=0.5)
     n = sc.n
+    print(f"moving wall, {adjoint}", flush=True)
     sim = pbf_mod.Simulator(p, sc.ulim, sc.llim, capacity=n)
-    tape = pbf_mod.Tape(sim, n, 3)
+    tape = pbf_mod.Tape(sim, n, 3, adjoint=adjoint)
     pos, vel = sc.as_layout(4)
     tape.forward(to_dev(pos, device), to_dev(vel, device), to_dev(sc.iid, device), sweep=sw)
     assert sim.getLim()[0][0] == pytest.approx(sw.box(2)[0][0])
@@ -274,8 +283,59 @@ def test_tape_rejects_unknown_flags(pbf_mod, device):
     sim = pbf_mod.Simulator(p, sc.ulim, sc.llim, capacity=sc.n)
     lib = _lib.load()
     t = ctypes.c_void_p()
-    assert lib.pbf_tape_create_ex(sim._handle(), sc.n, 2, 4, ctypes.byref(t)) == 1
+    assert lib.pbf_tape_create_ex(sim._handle(), sc.n, 2, 8, ctypes.byref(t)) == 1
     assert b"flags" in lib.pbf_last_error(sim._handle())
+    assert lib.pbf_tape_create_ex(sim._handle(), sc.n, 2, 2 | 4, ctypes.byref(t)) == 1
+    assert b"exclude" in lib.pbf_last_error(sim._handle())
+    with pytest.raises(ValueError):
+        pbf_mod.Tape(sim, sc.n, 2, adjoint="atomic")
+
+
+def test_gather_adjoint_needs_small_grids(pbf_mod, device):
+    """A forced gather-mode tape on a grid wider than 1024 cells fails its
+    backward pass with PBF_ERR_INVALID_ARGUMENT; the automatic mode falls
+    back to scattering there and still gives finite gradients."""
+    p = P.reference_defaults()
+    sc = scenes.block((8, 8, 8), (1.0, 1.0, 1.0), seed=5, vel_sigma=0.5)
+    ulim = (110.0, 1.0, 1.0)  # 1100 cells along x at h = 0.1
+    n = sc.n
+    out = {}
+    for adjoint in ("gather", None):
+        sim = pbf_mod.Simulator(p, ulim, sc.llim, capacity=n)
+        tape = pbf_mod.Tape(sim, n, 1, adjoint=adjoint)
+        pos, vel = sc.as_layout(4)
+        tape.forward(to_dev(pos, device), to_dev(vel, device), to_dev(sc.iid, device))
+        g = torch.ones(n * 4, device=device)
+        gp0 = torch.zeros(n * 4, device=device)
+        gv0 = torch.zeros(n * 4, device=device)
+        if adjoint == "gather":
+            with pytest.raises(RuntimeError, match="1024"):
+                tape.backward(g, None, gp0, gv0)
+        else:
+            tape.backward(g, None, gp0, gv0)
+            out["g"] = from_dev(gv0)
+        tape.close()
+    assert np.all(np.isfinite(out["g"])) and np.abs(out["g"]).sum() > 0
+
+
+@pytest.mark.parametrize("store", [False, True])
+def test_gather_adjoint_repeats_bit_for_bit_and_matches_scatter(pbf_mod, device, store):
+    """The gather mode adds every term in a fixed order: two backward passes
+    (and a second tape) give the same bits; the scatter mode, whose atomics
+    add the one-sided terms in varying order, agrees within 1e-5."""
+    p = P.reference_defaults()
+    sc = scenes.block((10, 10, 10), (1.0, 1.0, 1.0), seed=9, vel_sigma=4.0)
+    tgt = _target(sc)
+    a = gpu_grad(pbf_mod, sc, p, 3, tgt, store=store, adjoint="gather")
+    b = gpu_grad(pbf_mod, sc, p, 3, tgt, store=store, adjoint="gather")
+    c = gpu_grad(pbf_mod, sc, p, 3, tgt, store=store, adjoint="scatter")
+    np.testing.assert_array_equal(a[0], b[0])
+    np.testing.assert_array_equal(a[1], b[1])
+    assert a[2] == b[2]
+    np.testing.assert_array_equal(a[3], c[3])
+    assert rel(a[0], c[0]) < 1e-5, rel(a[0], c[0])
+    assert rel(a[1], c[1]) < 1e-5, rel(a[1], c[1])
+    assert abs(a[2] - c[2]) <= 1e-5 * abs(c[2]), (a[2], c[2])
 
 
 # ---- the differentiable workload at its own size --------------------------
@@ -297,16 +357,17 @@ def _unit_dirs(shape, k, seed):
     return out
 
 
-@pytest.mark.parametrize("store", [False, True])
-def test_c4_adjoint_matches_oracle_over_30_frames(pbf_mod, orc, device, store):
+@pytest.mark.parametrize("store,adjoint", [(False, None), (True, None), (False, "gather")])
+def test_c4_adjoint_matches_oracle_over_30_frames(pbf_mod, orc, device, store, adjoint):
     """Reference constants, all 30 frames: the GPU's final state equals the
     oracle's FP32 trajectory bit for bit and its gradients (checkpointed tape
-    with the iterations recomputed, and the stored tape) equal the oracle's
-    adjoint along that trajectory within 1e-4."""
+    with the iterations recomputed, and the stored tape; the automatic mode
+    scatters at this size, and the gather mode) equal the oracle's adjoint
+    along that trajectory within 1e-4."""
     p = P.reference_defaults()
     sc, tgt = _c4()
-    print(f"c4 adjoint, reference constants, 30 frames, store={store}", flush=True)
-    gp, gv, gr, xf = gpu_grad(pbf_mod, sc, p, 30, tgt, store=store)
+    print(f"c4 adjoint, reference constants, 30 frames, store={store} {adjoint}", flush=True)
+    gp, gv, gr, xf = gpu_grad(pbf_mod, sc, p, 30, tgt, store=store, adjoint=adjoint)
     o = orc.rollout(p, sc.ulim, sc.llim, sc.pos, sc.vel, sc.iid, 30, tgt, fp32=True)
     np.testing.assert_array_equal(xf.astype(np.float32), o["x_final"].astype(np.float32))
     assert np.all(np.isfinite(gv)) and np.all(np.isfinite(gp)) and np.isfinite(gr)

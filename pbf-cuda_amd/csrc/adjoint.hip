@@ -310,6 +310,9 @@ struct BwdDeltaPass {
   // scatter mode: round-robin workgroups (see BwdXsphPass); gather mode: one
   // contiguous range per XCD
   static constexpr bool kXcdRanges = kGather;
+  // hit masks for the lambda pass of the same iteration, which sweeps the
+  // same candidates of the same positions (its hit test below)
+  static constexpr int kMask = kMaskStore;
   // hits reloaded per round of the sweep
   static constexpr int kHitBatch = 4;
   struct Args {
@@ -353,9 +356,11 @@ struct BwdDeltaPass {
   static __device__ __forceinline__ bool hit(const Target& t, const float3& xj,
                                              const StepConsts& c) {
     float dx, dy, dz;
-    // i itself (r2 = 0) and coincident pairs add nothing
+    // i itself and coincident pairs (r2 = 0) add nothing; pairs closer than
+    // KERNAL_EPS add nothing here either (add() drops them) but are hits of
+    // the lambda pass, which uses these masks
     const float r2 = dist2(t.x, xj, dx, dy, dz);
-    return (r2 < c.h2_outer) & (r2 >= c.keps2_lo);
+    return (r2 < c.h2_outer) & (r2 > 0.0f);
   }
   static __device__ __forceinline__ void add(Acc& s, const Target& t, uint32_t j, const Nb& nb,
                                              const StepConsts& c, const Args& a) {
@@ -420,6 +425,8 @@ struct BwdLambdaPass {
   // scatter mode: round-robin workgroups (see BwdXsphPass); gather mode: one
   // contiguous range per XCD
   static constexpr bool kXcdRanges = kGather;
+  // the delta-p pass's hit masks (its hit test is this pass's)
+  static constexpr int kMask = kMaskLoad;
   // hits reloaded per round of the sweep
   static constexpr int kHitBatch = 4;
   struct Args {
@@ -464,6 +471,7 @@ struct BwdLambdaPass {
                                              const StepConsts& c) {
     float dx, dy, dz;
     // coincident pairs (and i itself) have d = 0: every term below vanishes
+    // (the delta-p pass stores masks of this test)
     const float r2 = dist2(t.x, xj, dx, dy, dz);
     return (r2 < c.h2_outer) & (r2 > 0.0f);
   }

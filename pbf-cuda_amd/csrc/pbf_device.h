@@ -517,7 +517,8 @@ struct HeavyLists {
   uint32_t* counters;
   uint4* recs;         // per heavy target: (i, first partial slot, chunks, 0)
   uint4* items;        // per chunk: (i, chunk index, partial slot, 0); i = ~0u: unused
-  float* partials;     // kPartialWords floats per partial slot
+  float* partials;     // per partial slot the pass's running sum, packed (at most
+                       // kPartialWords floats)
   uint32_t item_cap;   // capacity of items / partials
   // hit masks of the current iteration: block b of target i at
   // masks[b * mask_stride + i], b < kMaskBlocks (null: not kept)

@@ -67,6 +67,15 @@ inline size_t light_lds_bytes(int lb, bool staged) {
 
 inline int blocks_for(long long n) { return (int)((n + kBlock - 1) / kBlock); }
 
+// Floats per chunk partial of pass policy P: its Acc, packed (the partial
+// buffer holds kPartialWords floats per slot, the largest Acc).
+template <class P>
+__host__ __device__ constexpr uint32_t partial_words() {
+  static_assert(sizeof(typename P::Acc) % sizeof(float) == 0, "Acc must be floats");
+  static_assert(sizeof(typename P::Acc) <= kPartialWords * sizeof(float), "partial too large");
+  return (uint32_t)(sizeof(typename P::Acc) / sizeof(float));
+}
+
 // ---------------------------------------------------------------------------
 // The hit-mask hooks of sweep_seq for target ti with m candidates under
 // policy P (MaskMode, pbf_device.h).
@@ -387,8 +396,7 @@ __global__ __launch_bounds__(kBlock, PBF_PASS_WAVES) void k_chunks(typename P::A
         [&](uint32_t j, const auto& d) { P::add(part, t, j, d, c, a); }, [] {},
         [&](uint32_t b, uint64_t& mk) { return mio.get(b, mk); },
         [&](uint32_t b, uint64_t mk) { mio.put(b, mk); });
-    static_assert(sizeof(part) <= kPartialWords * sizeof(float), "partial too large");
-    *reinterpret_cast<typename P::Acc*>(hl.partials + (size_t)it.z * kPartialWords) = part;
+    *reinterpret_cast<typename P::Acc*>(hl.partials + (size_t)it.z * partial_words<P>()) = part;
   }
 }
 
@@ -429,9 +437,9 @@ __global__ __launch_bounds__(kBlock) void k_combine(typename P::Args a, int n, S
     const typename P::Target t = P::target(a, rec.x, c);
     typename P::Acc s;
     using PAcc = typename P::Acc;
-    const float* base = hl.partials + (size_t)rec.y * kPartialWords;
+    const float* base = hl.partials + (size_t)rec.y * partial_words<P>();
     auto part = [&](uint32_t k) {
-      return *reinterpret_cast<const PAcc*>(base + (size_t)k * kPartialWords);
+      return *reinterpret_cast<const PAcc*>(base + (size_t)k * partial_words<P>());
     };
     // eight loads in flight, then the adds in chunk order
     uint32_t k = 0;

@@ -344,6 +344,8 @@ def c4_workload(pbf, dev, stream):
     head = out["reference_constants"]["recompute"]
     return {"value": head["value"], "unit": "backward time / forward time",
             "tape": head["tape"], "constants": "reference defaults (FluidSystem.cpp:15-25)",
+            "adjoint_mode": "scatter: one-sided pair terms added with float atomics (the "
+                            "library's choice below 2^18 particles)",
             "forward_ms": head["forward_ms"], "backward_ms": head["backward_ms"],
             "gradients_finite": head["gradients_finite"],
             "workload": "differentiable_16k: 16,384 particles (32x16x32 jittered lattice, "
@@ -481,6 +483,8 @@ def sweep_workload(pbf, dev, stream, frames=120, back=10):
             del tape, xf, iidf, g, gp0, gv0
     out["backward_forward_ratio"] = out["recompute"]["backward_forward_ratio"]
     out["tape"] = "checkpoints, iterations recomputed"
+    out["adjoint_mode"] = ("gather: each backward pass over a grid of its own positions, no "
+                           "atomics (the library's choice from 2^18 particles)")
     out["gradients_note"] = SWEEP_GRAD_NOTE
     out["workload"] = ("sweep_dam_4m: 256x64x256 jittered lattice in a 160h x 64h x 160h box, "
                        "+x wall 160h + 20h sin(2 pi f/120), dam constants, 6 iterations; "

@@ -43,6 +43,9 @@ HBM_PEAK_GBS = 8000.0
 # Algorithmic bytes per particle of one lambda launch: read x (12 B), write
 # lambda and rho (8 B).
 LAMBDA_BYTES_PER_PARTICLE = 20.0
+# Algorithmic bytes per particle of the predict kernel (advect + cell key):
+# read x, v (32 B), write the predicted x (16 B), key and index (8 B).
+PREDICT_BYTES_PER_PARTICLE = 56.0
 # Non-packed FP32 VALU issue peak: 256 CUs x 64 lanes x 2.4 GHz, i.e. one
 # wave64 instruction per CU per clock (the build keeps packed FP32 off).
 VALU_WAVE_INSTS_PER_S = 256 * 2.4e9
@@ -665,6 +668,14 @@ def main():
                 "algorithmic_bytes": LAMBDA_BYTES_PER_PARTICLE * n,
                 "algorithmic_bytes_per_particle": LAMBDA_BYTES_PER_PARTICLE}
 
+    p_ms = phases["predict"][0] / phases["predict"][1]
+    p_gbs = PREDICT_BYTES_PER_PARTICLE * n / (p_ms * 1e-3) / 1e9
+    predict_roofline = {"bound": "hbm", "kernel": "k_predict_key (advect + cell key)",
+                        "achieved": round(p_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": p_gbs / HBM_PEAK_GBS, "launch_us": round(p_ms * 1e3, 2),
+                        "algorithmic_bytes_per_particle": PREDICT_BYTES_PER_PARTICLE,
+                        "launch_timing": "HIP events around each predict launch, eager run"}
+
     # batched: 64 scenes of uneven size in one launch sequence, graph mode
     batched = None
     if not args.no_batch:
@@ -786,6 +797,7 @@ def main():
                           "achieved": round(sb / (ms * 1e-3) / 1e9, 3), "peak": HBM_PEAK_GBS,
                           "unit": "GB/s", "frac": sb / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "valu": measured_valu(),
+        "predict_roofline": predict_roofline,
         "cpu_baseline": cpu,
         "phase_ms": phase_ms,
         "batched_rate": batched,

@@ -383,6 +383,29 @@ struct SeqCursor1 {
   }
 };
 
+// The hit mask of the candidates [bb, be) (at most 64, one canonical block)
+// of a target's sequence: eight screening loads in flight at a time, bit
+// u - bb set where hit() holds.  c1 walks forward only.
+template <class LoadS, class Hit>
+__device__ __forceinline__ uint64_t screen_block(SeqCursor& c1, uint32_t bb, uint32_t be,
+                                                 LoadS&& load_s, Hit&& hit) {
+  uint64_t mask = 0;
+#pragma unroll 1
+  for (uint32_t u = bb; u < be; u += 8) {
+    uint32_t off[8];
+    c1.offsets8(u, be - 1u, off);
+    float3 d[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) d[t] = load_s(off[t]);
+    uint32_t bits = 0;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) bits |= (hit(d[t]) ? 1u : 0u) << t;
+    const uint32_t valid = be - u >= 8u ? 0xFFu : (1u << (be - u)) - 1u;
+    mask |= (uint64_t)(bits & valid) << (u - bb);
+  }
+  return mask;
+}
+
 // Visits sequence positions [u0, u1) of a candidate sequence of m positions
 // in order,
 // calling add() only for candidates passing the cheap test hit() on their
@@ -425,19 +448,7 @@ __device__ __forceinline__ void sweep_seq(const SeqTable& st, uint32_t m, uint32
       uint64_t mask = 0;
       if (bb < u1 && !get_mask(bb >> 6, mask)) {
         const uint32_t be = u1 - bb < 64u ? u1 : bb + 64u;
-#pragma unroll 1
-        for (uint32_t u = bb; u < be; u += 8) {
-          uint32_t off[8];
-          c1.offsets8(u, be - 1u, off);
-          float3 d[8];
-#pragma unroll
-          for (int t = 0; t < 8; ++t) d[t] = load_s(off[t]);
-          uint32_t bits = 0;
-#pragma unroll
-          for (int t = 0; t < 8; ++t) bits |= (hit(d[t]) ? 1u : 0u) << t;
-          const uint32_t valid = be - u >= 8u ? 0xFFu : (1u << (be - u)) - 1u;
-          mask |= (uint64_t)(bits & valid) << (u - bb);
-        }
+        mask = screen_block(c1, bb, be, load_s, hit);
         put_mask(bb >> 6, mask);
       }
       w[q] = mask;

@@ -51,6 +51,7 @@ PREDICT_BYTES_PER_PARTICLE = 56.0
 VALU_WAVE_INSTS_PER_S = 256 * 2.4e9
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "lambda_pass_single_dam_1m_traffic.md")
 SQ_FILE = os.path.join(ROOT, "profiles", "lambda_pass_single_dam_1m_sq_counters.md")
+STEP_FILE = os.path.join(ROOT, "profiles", "step_kernels_single_dam_1m.md")
 
 
 def parse_args():
@@ -120,6 +121,17 @@ def measured_valu():
             "lane_utilisation": got["lane_utilisation"],
             "peak": "256 CUs x 64 FP32 lanes x 2.4 GHz (non-packed)",
             "source": os.path.relpath(SQ_FILE, ROOT)}
+
+
+def traced_predict_us():
+    """Mean duration of the predict kernel in the committed kernel-trace
+    summary of this benchmark's single dam (eager run), or None."""
+    try:
+        text = open(STEP_FILE).read()
+    except OSError:
+        return None
+    m = re.search(r"\| `k_predict_key<4, OneScene>` \| [0-9.]+ \| ([0-9.]+) \|", text)
+    return float(m.group(1)) if m else None
 
 
 def step_bytes(n, niter, cells):
@@ -674,7 +686,14 @@ def main():
                         "achieved": round(p_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": p_gbs / HBM_PEAK_GBS, "launch_us": round(p_ms * 1e3, 2),
                         "algorithmic_bytes_per_particle": PREDICT_BYTES_PER_PARTICLE,
-                        "launch_timing": "HIP events around each predict launch, eager run"}
+                        "launch_timing": "HIP events around each predict launch, eager run "
+                                         "(the event pair adds several us to a ~10 us kernel)"}
+    tp = traced_predict_us()
+    if tp:
+        t_gbs = PREDICT_BYTES_PER_PARTICLE * n / (tp * 1e-6) / 1e9
+        predict_roofline.update({"traced_us": tp, "traced_achieved": round(t_gbs, 1),
+                                 "traced_frac": t_gbs / HBM_PEAK_GBS,
+                                 "traced_source": os.path.relpath(STEP_FILE, ROOT)})
 
     # batched: 64 scenes of uneven size in one launch sequence, graph mode
     batched = None

@@ -352,7 +352,11 @@ __global__ __launch_bounds__(kBlock, PBF_PASS_WAVES) void k_light(typename P::Ar
     }
   }
   const Float4Buf sb = float4_buf(P::screen(a), (uint32_t)n);
-  sweep_seq<P::kHitBatch, kLightGroup>(
+  // a pass on stored hit masks screens nothing: its hits of up to four
+  // blocks run through one loop (kLightMax candidates), which balances the
+  // lanes best
+  constexpr int kGroup = MaskMode<P>::value == kMaskLoad ? (int)(kLightMax / kChunk) : kLightGroup;
+  sweep_seq<P::kHitBatch, kGroup>(
       sts, ms, 0u, ms, [&](uint32_t off) { return load_xyz_at(sb, off); },
       [&](uint32_t j) { return P::load_f(a, j); }, hit, add, block_end, get_mask, put_mask);
   P::finish(a, t, s, c, sc.at(ti, c));

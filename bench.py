@@ -243,23 +243,25 @@ def timed(run, steps, stream):
     return e0.elapsed_time(e1) / steps
 
 
-GRAD_NOTE = ("with the dam constants the exact derivative of the rollout grows by ~1e3-1e4 per "
-             "frame (Delta-p clamped at MAX_DP makes the map piecewise smooth); the FP64 "
-             "oracle's gradient of the 30-frame 16K rollout is ~1e110, beyond the FP32 range, so "
-             "non-finite gradients are expected there; with the reference constants the 30-frame "
-             "gradient is finite and is checked in tests/test_gpu_adjoint.py against the oracle's "
-             "adjoint of the same trajectory (1e-4) and, over 15 frames, against FP64 central "
-             "differences (1e-3)")
+GRAD_NOTE = ("tapes with PBF_TAPE_RESCALE: the adjoint is renormalised by a power of two before "
+             "each frame, the outputs are the true gradients x 2^-log2_scale; with the dam "
+             "constants the exact derivative of the rollout grows by ~1e3-1e4 per frame (the FP64 "
+             "oracle's 30-frame gradient is ~1e110, beyond the FP32 range), which the exponent "
+             "carries; with the reference constants the 30-frame gradient is checked in "
+             "tests/test_gpu_adjoint.py against the oracle's adjoint of the same trajectory (1e-4) "
+             "and, over 15 frames, against FP64 central differences (1e-3)")
 
 
 SWEEP_GRAD_NOTE = ("by frame 110 the moving wall has piled thousands of particles into a box "
                    "corner at distances just above KERNAL_EPS, where the spiky gradient's "
                    "derivative scales as 1/r: the derivative of the frame map grows by ~1e5 per "
                    "frame with the reference constants and ~1e7-1e8 with the dam constants, so "
-                   "through all 10 frames it exceeds the FP32 range (non-finite is expected); "
-                   "tests/test_gpu_sweep_adjoint.py checks finite gradients through the last 5 "
-                   "(reference) and 2 (dam) frames, both tapes, and the last frame against the "
-                   "oracle's adjoint at full size")
+                   "through all 10 frames it exceeds the FP32 range; the tapes use "
+                   "PBF_TAPE_RESCALE (a power-of-two renormalisation of the adjoint before each "
+                   "frame), so their outputs are finite: the true gradients x 2^-log2_scale; "
+                   "tests/test_gpu_sweep_adjoint.py checks the unscaled gradients through the "
+                   "last 5 (reference) and 2 (dam) frames, both tapes, the last frame against the "
+                   "oracle's adjoint at full size, and the rescaled pass through all 10 frames")
 
 
 def adjoint_ratio(pbf, p, dev, stream, reps=3, store=True):
@@ -280,7 +282,7 @@ def adjoint_ratio(pbf, p, dev, stream, reps=3, store=True):
     n, nf = sc.n, 30
     tgt = torch.from_numpy(scenes.target_cloud(sc)).to(dev)
     sim = pbf.Simulator(p, sc.ulim, sc.llim, capacity=n)
-    tape = pbf.Tape(sim, n, nf, store=store)
+    tape = pbf.Tape(sim, n, nf, store=store, rescale=True)
     pos, vel = sc.as_layout(4)
     pos0 = torch.from_numpy(pos.ravel()).to(dev)
     vel0 = torch.from_numpy(vel.ravel()).to(dev)
@@ -337,7 +339,8 @@ def adjoint_ratio(pbf, p, dev, stream, reps=3, store=True):
             "forward_ms": t_f, "backward_ms": t_b, "plain_forward_ms": t_p,
             "backward_over_plain_forward": t_b / t_p,
             "loss": float((loss[0] ** 2).sum().item() / n),
-            "grad_vel0_norm": float(gv.double().norm().item()), "grad_pho0": float(grho),
+            "grad_vel0_norm_scaled": float(gv.double().norm().item()),
+            "log2_scale": tape.gradient_log2_scale(), "grad_pho0": float(grho),
             "gradients_finite": finite}
 
 
@@ -470,7 +473,7 @@ def sweep_workload(pbf, dev, stream, frames=120, back=10):
         cur = b if which else a
         for store in (True, False):
             free0 = torch.cuda.mem_get_info()[0]
-            tape = pbf.Tape(sim, n, back, store=store)
+            tape = pbf.Tape(sim, n, back, store=store, rescale=True)
             tape_bytes = free0 - torch.cuda.mem_get_info()[0]
             ms_f, _ = span(lambda: tape.forward(cur[0], cur[1], iid, stream, first_frame=plain,
                                                 sweep=sw))
@@ -483,17 +486,23 @@ def sweep_workload(pbf, dev, stream, frames=120, back=10):
             gv0 = torch.empty(n * 4, device=dev)
             ms_b, grho = span(lambda: tape.backward(g.view(-1), None, gp0, gv0, stream))
             gv = gv0.view(n, 4)[:, :3].double()
+            e = tape.gradient_log2_scale()
             key = "stored" if store else "recompute"
+            gnorm = float(gv.norm().item())
             out[key] = {"tape_forward_ms": ms_f, "backward_ms": ms_b,
                         "backward_forward_ratio": ms_b / ms_f, "tape_device_bytes": int(tape_bytes),
                         "gradients_finite": bool(torch.isfinite(gv).all().item())
                         and bool(np.isfinite(grho)),
-                        "grad_vel_norm": float(gv.norm().item()), "grad_pho0": float(grho)}
+                        "grad_vel_norm_scaled": gnorm, "log2_scale": e,
+                        "grad_vel_norm_log10": (float(np.log10(gnorm)) + e * float(np.log10(2.0))
+                                                if gnorm > 0 else None),
+                        "grad_pho0": float(grho)}
+            gv_true = gv * 2.0 ** e
             if store:
-                gv_stored = gv.clone()
+                gv_stored = gv_true.clone()
             else:
                 out["recompute"]["rel_diff_to_stored_tape"] = float(
-                    ((gv - gv_stored).norm() / gv_stored.norm()).item())
+                    ((gv_true - gv_stored).norm() / gv_stored.norm()).item())
             tape.close()
             del tape, xf, iidf, g, gp0, gv0
     out["backward_forward_ratio"] = out["recompute"]["backward_forward_ratio"]

@@ -313,6 +313,14 @@ int pbf_autotune(pbf_solver* s, const float* pos, const float* vel, const uint32
  *   run to run).  Without either flag, tapes of 2^18 particles or more
  *   gather when their grids allow it and smaller tapes scatter (there the
  *   per-pass sort of the gather costs more than the atomics).
+ *   PBF_TAPE_RESCALE: before the backward pass of each frame the adjoint
+ *   state is multiplied by a power of two that brings its largest finite
+ *   magnitude into [1, 2), so gradients that grow by many orders of
+ *   magnitude over the frames (chaotic flows, piles against a wall) stay in
+ *   FP32 range; gpos0 and gvel0 then hold the true gradients x 2^-e with e
+ *   from pbf_tape_gradient_scale (exact powers of two: the same bits as an
+ *   unscaled pass wherever that one stays finite), and *grho0 is the true
+ *   dL/dpho0 (accumulated in FP64).
  * pbf_tape_forward: runs the frames from (pos, vel, iid) (caller layout,
  *   not modified); the final state is pbf_tape_state(t, nframes).  With a
  *   sweep, frame k runs in the box pbf_sweep_box(sweep, first_frame + k)
@@ -332,9 +340,13 @@ typedef struct pbf_tape pbf_tape;
 #define PBF_TAPE_STORE 1
 #define PBF_TAPE_GATHER 2
 #define PBF_TAPE_SCATTER 4
+#define PBF_TAPE_RESCALE 8
 int pbf_tape_create(pbf_solver* s, int n, int nframes, pbf_tape** out);
 int pbf_tape_create_ex(pbf_solver* s, int n, int nframes, int flags, pbf_tape** out);
 void pbf_tape_destroy(pbf_tape* t);
+/* After pbf_tape_backward on a PBF_TAPE_RESCALE tape: the gradients it wrote
+ * are the true ones x 2^-log2_scale (0 for other tapes). */
+int pbf_tape_gradient_scale(const pbf_tape* t, int* log2_scale);
 int pbf_tape_forward(pbf_solver* s, pbf_tape* t, const float* pos, const float* vel,
                      const uint32_t* iid, int first_frame, const pbf_sweep* sweep,
                      void* stream);

@@ -583,7 +583,7 @@ __global__ __launch_bounds__(kBlock) void k_bwd_clamp(
     int n, StepConsts c, const float4* __restrict__ gx, const float4* __restrict__ xl,
     const float4* __restrict__ dp, float4* __restrict__ gxn, float4* __restrict__ A,
     uint32_t* __restrict__ qc, uint32_t* __restrict__ qkey, uint32_t* __restrict__ qval,
-    double* __restrict__ r0) {
+    double* __restrict__ r0, const int* __restrict__ escale) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   const float4 g = gx[i], x = xl[i], u = dp[i];
@@ -601,7 +601,7 @@ __global__ __launch_bounds__(kBlock) void k_bwd_clamp(
   const uint32_t q = note_query_cell(i, x, c, qc, qkey, qval);
   gxn[i] = make_float4(e[0], e[1], e[2], 0.0f);
   A[i] = make_float4(a[0], a[1], a[2], __uint_as_float(q));
-  r0[i] += rr;
+  r0[i] += escale ? ldexp(rr, *escale) : rr;
 }
 
 // Query cells of the final positions (for the XSPH pass).
@@ -680,7 +680,7 @@ __global__ __launch_bounds__(kBlock) void k_bwd_lambda_pre(
     const float4* __restrict__ out, float4* __restrict__ ex, const float4* __restrict__ xl,
     const float4* __restrict__ sd, const float* __restrict__ rho,
     const float* __restrict__ grho_x, float4* __restrict__ gxn, float4* __restrict__ xr,
-    float4* __restrict__ sdb, double* __restrict__ r0) {
+    float4* __restrict__ sdb, double* __restrict__ r0, const int* __restrict__ escale) {
   const int u = blockIdx.x * kBlock + threadIdx.x;
   if (u >= n) return;
   const int i = kGather ? (int)perm[u] : u;
@@ -694,8 +694,9 @@ __global__ __launch_bounds__(kBlock) void k_bwd_lambda_pre(
   const float gD = gl * C / (D * D);
   float gr = gC / c.pho0;
   if (last) gr += grho_x[i];
-  r0[i] += (double)gC * (-(double)rh / ((double)c.pho0 * (double)c.pho0)) +
-           (double)gD * (-2.0 * ((double)D - (double)c.lambda_eps) / (double)c.pho0);
+  const double dr0 = (double)gC * (-(double)rh / ((double)c.pho0 * (double)c.pho0)) +
+                     (double)gD * (-2.0 * ((double)D - (double)c.lambda_eps) / (double)c.pho0);
+  r0[i] += escale ? ldexp(dr0, *escale) : dr0;
   float4 g = gxn[i];
   g.x += o.x + e.x;
   g.y += o.y + e.y;
@@ -721,6 +722,45 @@ __global__ __launch_bounds__(kBlock) void k_bwd_fold(int n, const float4* __rest
   gx[i] = make_float4(g.x + e.x, g.y + e.y, g.z + e.z, 0.0f);
 }
 
+// Per-frame rescaling of the adjoint state (tapes with PBF_TAPE_RESCALE):
+// the largest finite |value| of two arrays, as float bits (all values are
+// non-negative, so the unsigned order of the bits is the float order) ...
+__global__ __launch_bounds__(kBlock) void k_absmax(const float* __restrict__ a, int na,
+                                                   const float* __restrict__ b, int nb,
+                                                   uint32_t* __restrict__ out) {
+  uint32_t m = 0u;
+  const int stride = gridDim.x * kBlock;
+  for (int k = blockIdx.x * kBlock + threadIdx.x; k < na + nb; k += stride) {
+    const float v = fabsf(k < na ? a[k] : b[k - na]);
+    const uint32_t bits = __float_as_uint(v);
+    m = (v <= 3.4028235e38f && bits > m) ? bits : m;  // NaN and inf are left out
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const uint32_t o = __shfl_xor(m, off);
+    m = o > m ? o : m;
+  }
+  if ((threadIdx.x & 63) == 0 && m) atomicMax(out, m);
+}
+
+// ... and both arrays multiplied by 2^e (exact) so that the largest becomes
+// [1, 2); escale -= e (true values = stored x 2^escale).  Every workgroup
+// derives e from the same maximum; workgroup 0 updates escale.
+__global__ __launch_bounds__(kBlock) void k_rescale(float* __restrict__ a, int na,
+                                                    float* __restrict__ b, int nb,
+                                                    const uint32_t* __restrict__ maxbits,
+                                                    int* __restrict__ escale) {
+  const float mx = __uint_as_float(*maxbits);
+  const int e = mx > 0.0f ? -ilogbf(mx) : 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *escale -= e;
+  if (e == 0) return;
+  const int stride = gridDim.x * kBlock;
+  for (int k = blockIdx.x * kBlock + threadIdx.x; k < na + nb; k += stride) {
+    float* p = k < na ? a + k : b + (k - na);
+    *p = ldexpf(*p, e);
+  }
+}
+
 // Prediction x[0] = x + dt (v + dt g) (fluids/Simulator_kernel.cuh:14-15),
 // back to the frame's input order through the sort permutation.
 template <int L>
@@ -736,6 +776,10 @@ __global__ __launch_bounds__(kBlock) void k_bwd_predict(int n, StepConsts c,
   const float4 g = gx[s], g0 = gx0[s];
   store_vec<L>(gpos, src, g0.x + g.x, g0.y + g.y, g0.z + g.z);
   store_vec<L>(gvel, src, c.dt * g.x, c.dt * g.y, c.dt * g.z);
+}
+
+__global__ void k_fill_u32_one(uint32_t* __restrict__ a) {
+  if (threadIdx.x == 0) *a = 0u;
 }
 
 __global__ __launch_bounds__(kBlock) void k_fill_f4(float4* __restrict__ a, int n) {
@@ -851,7 +895,7 @@ void launch_bwd_iteration(const BwdFrame& f, const BwdIter& it, int last, int n,
   if (f.q) {
     const BwdQuery& q = *f.q;
     k_bwd_clamp<<<blocks_for(n), kBlock, 0, st>>>(n, c, f.gx, it.xl, it.dp, f.gxn, f.A, nullptr,
-                                                  q.k0, q.v0, f.r0);
+                                                  q.k0, q.v0, f.r0, f.escale);
     const auto kv = sort_query(q, n, st);
     k_bwd_gather_iter<<<blocks_for(n), kBlock, 0, st>>>(n, kv.first, kv.second, it.xl, f.A, q.pc,
                                                         q.xq, q.aq, q.cq, q.cnt);
@@ -861,7 +905,8 @@ void launch_bwd_iteration(const BwdFrame& f, const BwdIter& it, int last, int n,
                     Scenes{}, q.cs, hl, pass, sh, st);
     k_bwd_lambda_pre<true><<<blocks_for(n), kBlock, 0, st>>>(n, c, last, kv.second, f.out_x,
                                                              nullptr, it.xl, it.sd, it.rho,
-                                                             f.grho_x, f.gxn, q.xr, q.sdb, f.r0);
+                                                             f.grho_x, f.gxn, q.xr, q.sdb, f.r0,
+                                                             f.escale);
     using PL = BwdLambdaPass<true>;
     launch_pass<PL>(PL::Args{q.xr, q.sdb, nullptr, f.gxn, f.gx, nullptr, (uint32_t)n, nullptr,
                              q.cq, kv.second},
@@ -869,13 +914,13 @@ void launch_bwd_iteration(const BwdFrame& f, const BwdIter& it, int last, int n,
     return;
   }
   k_bwd_clamp<<<blocks_for(n), kBlock, 0, st>>>(n, c, f.gx, it.xl, it.dp, f.gxn, f.A, f.qc,
-                                                nullptr, nullptr, f.r0);
+                                                nullptr, nullptr, f.r0, f.escale);
   using PD = BwdDeltaPass<false>;
   launch_pass<PD>(PD::Args{it.xl, f.A, f.key, f.out_x, f.ex_x, (uint32_t)n, nullptr}, n, c,
                   Scenes{}, cs, hl, pass, sh, st);
   k_bwd_lambda_pre<false><<<blocks_for(n), kBlock, 0, st>>>(n, c, last, nullptr, f.out_x, f.ex_x,
                                                             it.xl, it.sd, it.rho, f.grho_x, f.gxn,
-                                                            f.xr, f.sdb, f.r0);
+                                                            f.xr, f.sdb, f.r0, f.escale);
   using PL = BwdLambdaPass<false>;
   launch_pass<PL>(PL::Args{f.xr, f.sdb, f.key, f.gxn, f.gxn, f.ex_x, (uint32_t)n, f.qc, nullptr,
                            nullptr},
@@ -889,6 +934,14 @@ void launch_bwd_predict(int layout, const BwdFrame& f, const uint32_t* sval, int
     k_bwd_predict<3><<<blocks_for(n), kBlock, 0, st>>>(n, c, sval, f.gx, f.gx0, gpos, gvel);
   else
     k_bwd_predict<4><<<blocks_for(n), kBlock, 0, st>>>(n, c, sval, f.gx, f.gx0, gpos, gvel);
+}
+
+void launch_rescale(float* a, int na, float* b, int nb, uint32_t* maxbits, int* escale,
+                    hipStream_t st) {
+  k_fill_u32_one<<<1, 64, 0, st>>>(maxbits);
+  const int blocks = fill_blocks(na + nb);
+  k_absmax<<<blocks, kBlock, 0, st>>>(a, na, b, nb, maxbits);
+  k_rescale<<<blocks, kBlock, 0, st>>>(a, na, b, nb, maxbits, escale);
 }
 
 void launch_fill_f4(float4* a, int n, hipStream_t st) {

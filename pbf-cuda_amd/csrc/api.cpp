@@ -175,6 +175,12 @@ struct pbf_tape {
   // buffers and scratch, the packed sorted cells; its query-order arrays
   // live in the memory of ex_x and ex_v, which scatter mode needs all zero
   int adjoint_flag = 0;             // PBF_TAPE_GATHER, PBF_TAPE_SCATTER or 0 (automatic)
+  // PBF_TAPE_RESCALE: the adjoint state is renormalised by a power of two
+  // before each frame's backward; the outputs are true values x 2^log2_scale
+  bool rescale = false;
+  uint32_t* maxbits = nullptr;      // scratch word of the renormalisation
+  int* escale = nullptr;            // device: the exponent of the current scale
+  int log2_scale = 0;               // host: the exponent of the last backward's outputs
   bool ex_dirty = false;            // ex_x / ex_v hold gather-mode data
   uint32_t *qk0 = nullptr, *qv0 = nullptr, *qk1 = nullptr, *qv1 = nullptr, *qhist = nullptr,
            *qtotals = nullptr, *pc = nullptr;
@@ -1334,7 +1340,7 @@ int pbf_tape_create_ex(pbf_solver* s, int n, int nframes, int flags, pbf_tape** 
                 n);
   if (nframes < 1 || nframes > 100000)
     return fail(s, PBF_ERR_INVALID_ARGUMENT, "nframes must be in [1, 100000], got %d", nframes);
-  if (flags & ~(PBF_TAPE_STORE | PBF_TAPE_GATHER | PBF_TAPE_SCATTER))
+  if (flags & ~(PBF_TAPE_STORE | PBF_TAPE_GATHER | PBF_TAPE_SCATTER | PBF_TAPE_RESCALE))
     return fail(s, PBF_ERR_INVALID_ARGUMENT, "unknown tape flags %d", flags);
   if ((flags & PBF_TAPE_GATHER) && (flags & PBF_TAPE_SCATTER))
     return fail(s, PBF_ERR_INVALID_ARGUMENT, "PBF_TAPE_GATHER and PBF_TAPE_SCATTER exclude each other");
@@ -1349,6 +1355,7 @@ int pbf_tape_create_ex(pbf_solver* s, int n, int nframes, int flags, pbf_tape** 
   t->layout = s->layout;
   t->store = (flags & PBF_TAPE_STORE) != 0;
   t->adjoint_flag = flags & (PBF_TAPE_GATHER | PBF_TAPE_SCATTER);
+  t->rescale = (flags & PBF_TAPE_RESCALE) != 0;
   const size_t N = (size_t)n, L = (size_t)s->layout, NI = (size_t)t->niter;
   size_t off = 0;
   auto carve = [&](size_t bytes) {
@@ -1376,7 +1383,7 @@ int pbf_tape_create_ex(pbf_solver* s, int n, int nframes, int flags, pbf_tape** 
     o_f4[q] = carve(16 * N * (q == 2 || q == 3 ? (size_t)pbf::kScatterCopies : 1));
   const size_t o_qk0 = carve(4 * N), o_qv0 = carve(4 * N), o_qk1 = carve(4 * N),
                o_qv1 = carve(4 * N), o_qhist = carve(4 * 256 * (size_t)pbf::radix_tiles(n)),
-               o_qtot = carve(4 * 256), o_pc = carve(4 * N);
+               o_qtot = carve(4 * 256), o_pc = carve(4 * N), o_scale = carve(8);
   const size_t o_grx = carve(4 * N), o_qc = carve(4 * N), o_gp0 = carve(4 * L * N), o_gp1 = carve(4 * L * N),
                o_gv0 = carve(4 * L * N), o_gv1 = carve(4 * L * N), o_r0 = carve(8 * N),
                o_r0s = carve(8 * (1 + 1024));
@@ -1431,6 +1438,8 @@ int pbf_tape_create_ex(pbf_solver* s, int n, int nframes, int flags, pbf_tape** 
   t->qhist = (uint32_t*)(b + o_qhist);
   t->qtotals = (uint32_t*)(b + o_qtot);
   t->pc = (uint32_t*)(b + o_pc);
+  t->maxbits = (uint32_t*)(b + o_scale);
+  t->escale = (int*)(b + o_scale + 4);
   t->gp[0] = (float*)(b + o_gp0);
   t->gp[1] = (float*)(b + o_gp1);
   t->gv[0] = (float*)(b + o_gv0);
@@ -1459,6 +1468,12 @@ int pbf_tape_create_ex(pbf_solver* s, int n, int nframes, int flags, pbf_tape** 
     return fail_hip(s, e, "clearing the tape");
   }
   *out = t;
+  return PBF_OK;
+}
+
+int pbf_tape_gradient_scale(const pbf_tape* t, int* log2_scale) {
+  if (!t || !log2_scale) return fail(nullptr, PBF_ERR_INVALID_ARGUMENT, "NULL argument");
+  *log2_scale = t->log2_scale;
   return PBF_OK;
 }
 
@@ -1614,6 +1629,10 @@ int pbf_tape_backward(pbf_solver* s, pbf_tape* t, const float* gpos, const float
              : hipMemsetAsync(t->gv[0], 0, vb, st);
   if (e != hipSuccess) return fail_hip(s, e, "seeding the backward pass");
   pbf::launch_fill_f64(t->r0, n, st);
+  if (t->rescale) {
+    pbf::launch_fill_u32(reinterpret_cast<uint32_t*>(t->escale), 1, 0u, st);
+    pbf::launch_rescale(t->gp[0], n * L, t->gv[0], n * L, t->maxbits, t->escale, st);
+  }
   // Adjoint mode: gather (each pass over a grid of its own positions, no
   // atomics) from kGatherMinParticles particles or on request, when every
   // frame's grid has at most 1024 cells per axis; else scatter.
@@ -1737,6 +1756,7 @@ int pbf_tape_backward(pbf_solver* s, pbf_tape* t, const float* gpos, const float
     bf.grho_x = t->grho_x;
     bf.qc = t->qc;
     bf.q = gather ? &bq : nullptr;
+    bf.escale = t->rescale ? t->escale : nullptr;
     bq.key_bits = key_bits_for(c.cells);
     pbf::launch_bwd_frame(bf, n, c, st);
     bf.r0 = t->r0;
@@ -1749,15 +1769,20 @@ int pbf_tape_backward(pbf_solver* s, pbf_tape* t, const float* gpos, const float
     float* op = f == 0 ? gpos0 : t->gp[cur ^ 1];
     float* ov = f == 0 ? gvel0 : t->gv[cur ^ 1];
     pbf::launch_bwd_predict(L, bf, sval, n, c, op, ov, st);
+    if (t->rescale && f > 0) pbf::launch_rescale(op, n * L, ov, n * L, t->maxbits, t->escale, st);
     cur ^= 1;
     if ((e = hipGetLastError()) != hipSuccess) return fail_hip(s, e, "kernel launch");
     s->cnt_dirty = false;
   }
   pbf::launch_sum_f64(t->r0, n, t->r0_sum + 1, t->r0_sum, st);
   double host = 0.0;
-  if ((e = hipMemcpyAsync(&host, t->r0_sum, sizeof(double), hipMemcpyDeviceToHost, st)) != hipSuccess)
+  int escale = 0;
+  if ((e = hipMemcpyAsync(&host, t->r0_sum, sizeof(double), hipMemcpyDeviceToHost, st)) != hipSuccess ||
+      (t->rescale &&
+       (e = hipMemcpyAsync(&escale, t->escale, sizeof(int), hipMemcpyDeviceToHost, st)) != hipSuccess))
     return fail_hip(s, e, "hipMemcpyAsync");
   if ((e = hipStreamSynchronize(st)) != hipSuccess) return fail_hip(s, e, "hipStreamSynchronize");
+  t->log2_scale = escale;
   if (grho0) *grho0 = host;
   return PBF_OK;
 }

@@ -54,6 +54,8 @@ struct BwdFrame {
   float* grho_x;          // dL/drho of the last lambda pass, from XSPH
   double* r0;             // per-slot dL/dpho0 accumulators
   const BwdQuery* q;      // gather mode's arrays; null: scatter mode
+  const int* escale;      // rescaled adjoints: true = stored x 2^*escale (r0 gets true
+                          // values); null: no rescaling
 };
 
 // One solver iteration's record.
@@ -80,6 +82,11 @@ void launch_bwd_iteration(const BwdFrame& f, const BwdIter& it, int last, int n,
 void launch_bwd_predict(int layout, const BwdFrame& f, const uint32_t* sval, int n,
                         const StepConsts& c, float* gpos, float* gvel, hipStream_t st);
 
+// Rescales the adjoint arrays a (na floats) and b (nb floats) by a power of
+// two so that their largest finite |value| lies in [1, 2), and subtracts the
+// exponent applied from *escale (maxbits: one device word of scratch).
+void launch_rescale(float* a, int na, float* b, int nb, uint32_t* maxbits, int* escale,
+                    hipStream_t st);
 void launch_fill_f4(float4* a, int n, hipStream_t st);
 void launch_fill_f64(double* a, int n, hipStream_t st);
 // Caller-layout vector array -> float4.

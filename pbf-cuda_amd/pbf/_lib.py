@@ -118,6 +118,7 @@ _SIGNATURES = [
     ("pbf_tape_create_ex", ctypes.c_int,
      [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_P)]),
     ("pbf_tape_destroy", None, [_P]),
+    ("pbf_tape_gradient_scale", ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int)]),
     ("pbf_tape_forward", ctypes.c_int,
      [_P, _P, _P, _P, _P, ctypes.c_int, ctypes.POINTER(PbfSweep), _P]),
     ("pbf_tape_state", ctypes.c_int,

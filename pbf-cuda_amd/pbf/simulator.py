@@ -257,6 +257,7 @@ class Simulator:
 TAPE_STORE = 1
 TAPE_GATHER = 2
 TAPE_SCATTER = 4
+TAPE_RESCALE = 8
 
 
 class Tape:
@@ -266,10 +267,12 @@ class Tape:
     PBF_TAPE_STORE), so the backward pass recomputes nothing; the gradients
     are the same bits either way.  adjoint: None (automatic), "gather" or
     "scatter" (PBF_TAPE_GATHER / PBF_TAPE_SCATTER: how the backward pass
-    collects the pair terms the stale grid makes one-sided)."""
+    collects the pair terms the stale grid makes one-sided).  rescale=True
+    (PBF_TAPE_RESCALE) renormalises the adjoint by a power of two before each
+    frame; backward() then fills the true gradients x 2^-gradient_log2_scale()."""
 
     def __init__(self, sim: "Simulator", nparticle: int, nframes: int, store: bool = False,
-                 adjoint: Optional[str] = None):
+                 adjoint: Optional[str] = None, rescale: bool = False):
         self._lib = _lib.load()
         self.sim = sim
         self.n = int(nparticle)
@@ -281,8 +284,10 @@ class Tape:
         self.adjoint = adjoint
         self._t = ctypes.c_void_p()
         h = sim._handle()
-        _lib.check(self._lib.pbf_tape_create_ex(h, self.n, self.nframes,
-                                                (TAPE_STORE if self.store else 0) | modes[adjoint],
+        self.rescale = bool(rescale)
+        flags = ((TAPE_STORE if self.store else 0) | modes[adjoint]
+                 | (TAPE_RESCALE if self.rescale else 0))
+        _lib.check(self._lib.pbf_tape_create_ex(h, self.n, self.nframes, flags,
                                                 ctypes.byref(self._t)), h)
 
     def close(self) -> None:
@@ -329,6 +334,13 @@ class Tape:
             _ptr(gpos0, L, "gpos0", n, True), _ptr(gvel0, L, "gvel0", n, True), ctypes.byref(g),
             _stream_handle(stream)), h)
         return float(g.value)
+
+    def gradient_log2_scale(self) -> int:
+        """e such that the last backward() wrote the true gradients x 2^-e
+        (0 unless rescale=True)."""
+        e = ctypes.c_int(0)
+        _lib.check(self._lib.pbf_tape_gradient_scale(self._t, ctypes.byref(e)))
+        return int(e.value)
 
 
 class BatchedSimulator(Simulator):

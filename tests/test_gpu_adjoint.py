@@ -34,12 +34,14 @@ def _target(sc, seed=1):
     return sc.pos.astype(np.float64) + rng.normal(0, 0.05, (sc.n, 3))
 
 
-def gpu_grad(pbf_mod, sc, p, nframes, target, layout=4, store=False, sweep=None, adjoint=None):
+def gpu_grad(pbf_mod, sc, p, nframes, target, layout=4, store=False, sweep=None, adjoint=None,
+             rescale=False, scale_out=None):
     """Loss gradients from the C ABI: (g_pos0, g_vel0 as N x 3 in input order,
-    g_rho0, final positions by identity)."""
+    g_rho0, final positions by identity).  With rescale, the gradients as the
+    tape wrote them (true x 2^-e) and e appended to scale_out."""
     n = sc.n
     sim = pbf_mod.Simulator(p, sc.ulim, sc.llim, capacity=n, layout=layout)
-    tape = pbf_mod.Tape(sim, n, nframes, store=store, adjoint=adjoint)
+    tape = pbf_mod.Tape(sim, n, nframes, store=store, adjoint=adjoint, rescale=rescale)
     pos, vel = sc.as_layout(layout)
     dev = "cuda:0"
     tape.forward(to_dev(pos, dev), to_dev(vel, dev), to_dev(sc.iid, dev), sweep=sweep)
@@ -54,6 +56,8 @@ def gpu_grad(pbf_mod, sc, p, nframes, target, layout=4, store=False, sweep=None,
     gp0 = torch.zeros(n * layout, device=dev)
     gv0 = torch.zeros(n * layout, device=dev)
     grho = tape.backward(to_dev(g, dev), None, gp0, gv0)
+    if scale_out is not None:
+        scale_out.append(tape.gradient_log2_scale())
     x_by_iid = np.zeros((n, 3))
     x_by_iid[I] = X
     return (from_dev(gp0).reshape(n, layout)[:, :3].astype(np.float64),
@@ -283,7 +287,7 @@ def test_tape_rejects_unknown_flags(pbf_mod, device):
     sim = pbf_mod.Simulator(p, sc.ulim, sc.llim, capacity=sc.n)
     lib = _lib.load()
     t = ctypes.c_void_p()
-    assert lib.pbf_tape_create_ex(sim._handle(), sc.n, 2, 8, ctypes.byref(t)) == 1
+    assert lib.pbf_tape_create_ex(sim._handle(), sc.n, 2, 16, ctypes.byref(t)) == 1
     assert b"flags" in lib.pbf_last_error(sim._handle())
     assert lib.pbf_tape_create_ex(sim._handle(), sc.n, 2, 2 | 4, ctypes.byref(t)) == 1
     assert b"exclude" in lib.pbf_last_error(sim._handle())
@@ -419,3 +423,48 @@ def test_c4_dam_constants_adjoint_matches_oracle(pbf_mod, orc, device, nframes):
     assert rel(gv, o["g_vel0"]) < 1e-4, rel(gv, o["g_vel0"])
     assert rel(gp, o["g_pos0"]) < 1e-4, rel(gp, o["g_pos0"])
     assert abs(gr - o["g_rho0"]) < 1e-4 * abs(o["g_rho0"]), (gr, o["g_rho0"])
+
+
+@pytest.mark.parametrize("adjoint", ["gather", "scatter"])
+def test_rescaled_backward_equals_unscaled(pbf_mod, device, adjoint):
+    """PBF_TAPE_RESCALE multiplies the adjoint state by powers of two between
+    frames: its gradients times 2^e equal the unscaled ones (scaling by a
+    power of two is exact, and the adjoint is linear: in gather mode, whose
+    sums do not depend on timing, they agree to rounding of values that reach
+    the subnormal range in one of the two passes only; in scatter mode within
+    the atomics' run-to-run spread), and dL/dpho0 is the same value."""
+    p = P.reference_defaults()
+    sc = scenes.block((8, 8, 8), (1.0, 1.0, 1.0), seed=9, vel_sigma=3.0)
+    tgt = _target(sc)
+    e = []
+    a = gpu_grad(pbf_mod, sc, p, 4, tgt, adjoint=adjoint)
+    b = gpu_grad(pbf_mod, sc, p, 4, tgt, adjoint=adjoint, rescale=True, scale_out=e)
+    print(f"{adjoint}: log2 scale {e[0]}", flush=True)
+    assert e[0] != 0
+    gp, gv = np.ldexp(b[0], e[0]), np.ldexp(b[1], e[0])
+    if adjoint == "gather":
+        assert rel(gp, a[0]) < 1e-7 and rel(gv, a[1]) < 1e-7
+        assert b[2] == pytest.approx(a[2], rel=1e-9)
+    else:
+        assert rel(gp, a[0]) < 1e-5 and rel(gv, a[1]) < 1e-5
+        assert b[2] == pytest.approx(a[2], rel=1e-5)
+
+
+def test_rescaled_c4_dam_constants_gradient_is_finite(pbf_mod, device):
+    """The dam constants' 30-frame gradient of the 16K workload exceeds the
+    FP32 range (~1e110 in FP64); with PBF_TAPE_RESCALE the tape's outputs
+    stay finite, with the exponent carrying the magnitude, and over the
+    first frame (where the unscaled gradient is finite) they equal the
+    unscaled ones times 2^e."""
+    p = P.dam_params()
+    sc, tgt = _c4()
+    e30 = []
+    gp, gv, gr, _ = gpu_grad(pbf_mod, sc, p, 30, tgt, rescale=True, scale_out=e30)
+    print(f"c4 dam constants, 30 frames: log2 scale {e30[0]}, |g_v| scaled "
+          f"{np.linalg.norm(gv):.3g}, dL/dpho0 {gr:.3g}", flush=True)
+    assert np.all(np.isfinite(gv)) and np.all(np.isfinite(gp)) and np.isfinite(gr)
+    assert np.linalg.norm(gv) > 0 and e30[0] > 100
+    e1 = []
+    a = gpu_grad(pbf_mod, sc, p, 1, tgt)
+    b = gpu_grad(pbf_mod, sc, p, 1, tgt, rescale=True, scale_out=e1)
+    assert rel(np.ldexp(b[1], e1[0]), a[1]) < 1e-5

@@ -50,13 +50,14 @@ def _state_at(pbf_mod, p, frame):
     return sc, sim, sw, cur[0], cur[1], iid
 
 
-def _grads(pbf_mod, sim, sw, sc, pos, vel, iid, first, nframes, store):
+def _grads(pbf_mod, sim, sw, sc, pos, vel, iid, first, nframes, store, rescale=False,
+           scale_out=None):
     """dL/dv, dL/dx of the state entering `first` (N x 3, that state's order),
     dL/dpho0 and the final positions by identity, L = mean squared distance
     of the final positions to the seeded target cloud."""
     n = sc.n
     tgt = torch.from_numpy(scenes.target_cloud(sc)).to(DEV)
-    tape = pbf_mod.Tape(sim, n, nframes, store=store)
+    tape = pbf_mod.Tape(sim, n, nframes, store=store, rescale=rescale)
     tape.forward(pos, vel, iid, first_frame=first, sweep=sw)
     xf = torch.empty(n * 4, device=DEV)
     iidf = torch.empty(n, dtype=torch.int32, device=DEV)
@@ -66,6 +67,8 @@ def _grads(pbf_mod, sim, sw, sc, pos, vel, iid, first, nframes, store):
     gp0 = torch.empty(n * 4, device=DEV)
     gv0 = torch.empty(n * 4, device=DEV)
     grho = tape.backward(g.view(-1), None, gp0, gv0)
+    if scale_out is not None:
+        scale_out.append(tape.gradient_log2_scale())
     tape.close()
     X = from_dev(xf).reshape(n, 4)[:, :3]
     by_iid = np.zeros((n, 3), np.float32)
@@ -114,3 +117,27 @@ def test_sweep_last_frame_adjoint_matches_oracle(pbf_mod, orc, pname):
     assert rel(gv, o["g_vel0"]) < 1e-4, rel(gv, o["g_vel0"])
     assert rel(gp, o["g_pos0"]) < 1e-4, rel(gp, o["g_pos0"])
     assert abs(gr - o["g_rho0"]) <= 1e-4 * abs(o["g_rho0"]), (gr, o["g_rho0"])
+
+
+def test_sweep_rescaled_backward_through_all_ten_frames(pbf_mod):
+    """With PBF_TAPE_RESCALE the backward pass through all 10 taped frames
+    (dam constants) gives finite outputs, the magnitude carried by the
+    exponent (the unscaled pass overflows FP32 there), and through the last
+    2 frames the rescaled outputs times 2^e equal the unscaled ones (gather
+    mode; to rounding of subnormal values)."""
+    p = P.dam_params().replace(niter=6)
+    print("sweep 4M, dam constants: rescaled backward through frames 110..119", flush=True)
+    sc, sim, sw, pos, vel, iid = _state_at(pbf_mod, p, 110)
+    e10 = []
+    gv, gp, gr, _ = _grads(pbf_mod, sim, sw, sc, pos, vel, iid, 110, 10, False, True, e10)
+    print(f"log2 scale {e10[0]}, |g_v| scaled {np.linalg.norm(gv):.3g}, dL/dpho0 {gr:.3g}",
+          flush=True)
+    assert np.all(np.isfinite(gv)) and np.all(np.isfinite(gp)) and np.isfinite(gr)
+    assert np.linalg.norm(gv) > 0 and e10[0] > 60
+    sc, sim, sw, pos, vel, iid = _state_at(pbf_mod, p, 118)
+    e2 = []
+    a = _grads(pbf_mod, sim, sw, sc, pos, vel, iid, 118, 2, False)
+    b = _grads(pbf_mod, sim, sw, sc, pos, vel, iid, 118, 2, False, True, e2)
+    assert rel(np.ldexp(b[0], e2[0]), a[0]) < 1e-7
+    assert rel(np.ldexp(b[1], e2[0]), a[1]) < 1e-7
+    assert b[2] == pytest.approx(a[2], rel=1e-9)

@@ -425,6 +425,10 @@ struct Record {
   float4** dp = nullptr;
   const uint32_t* skey = nullptr;
   const uint32_t* sval = nullptr;
+  // a backward pass's recompute of the frame: the XSPH pass is left out (its
+  // outputs, the next frame's state, come from the checkpoints), so the last
+  // neighbour pass run is the last delta-p pass
+  bool skip_xsph = false;
 };
 
 // Caller arrays of one step.
@@ -515,9 +519,11 @@ void enqueue_step(pbf_solver* s, const StepArrays& a, const pbf::StepConsts& c, 
     te(PBF_PHASE_DELTA);
   }
   // correctVelocity (Simulator.cpp:77, Simulator.cu:251-265)
-  tb(PBF_PHASE_XSPH);
-  pbf::launch_xsph(L, s->xa, s->vr, s->cs, n, c, scn, a.npos, a.nvel, s->hl, pass, sh, st);
-  te(PBF_PHASE_XSPH);
+  if (!(rec && rec->skip_xsph)) {
+    tb(PBF_PHASE_XSPH);
+    pbf::launch_xsph(L, s->xa, s->vr, s->cs, n, c, scn, a.npos, a.nvel, s->hl, pass, sh, st);
+    te(PBF_PHASE_XSPH);
+  }
   te(PBF_PHASE_STEP);
 }
 
@@ -1687,14 +1693,16 @@ int pbf_tape_backward(pbf_solver* s, pbf_tape* t, const float* gpos, const float
   std::vector<float4*> xl(t->xl), sd(t->sd), dp(t->dp);
   std::vector<float*> rho(t->rho);
   Record rec;
+  rec.skip_xsph = true;
   rec.xl = xl.data();
   rec.sd = sd.data();
   rec.dp = dp.data();
   rec.rho = rho.data();
   int cur = 0;
   // Neighbour passes of the adjoint kernels: after a recompute they follow
-  // its passes 0 .. 2 niter; a stored tape numbers them on from the forward
-  // run's last pass (any step leaves the odd parity's counters cleared).
+  // its passes 0 .. 2 niter - 1 (it leaves out XSPH); a stored tape numbers
+  // them on from the forward run's last pass (any full step leaves the odd
+  // parity's counters cleared).
   int pass = 2 * NI + 1;
   for (int f = t->nframes - 1; f >= 0; --f) {
     // frame f under its own box
@@ -1732,7 +1740,9 @@ int pbf_tape_backward(pbf_solver* s, pbf_tape* t, const float* gpos, const float
         iters[k] = pbf::BwdIter{t->xl[k], t->sd[k], t->rho[k], t->dp[k]};
       skey = rec.skey;
       sval = rec.sval;
-      pass = 2 * NI + 1;
+      // the recompute's last neighbour pass was the last delta-p pass (2 NI -
+      // 1), whose light kernel cleared the counters of pass 2 NI
+      pass = 2 * NI;
     }
     // adjoint of the frame
     pbf::launch_to_f4(L, t->gp[cur], n, t->gxo, st);

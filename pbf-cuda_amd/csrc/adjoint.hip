@@ -11,16 +11,18 @@
 // dp_i = sum_{j!=i} (lambda_i + lambda_j + s_corr) spikyGrad / pho0, clamped;
 // XSPH v'_i = v_i + c sum_j 2 (v_j - v_i) W_ij / (rho_i + rho_j).
 //
-// Every backward neighbour pass is a pass policy of pbf_pass.h over target
-// i's candidate list cand(i).  A pair term of target i depends on x_i and
-// x_j; target i gathers d(term)/dx_i of its own terms, and, for each j in
-// cand(i) with i in cand(j) (i's sorted cell next to j's query cell), the
-// derivative of j's term about i with respect to x_i.  The stale grid makes
-// candidate lists asymmetric: when i is not in cand(j), j never visits i, so
-// i scatters d(its term)/dx_j to j with an atomic add into an "extra" array
-// that a later kernel folds in.  Gathered sums follow the canonical order of
-// the forward passes; only the (rare) scattered terms arrive in a
-// hardware-dependent order.
+// Every backward neighbour pass is a pass policy of pbf_pass.h.  A pair term
+// of target i (j in its candidate list cand(i)) depends on x_i and x_j; its
+// derivatives belong to both.  The stale grid makes candidate lists
+// one-sided (j in cand(i) does not imply i in cand(j)), and two modes
+// collect the terms (see the comment before within_one below):
+//  * scatter: i sweeps cand(i) on the sorted grid, gathers its own terms
+//    and, where i is in cand(j), j's term about i; where it is not, i adds
+//    d(its term)/dx_j to j with float atomics into "extra" arrays that a
+//    later kernel folds in (their order of addition depends on timing);
+//  * gather: i sweeps a grid built on the pass's own positions, which holds
+//    every particle within h, and gathers every term that concerns it; no
+//    atomics, every sum in a fixed order.
 #include <utility>
 
 #include "pbf_adjoint.h"

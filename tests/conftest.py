@@ -13,7 +13,7 @@ for p in (ROOT, PKG):
 
 
 def pytest_configure(config):
-    config.addinivalue_line("markers", "gpu: needs a HIP device (run with -m gpu on the GPU box)")
+    config.addinivalue_line("markers", "gpu: needs a HIP device (an MI355X; select with -m gpu)")
     config.addinivalue_line("markers", "slow: CPU test that takes tens of seconds")
 
 

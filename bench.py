@@ -639,7 +639,7 @@ def main():
         shapes = eager.sim.get_shapes()
         if not args.no_tune:
             cur = eager.current()
-            shapes, _ = eager.sim.autotune(cur[0], cur[1], eager.iid, n, reps=3, stream=stream)
+            shapes, _ = eager.sim.autotune(cur[0], cur[1], eager.iid, n, reps=5, stream=stream)
         eager.sim.set_timing(True)
         eager.sim.get_timing()  # reset
         ms_eager = timed(eager, args.steps, stream)
@@ -715,7 +715,7 @@ def main():
             bshapes = br.sim.get_shapes()
             if not args.no_tune:
                 cur = br.current()
-                bshapes, _ = br.sim.autotune(cur[0], cur[1], br.iid, br.n, reps=3, stream=stream)
+                bshapes, _ = br.sim.autotune(cur[0], cur[1], br.iid, br.n, reps=5, stream=stream)
             if not args.no_graph:
                 br.sim.set_graph(True)
             ms_b = timed(br, args.steps, stream)
